@@ -1,0 +1,557 @@
+// C ABI: contexts, state upload, forward launch sequencing, hipGraph plans.
+#include "hddm_lb.h"
+#include "errors.h"
+#include "kernels.h"
+#include "layout_build.h"
+#include "context.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <vector>
+
+namespace hddm_lb {
+
+char* error_buffer()
+{
+	static thread_local char buf[1024] = "no error";
+	return buf;
+}
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line)
+{
+	return set_error(HDDM_LB_ERROR_HIP, "%s: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+}
+
+void* dev_alloc(size_t bytes, int& rc)
+{
+	void* p = nullptr;
+	hipError_t e = hipMalloc(&p, bytes ? bytes : 16);
+	if (e != hipSuccess)
+	{
+		(void)hipGetLastError();
+		rc = set_error(HDDM_LB_ERROR_OUT_OF_MEMORY, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+		return nullptr;
+	}
+	rc = HDDM_LB_SUCCESS;
+	return p;
+}
+
+void dev_free(void* p)
+{
+	if (p) (void)hipFree(p);
+}
+
+StateSlot::~StateSlot() { release(); }
+
+Scratch::~Scratch() { release(); }
+
+void Scratch::release()
+{
+	for (void* p : { (void*)bkey, (void*)perm, (void*)hist, (void*)cursor, (void*)partial, (void*)plans, (void*)hdr })
+		dev_free(p);
+	bkey = perm = hist = cursor = nullptr;
+	partial = nullptr;
+	plans = nullptr;
+	hdr = nullptr;
+	cap_bkey = cap_perm = cap_hist = cap_cursor = cap_partial = cap_plans = cap_hdr = 0;
+}
+
+void StateSlot::release()
+{
+	for (void* p : bufs) dev_free(p);
+	bufs.clear();
+	loaded = false;
+}
+
+DeviceGuard::DeviceGuard(int dev) : prev(-1)
+{
+	if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+	if (prev != dev) (void)hipSetDevice(dev);
+}
+
+DeviceGuard::~DeviceGuard()
+{
+	int cur = -1;
+	if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+}
+
+} // namespace hddm_lb
+
+using namespace hddm_lb;
+
+#define HIP_TRY(expr) do { hipError_t e_ = (expr); if (e_ != hipSuccess) return hip_fail(e_, #expr, __FILE__, __LINE__); } while (0)
+
+hddm_lb_ctx::~hddm_lb_ctx()
+{
+	DeviceGuard g(device);
+	states.clear();
+	scratch.release();
+	for (void* p : { (void*)d_states, (void*)d_cost_order, (void*)d_state_ok,
+		(void*)d_xstage, (void*)d_ystage, (void*)d_sidstage, (void*)d_wbuf })
+		dev_free(p);
+	if (ev0) (void)hipEventDestroy(ev0);
+	if (ev1) (void)hipEventDestroy(ev1);
+	if (own_stream) (void)hipStreamDestroy(own_stream);
+}
+
+extern "C" const char* hddm_lb_last_error(void) { return error_buffer(); }
+
+extern "C" int hddm_lb_device_count(int* count)
+{
+	if (!count) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "device_count: count is NULL");
+	int n = 0;
+	hipError_t e = hipGetDeviceCount(&n);
+	if (e != hipSuccess) { (void)hipGetLastError(); n = 0; }
+	*count = n;
+	return HDDM_LB_SUCCESS;
+}
+
+extern "C" int hddm_lb_create(int device, hddm_lb_ctx** out)
+{
+	if (!out) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "create: ctx is NULL");
+	*out = nullptr;
+	int n = 0;
+	hddm_lb_device_count(&n);
+	if (n == 0) return set_error(HDDM_LB_ERROR_NO_DEVICE, "create: no HIP device is visible");
+	if (device < 0 || device >= n)
+		return set_error(HDDM_LB_ERROR_INVALID_VALUE, "create: device %d out of range [0, %d)", device, n);
+	DeviceGuard g(device);
+	std::unique_ptr<hddm_lb_ctx> ctx(new hddm_lb_ctx);
+	ctx->device = device;
+	hipDeviceProp_t prop;
+	HIP_TRY(hipGetDeviceProperties(&prop, device));
+	ctx->num_cus = prop.multiProcessorCount;
+	ctx->max_lds = (size_t)prop.maxSharedMemoryPerMultiProcessor;
+	HIP_TRY(hipEventCreate(&ctx->ev0));
+	HIP_TRY(hipEventCreate(&ctx->ev1));
+	*out = ctx.release();
+	return HDDM_LB_SUCCESS;
+}
+
+extern "C" int hddm_lb_destroy(hddm_lb_ctx* ctx)
+{
+	delete ctx;
+	return HDDM_LB_SUCCESS;
+}
+
+extern "C" int hddm_lb_set_option(hddm_lb_ctx* ctx, int option, int64_t value)
+{
+	if (!ctx) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "set_option: ctx is NULL");
+	switch (option)
+	{
+	case HDDM_LB_OPT_AUTOTUNE: ctx->opt_autotune = value != 0; break;
+	case HDDM_LB_OPT_SEGMENT_NODES:
+		if (value < 0) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "set_option: segment nodes must be >= 0");
+		ctx->opt_seg_nodes = value; ctx->tables_dirty = true; ctx->generation++; break;
+	case HDDM_LB_OPT_WAVES_PER_WG:
+		if (!(value == 0 || value == 1 || value == 2 || value == 4))
+			return set_error(HDDM_LB_ERROR_INVALID_VALUE, "set_option: waves per workgroup must be 0, 1, 2 or 4");
+		ctx->opt_waves = (int)value; break;
+	case HDDM_LB_OPT_SORT_POINTS: ctx->opt_sort = value != 0; break;
+	case HDDM_LB_OPT_TIMING: ctx->opt_timing = value != 0; break;
+	case HDDM_LB_OPT_SLAB_TILES:
+		if (!(value == 0 || value == 1 || value == 2))
+			return set_error(HDDM_LB_ERROR_INVALID_VALUE, "set_option: slab tiles must be 0, 1 or 2");
+		ctx->opt_slab = (int)value; break;
+	case HDDM_LB_OPT_FLUSH_LOG2:
+		if (value < 0 || value > 20) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "set_option: flush log2 in [0, 20]");
+		ctx->opt_wlog = (int)value; break;
+	default:
+		return set_error(HDDM_LB_ERROR_INVALID_VALUE, "set_option: unknown option %d", option);
+	}
+	return HDDM_LB_SUCCESS;
+}
+
+extern "C" int hddm_lb_upload_state(hddm_lb_ctx* ctx, int istate, int dim, int nno, int dofs,
+	const int32_t* index_i, const int32_t* index_j, int64_t ld_index,
+	const double* surplus, int64_t ld_surplus)
+{
+	if (!ctx) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "upload_state: ctx is NULL");
+	if (istate < 0 || istate > (1 << 14))
+		return set_error(HDDM_LB_ERROR_INVALID_VALUE, "upload_state: state %d out of range [0, 16384]", istate);
+	int loaded_others = 0;
+	for (size_t s = 0; s < ctx->states.size(); s++)
+		if ((int)s != istate && ctx->states[s] && ctx->states[s]->loaded) loaded_others++;
+	if (loaded_others && dim != ctx->dim)
+		return set_error(HDDM_LB_ERROR_INVALID_VALUE,
+			"upload_state: # of dimensions (%d) mismatches another state (%d)", dim, ctx->dim);
+	if ((size_t)dim * kXRowBytes > 160 * 1024)
+		return set_error(HDDM_LB_ERROR_UNSUPPORTED, "upload_state: dim %d exceeds the LDS stage (at most 320)", dim);
+
+	HostLayout L;
+	int rc = build_layout(dim, nno, dofs, index_i, index_j, ld_index, surplus, ld_surplus, L);
+	if (rc) return rc;
+
+	DeviceGuard g(ctx->device);
+	std::unique_ptr<StateSlot> slot(new StateSlot);
+	auto up = [&](const void* src, size_t bytes) -> void* {
+		if (rc) return nullptr;
+		void* p = dev_alloc(bytes, rc);
+		if (!p) return nullptr;
+		slot->bufs.push_back(p);
+		if (bytes)
+		{
+			hipError_t e = hipMemcpy(p, src, bytes, hipMemcpyHostToDevice);
+			if (e != hipSuccess) rc = hip_fail(e, "hipMemcpy(upload)", __FILE__, __LINE__);
+		}
+		return p;
+	};
+	StateDesc& d = slot->desc;
+	std::memset(&d, 0, sizeof(d));
+	d.dim = dim; d.nno = nno; d.dofs = dofs; d.dofs_pad = L.dofs_pad;
+	d.fmax = L.fmax; d.rec_words = L.rec_words; d.nchunks = L.nchunks;
+	d.rec = (const uint32_t*)up(L.rec.data(), L.rec.size() * sizeof(uint32_t));
+	d.fac = (const Factor*)up(L.fac.data(), L.fac.size() * sizeof(Factor));
+	d.s64 = (const double*)up(L.s64.data(), L.s64.size() * sizeof(double));
+	d.s16 = (const uint16_t*)up(L.s16.data(), L.s16.size() * sizeof(uint16_t));
+	d.sbf = (const uint16_t*)up(L.sbf.data(), L.sbf.size() * sizeof(uint16_t));
+	d.s32 = (const float*)up(L.s32.data(), L.s32.size() * sizeof(float));
+	d.iscale = (const double*)up(L.iscale.data(), L.iscale.size() * sizeof(double));
+	d.order = (const int32_t*)up(L.order.data(), L.order.size() * sizeof(int32_t));
+	d.lomask = (const uint64_t*)up(L.lomask.data(), L.lomask.size() * sizeof(uint64_t));
+	d.himask = (const uint64_t*)up(L.himask.data(), L.himask.size() * sizeof(uint64_t));
+	if (rc) return rc;
+	slot->loaded = true;
+
+	if ((size_t)istate >= ctx->states.size()) ctx->states.resize(istate + 1);
+	ctx->states[istate] = std::move(slot);
+	ctx->dim = dim;
+	ctx->tables_dirty = true;
+	ctx->generation++;
+	ctx->tuned.clear();
+	return HDDM_LB_SUCCESS;
+}
+
+extern "C" int hddm_lb_state_info(hddm_lb_ctx* ctx, int istate, int* dim, int* nno, int* dofs, int* max_factors)
+{
+	if (!ctx) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "state_info: ctx is NULL");
+	if (istate < 0 || (size_t)istate >= ctx->states.size() || !ctx->states[istate] || !ctx->states[istate]->loaded)
+		return set_error(HDDM_LB_ERROR_STATE, "state_info: state %d is not loaded", istate);
+	const StateDesc& d = ctx->states[istate]->desc;
+	if (dim) *dim = d.dim;
+	if (nno) *nno = d.nno;
+	if (dofs) *dofs = d.dofs;
+	if (max_factors) *max_factors = d.fmax;
+	return HDDM_LB_SUCCESS;
+}
+
+extern "C" int hddm_lb_surplus_offset(hddm_lb_ctx* ctx, int istate, int64_t* offset, int64_t* total)
+{
+	if (!ctx) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "surplus_offset: ctx is NULL");
+	int64_t off = 0, tot = 0;
+	bool found = false;
+	for (size_t s = 0; s < ctx->states.size(); s++)
+	{
+		if ((int)s == istate) { off = tot; found = ctx->states[s] && ctx->states[s]->loaded; }
+		if (ctx->states[s] && ctx->states[s]->loaded)
+			tot += (int64_t)ctx->states[s]->desc.nno * ctx->states[s]->desc.dofs;
+	}
+	if (istate >= 0 && !found) return set_error(HDDM_LB_ERROR_STATE, "surplus_offset: state %d is not loaded", istate);
+	if (offset) *offset = off;
+	if (total) *total = tot;
+	return HDDM_LB_SUCCESS;
+}
+
+namespace hddm_lb {
+
+// Uploads the per-state descriptor table when states or segmenting changed.
+int prepare_tables(hddm_lb_ctx* ctx)
+{
+	const int ns = (int)ctx->states.size();
+	if (ns == 0) return set_error(HDDM_LB_ERROR_STATE, "no state is loaded");
+	if (!ctx->tables_dirty && ctx->tables_n == ns) return HDDM_LB_SUCCESS;
+	int nloaded = 0;
+	for (auto& s : ctx->states) if (s && s->loaded) nloaded++;
+	if (!nloaded) return set_error(HDDM_LB_ERROR_STATE, "no state is loaded");
+
+	std::vector<StateDesc> descs(ns);
+	std::vector<uint8_t> ok(ns, 0);
+	std::vector<std::pair<double, int> > cost;
+	ctx->dt_needed = 1;
+	ctx->fmax_all = 1;
+	ctx->max_nseg_dp = 0;
+	for (int s = 0; s < ns; s++)
+	{
+		std::memset(&descs[s], 0, sizeof(StateDesc));
+		if (!ctx->states[s] || !ctx->states[s]->loaded) { descs[s].nseg = 1; descs[s].seg_len = 16; descs[s].dofs_pad = 32; continue; }
+		StateDesc d = ctx->states[s]->desc;
+		int64_t seg = ctx->opt_seg_nodes;
+		if (seg <= 0) seg = d.nno;
+		seg = (seg + kChunk - 1) / kChunk * kChunk;
+		if (seg > d.nno) seg = (d.nno + kChunk - 1) / kChunk * kChunk;
+		d.seg_len = (int)seg;
+		d.nseg = (int)((d.nno + seg - 1) / seg);
+		descs[s] = d;
+		ok[s] = 1;
+		ctx->dt_needed = std::max(ctx->dt_needed, d.dofs_pad / 32);
+		ctx->fmax_all = std::max(ctx->fmax_all, d.fmax);
+		if (d.nseg > 1) ctx->max_nseg_dp = std::max<int64_t>(ctx->max_nseg_dp, (int64_t)d.nseg * d.dofs_pad);
+		cost.push_back(std::make_pair(-(double)d.seg_len * d.dofs_pad, s));
+	}
+	std::stable_sort(cost.begin(), cost.end());
+	std::vector<int32_t> order;
+	for (auto& c : cost) order.push_back(c.second);
+	for (int s = 0; s < ns; s++) if (!ok[s]) order.push_back(s);
+
+	if (ctx->tables_cap < ns)
+	{
+		for (void* p : { (void*)ctx->d_states, (void*)ctx->d_cost_order, (void*)ctx->d_state_ok })
+			dev_free(p);
+		int rc = 0;
+		ctx->d_states = (StateDesc*)dev_alloc(sizeof(StateDesc) * ns, rc); if (rc) return rc;
+		ctx->d_cost_order = (int32_t*)dev_alloc(sizeof(int32_t) * ns, rc); if (rc) return rc;
+		ctx->d_state_ok = (uint8_t*)dev_alloc(ns, rc); if (rc) return rc;
+		ctx->tables_cap = ns;
+	}
+	HIP_TRY(hipMemcpy(ctx->d_states, descs.data(), sizeof(StateDesc) * ns, hipMemcpyHostToDevice));
+	HIP_TRY(hipMemcpy(ctx->d_cost_order, order.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice));
+	HIP_TRY(hipMemcpy(ctx->d_state_ok, ok.data(), ns, hipMemcpyHostToDevice));
+	ctx->tables_n = ns;
+	ctx->tables_dirty = false;
+	return HDDM_LB_SUCCESS;
+}
+
+template <typename T>
+static int grow(T*& p, size_t& cap, size_t need)
+{
+	if (need <= cap && p) return HDDM_LB_SUCCESS;
+	dev_free(p);
+	p = nullptr;
+	cap = 0;
+	int rc = 0;
+	size_t n = std::max<size_t>(need, 1);
+	p = (T*)dev_alloc(sizeof(T) * n, rc);
+	if (rc) return rc;
+	cap = n;
+	return HDDM_LB_SUCCESS;
+}
+
+int bucket_bits(int64_t npoints, int nstates, int sort)
+{
+	if (!sort) return 0;
+	int64_t per = npoints / std::max(1, nstates) / 16;
+	int b = 0;
+	while (b < 16 && ((int64_t)1 << (b + 1)) <= per) b++;
+	while (b > 0 && ((int64_t)(nstates + 1) << b) > (1 << 18)) b--;
+	return b;
+}
+
+// Allocates scratch for npoints; must run outside graph capture.
+int prepare_scratch(hddm_lb_ctx* ctx, Scratch& sc, int64_t npoints)
+{
+	int rc = prepare_tables(ctx);
+	if (rc) return rc;
+	const int ns = ctx->tables_n;
+	const int bb = bucket_bits(npoints, ns, ctx->opt_sort);
+	const size_t nb = (size_t)(ns + 1) << bb;
+	if ((rc = grow(sc.bkey, sc.cap_bkey, (size_t)npoints))) return rc;
+	if ((rc = grow(sc.perm, sc.cap_perm, (size_t)npoints))) return rc;
+	if ((rc = grow(sc.hist, sc.cap_hist, nb))) return rc;
+	if ((rc = grow(sc.cursor, sc.cap_cursor, nb))) return rc;
+	if ((rc = grow(sc.plans, sc.cap_plans, (size_t)ns))) return rc;
+	if ((rc = grow(sc.hdr, sc.cap_hdr, 1))) return rc;
+	if (ctx->max_nseg_dp > 0)
+		if ((rc = grow(sc.partial, sc.cap_partial, (size_t)npoints * (size_t)ctx->max_nseg_dp))) return rc;
+	return HDDM_LB_SUCCESS;
+}
+
+TuneConfig default_config(hddm_lb_ctx* ctx, int prec)
+{
+	TuneConfig c;
+	c.waves = ctx->opt_waves ? ctx->opt_waves : 4;
+	while (c.waves > 1 && (size_t)c.waves * ctx->dim * kXRowBytes > 64 * 1024) c.waves /= 2;
+	int slab = ctx->opt_slab;
+	if (!slab) slab = (prec == HDDM_LB_FP64) ? std::min(2, ctx->dt_needed) : 1;
+	c.slab = slab;
+	c.wlog = ctx->opt_wlog >= 0 ? ctx->opt_wlog : 3;
+	return c;
+}
+
+// Enqueues the whole forward sequence on st. No allocation, no host sync.
+int enqueue_forward(hddm_lb_ctx* ctx, Scratch& sc, int64_t npoints, const double* x, int64_t ldx,
+	const int32_t* sid, double* y, int64_t ldy, int prec, hipStream_t st, const TuneConfig& cfg, bool timed)
+{
+	const int ns = ctx->tables_n;
+	const int bb = bucket_bits(npoints, ns, ctx->opt_sort);
+	const int64_t nb = (int64_t)(ns + 1) << bb;
+	HIP_TRY(launch_fill_u32(sc.hist, nb, 0u, st));
+	HIP_TRY(launch_bucket_key(npoints, x, ldx, sid, ctx->d_state_ok, ns, ctx->dim, bb, sc.bkey, sc.hist, st));
+	HIP_TRY(launch_scan_plan(sc.hist, sc.cursor, nb, bb, ns, ctx->d_states, ctx->d_cost_order,
+		sc.plans, sc.hdr, st));
+	HIP_TRY(launch_scatter(npoints, sc.bkey, sc.cursor, sc.perm, st));
+
+	const void* kern = sweep_kernel_ptr(prec, cfg.slab, ctx->fmax_all);
+	if (!kern) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "forward: unknown precision %d", prec);
+	const size_t lds = (size_t)cfg.waves * ctx->dim * kXRowBytes;
+	if (lds > 64 * 1024)
+		HIP_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+	int per_cu = 0;
+	HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, cfg.waves * kWave, lds));
+	if (per_cu < 1)
+		return set_error(HDDM_LB_ERROR_UNSUPPORTED, "forward: sweep kernel does not fit a CU (dim %d, %d waves)", ctx->dim, cfg.waves);
+	const int grid = per_cu * ctx->num_cus;
+
+	SweepArgs a;
+	std::memset(&a, 0, sizeof(a));
+	a.states = ctx->d_states; a.plans = sc.plans; a.cost_order = ctx->d_cost_order; a.hdr = sc.hdr;
+	a.perm = sc.perm; a.x = x; a.ldx = ldx; a.y = y; a.ldy = ldy; a.partial = sc.partial;
+	a.nstates = ns; a.dim = ctx->dim; a.wlog = cfg.wlog;
+	const int nslab = (ctx->dt_needed + cfg.slab - 1) / cfg.slab;
+	if (timed) HIP_TRY(hipEventRecord(ctx->ev0, st));
+	for (int sl = 0; sl < nslab; sl++)
+	{
+		a.dt_base = sl * cfg.slab;
+		if (sl > 0) HIP_TRY(launch_reset_counter(sc.hdr, st));
+		HIP_TRY(launch_sweep(kern, grid, cfg.waves, lds, a, st));
+	}
+	if (timed) HIP_TRY(hipEventRecord(ctx->ev1, st));
+	if (ctx->max_nseg_dp > 0)
+		HIP_TRY(launch_reduce(ctx->d_states, sc.plans, sc.perm, sc.partial, y, ldy, ns, st));
+	return HDDM_LB_SUCCESS;
+}
+
+int check_forward_args(hddm_lb_ctx* ctx, int64_t npoints, const double* x, int64_t ldx,
+	double* y, int64_t ldy, int prec)
+{
+	if (!ctx) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "forward: ctx is NULL");
+	if (npoints < 0 || npoints > ((int64_t)1 << 31) - 1)
+		return set_error(HDDM_LB_ERROR_INVALID_VALUE, "forward: npoints %lld out of range", (long long)npoints);
+	if (prec < HDDM_LB_FP64 || prec > HDDM_LB_F32)
+		return set_error(HDDM_LB_ERROR_INVALID_VALUE, "forward: unknown precision %d", prec);
+	if (ctx->dim < 1) return set_error(HDDM_LB_ERROR_STATE, "forward: no state is loaded");
+	if (npoints && (!x || !y)) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "forward: x and y must not be NULL");
+	if (ldx < ctx->dim) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "forward: ldx %lld < dim %d", (long long)ldx, ctx->dim);
+	int maxdofs = 0;
+	for (auto& s : ctx->states) if (s && s->loaded) maxdofs = std::max(maxdofs, s->desc.dofs);
+	if (ldy < maxdofs) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "forward: ldy %lld < dofs %d", (long long)ldy, maxdofs);
+	return HDDM_LB_SUCCESS;
+}
+
+} // namespace hddm_lb
+
+extern "C" int hddm_lb_forward(hddm_lb_ctx* ctx, int64_t npoints,
+	const double* x, int64_t ldx, const int32_t* state_ids,
+	double* y, int64_t ldy, int precision, void* stream)
+{
+	int rc = check_forward_args(ctx, npoints, x, ldx, y, ldy, precision);
+	if (rc) return rc;
+	if (npoints == 0) return HDDM_LB_SUCCESS;
+	DeviceGuard g(ctx->device);
+	if ((rc = prepare_scratch(ctx, ctx->scratch, npoints))) return rc;
+	TuneConfig cfg = default_config(ctx, precision);
+	if (ctx->opt_autotune && (rc = autotune(ctx, npoints, x, ldx, state_ids, y, ldy, precision, (hipStream_t)stream, cfg)))
+		return rc;
+	ctx->last_timed = ctx->opt_timing;
+	return enqueue_forward(ctx, ctx->scratch, npoints, x, ldx, state_ids, y, ldy, precision, (hipStream_t)stream, cfg, ctx->opt_timing);
+}
+
+extern "C" double hddm_lb_last_sweep_ms(hddm_lb_ctx* ctx)
+{
+	if (!ctx || !ctx->last_timed) return -1.0;
+	DeviceGuard g(ctx->device);
+	if (hipEventSynchronize(ctx->ev1) != hipSuccess) { (void)hipGetLastError(); return -1.0; }
+	float ms = -1.0f;
+	if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) != hipSuccess) { (void)hipGetLastError(); return -1.0; }
+	return (double)ms;
+}
+
+extern "C" int hddm_lb_forward_host(hddm_lb_ctx* ctx, int64_t npoints,
+	const double* x, int64_t ldx, const int32_t* state_ids,
+	double* y, int64_t ldy, int precision)
+{
+	int rc = check_forward_args(ctx, npoints, x, ldx, y, ldy, precision);
+	if (rc) return rc;
+	if (npoints == 0) return HDDM_LB_SUCCESS;
+	DeviceGuard g(ctx->device);
+	if (!ctx->own_stream) HIP_TRY(hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking));
+	if ((rc = grow(ctx->d_xstage, ctx->cap_xstage, (size_t)npoints * ldx))) return rc;
+	if ((rc = grow(ctx->d_ystage, ctx->cap_ystage, (size_t)npoints * ldy))) return rc;
+	if (state_ids && (rc = grow(ctx->d_sidstage, ctx->cap_sidstage, (size_t)npoints))) return rc;
+	hipStream_t st = ctx->own_stream;
+	HIP_TRY(hipMemcpyAsync(ctx->d_xstage, x, sizeof(double) * npoints * ldx, hipMemcpyHostToDevice, st));
+	if (state_ids) HIP_TRY(hipMemcpyAsync(ctx->d_sidstage, state_ids, sizeof(int32_t) * npoints, hipMemcpyHostToDevice, st));
+	// rows of points with an invalid state stay as the caller passed them
+	HIP_TRY(hipMemcpyAsync(ctx->d_ystage, y, sizeof(double) * npoints * ldy, hipMemcpyHostToDevice, st));
+	rc = hddm_lb_forward(ctx, npoints, ctx->d_xstage, ldx, state_ids ? ctx->d_sidstage : nullptr,
+		ctx->d_ystage, ldy, precision, st);
+	if (rc) return rc;
+	HIP_TRY(hipMemcpyAsync(y, ctx->d_ystage, sizeof(double) * npoints * ldy, hipMemcpyDeviceToHost, st));
+	HIP_TRY(hipStreamSynchronize(st));
+	return HDDM_LB_SUCCESS;
+}
+
+// ---- hipGraph plans ----
+
+struct hddm_lb_plan
+{
+	hddm_lb_ctx* ctx;
+	uint64_t generation;
+	hddm_lb::Scratch scratch;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	~hddm_lb_plan()
+	{
+		if (exec) (void)hipGraphExecDestroy(exec);
+		if (graph) (void)hipGraphDestroy(graph);
+	}
+};
+
+extern "C" int hddm_lb_plan_create(hddm_lb_ctx* ctx, int64_t npoints,
+	const double* x, int64_t ldx, const int32_t* state_ids,
+	double* y, int64_t ldy, int precision, void* stream, hddm_lb_plan** out)
+{
+	if (!out) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "plan_create: plan is NULL");
+	*out = nullptr;
+	int rc = check_forward_args(ctx, npoints, x, ldx, y, ldy, precision);
+	if (rc) return rc;
+	if (npoints < 1) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "plan_create: npoints must be >= 1");
+	DeviceGuard g(ctx->device);
+	if ((rc = prepare_scratch(ctx, ctx->scratch, npoints))) return rc;
+	TuneConfig cfg = default_config(ctx, precision);
+	if (ctx->opt_autotune && (rc = autotune(ctx, npoints, x, ldx, state_ids, y, ldy, precision, (hipStream_t)stream, cfg)))
+		return rc;
+	std::unique_ptr<hddm_lb_plan> plan(new hddm_lb_plan);
+	plan->ctx = ctx;
+	plan->generation = ctx->generation;
+	if ((rc = prepare_scratch(ctx, plan->scratch, npoints))) return rc;
+	HIP_TRY(hipDeviceSynchronize());
+	hipStream_t cap;
+	HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+	hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+	if (e != hipSuccess) { (void)hipStreamDestroy(cap); return hip_fail(e, "hipStreamBeginCapture", __FILE__, __LINE__); }
+	rc = enqueue_forward(ctx, plan->scratch, npoints, x, ldx, state_ids, y, ldy, precision, cap, cfg, false);
+	hipGraph_t graph = nullptr;
+	e = hipStreamEndCapture(cap, &graph);
+	(void)hipStreamDestroy(cap);
+	if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+	if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+	plan->graph = graph;
+	HIP_TRY(hipGraphInstantiate(&plan->exec, graph, nullptr, nullptr, 0));
+	*out = plan.release();
+	return HDDM_LB_SUCCESS;
+}
+
+extern "C" int hddm_lb_plan_launch(hddm_lb_plan* plan, void* stream)
+{
+	if (!plan) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "plan_launch: plan is NULL");
+	if (plan->generation != plan->ctx->generation)
+		return set_error(HDDM_LB_ERROR_STATE, "plan_launch: the context's states changed after the plan was created");
+	DeviceGuard g(plan->ctx->device);
+	HIP_TRY(hipGraphLaunch(plan->exec, (hipStream_t)stream));
+	return HDDM_LB_SUCCESS;
+}
+
+extern "C" int hddm_lb_plan_destroy(hddm_lb_plan* plan)
+{
+	if (plan)
+	{
+		DeviceGuard g(plan->ctx->device);
+		delete plan;
+	}
+	return HDDM_LB_SUCCESS;
+}

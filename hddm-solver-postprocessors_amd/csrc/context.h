@@ -1,0 +1,120 @@
+// Host-side context of one device.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <tuple>
+#include <vector>
+
+#include "hddm_lb.h"
+#include "layout.h"
+
+namespace hddm_lb {
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+void* dev_alloc(size_t bytes, int& rc);
+void dev_free(void* p);
+
+struct DeviceGuard
+{
+	int prev;
+	explicit DeviceGuard(int dev);
+	~DeviceGuard();
+};
+
+struct StateSlot
+{
+	bool loaded = false;
+	StateDesc desc;              // device pointers, seg fields filled per call
+	std::vector<void*> bufs;     // device allocations owned by the slot
+	~StateSlot();
+	void release();
+};
+
+// Per-call device scratch of the forward sequence. The context owns one;
+// every hipGraph plan owns its own, so replays never alias later eager calls.
+struct Scratch
+{
+	uint32_t* bkey = nullptr; size_t cap_bkey = 0;
+	uint32_t* perm = nullptr; size_t cap_perm = 0;
+	uint32_t* hist = nullptr; size_t cap_hist = 0;
+	uint32_t* cursor = nullptr; size_t cap_cursor = 0;
+	double* partial = nullptr; size_t cap_partial = 0;
+	StatePlan* plans = nullptr; size_t cap_plans = 0;
+	PlanHeader* hdr = nullptr; size_t cap_hdr = 0;
+	~Scratch();
+	void release();
+};
+
+// Launch shape of the sweep, chosen by default or by the autotuner.
+struct TuneConfig
+{
+	int waves = 4;     // waves per workgroup
+	int slab = 1;      // dof tiles of 32 per sweep pass
+	int wlog = 3;      // log2 flush window (chunks)
+	int64_t seg = -1;  // node segment length used while tuning (-1: context option)
+};
+
+} // namespace hddm_lb
+
+struct hddm_lb_ctx
+{
+	int device = 0;
+	int dim = -1;
+	int num_cus = 0;
+	size_t max_lds = 0;
+	std::vector<std::unique_ptr<hddm_lb::StateSlot> > states;
+
+	// options
+	bool opt_autotune = false;
+	int64_t opt_seg_nodes = 0;
+	int opt_waves = 0;
+	bool opt_sort = true;
+	bool opt_timing = false;
+	int opt_slab = 0;
+	int opt_wlog = -1;
+
+	// device tables
+	bool tables_dirty = true;
+	int tables_n = 0, tables_cap = 0;
+	int dt_needed = 1, fmax_all = 1;
+	int64_t max_nseg_dp = 0;
+	hddm_lb::StateDesc* d_states = nullptr;
+	int32_t* d_cost_order = nullptr;
+	uint8_t* d_state_ok = nullptr;
+	uint64_t generation = 0;     // bumped whenever states or tables change
+
+	// per-call scratch of eager forwards
+	hddm_lb::Scratch scratch;
+	double* d_xstage = nullptr; size_t cap_xstage = 0;
+	double* d_ystage = nullptr; size_t cap_ystage = 0;
+	int32_t* d_sidstage = nullptr; size_t cap_sidstage = 0;
+	hipStream_t own_stream = nullptr;
+
+	// adjoint scratch
+	float* d_wbuf = nullptr; size_t cap_wbuf = 0;
+
+	// timing of the sweep
+	hipEvent_t ev0 = nullptr, ev1 = nullptr;
+	bool last_timed = false;
+
+	// autotuner cache: (prec, dim, total nodes, max dofs, log2 npoints) -> config
+	std::map<std::tuple<int, int, int64_t, int, int>, hddm_lb::TuneConfig> tuned;
+
+	~hddm_lb_ctx();
+};
+
+namespace hddm_lb {
+
+int prepare_tables(hddm_lb_ctx* ctx);
+int prepare_scratch(hddm_lb_ctx* ctx, Scratch& sc, int64_t npoints);
+TuneConfig default_config(hddm_lb_ctx* ctx, int prec);
+int enqueue_forward(hddm_lb_ctx* ctx, Scratch& sc, int64_t npoints, const double* x, int64_t ldx,
+	const int32_t* sid, double* y, int64_t ldy, int prec, hipStream_t st, const TuneConfig& cfg, bool timed);
+int autotune(hddm_lb_ctx* ctx, int64_t npoints, const double* x, int64_t ldx, const int32_t* sid,
+	double* y, int64_t ldy, int prec, hipStream_t st, TuneConfig& cfg);
+int bucket_bits(int64_t npoints, int nstates, int sort);
+
+} // namespace hddm_lb

@@ -1,0 +1,53 @@
+// Device helpers shared by the forward and adjoint kernels (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "layout.h"
+
+namespace hddm_lb {
+
+// Loads through the constant address space: a wave-uniform address becomes a
+// scalar (SMEM) load, which keeps node records and surplus rows out of VGPRs.
+template <typename T>
+__device__ __forceinline__ T ldc(const T* p)
+{
+	return *(const __attribute__((address_space(4))) T*)p;
+}
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ uint64_t uniform64(uint64_t v)
+{
+	uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+	uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+	return ((uint64_t)hi << 32) | lo;
+}
+
+// True if any lane of the wave has pred set.
+__device__ __forceinline__ bool wave_any(bool pred)
+{
+	return __builtin_amdgcn_ballot_w64(pred) != 0;
+}
+
+// Top `bits` bits of the interleaved dyadic expansion of x: bit order
+// (level 1, dim 0), (level 1, dim 1), ..., (level 2, dim 0), ...
+// Bit (l, d) is the l-th binary digit of x_d, i.e. which half, quarter, ...
+// of [0, 1] holds x_d. Points adjacent in this order share supports.
+__device__ __forceinline__ uint64_t morton_key(const double* x, int dim, int bits)
+{
+	uint64_t key = 0;
+	int got = 0;
+	for (int lev = 1; got < bits && lev <= 30; lev++)
+		for (int d = 0; d < dim && got < bits; d++, got++)
+		{
+			double v = x[d];
+			v = v < 0.0 ? 0.0 : (v >= 1.0 ? 0.9999999999999999 : v);
+			uint64_t digit = (uint64_t)(int64_t)(v * (double)(1u << lev)) & 1u;
+			key = (key << 1) | digit;
+		}
+	return key;
+}
+
+} // namespace hddm_lb

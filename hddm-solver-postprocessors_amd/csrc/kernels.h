@@ -1,0 +1,45 @@
+// Host-visible declarations of the HIP kernels and their launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hddm_lb.h"
+#include "layout.h"
+
+namespace hddm_lb {
+
+struct SweepArgs
+{
+	const StateDesc* states;
+	const StatePlan* plans;
+	const int32_t* cost_order;
+	PlanHeader* hdr;
+	const uint32_t* perm;      // sorted position -> point
+	const double* x;
+	int64_t ldx;
+	double* y;
+	int64_t ldy;
+	double* partial;           // node-segment partials of split states
+	int nstates;
+	int dim;
+	int wlog;                  // log2 of the FP64 flush window, in 16-node chunks
+	int dt_base;               // first dof tile of 32 handled by this pass
+};
+
+// Kernel symbol of the sweep instantiation for (precision, dof tiles of 32, max factors).
+const void* sweep_kernel_ptr(int prec, int dt, int fmax);
+int fmax_bucket(int fmax);
+
+hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st);
+hipError_t launch_bucket_key(int64_t n, const double* x, int64_t ldx, const int32_t* sid,
+	const uint8_t* state_ok, int nstates, int dim, int bbits, uint32_t* bkey, uint32_t* hist, hipStream_t st);
+hipError_t launch_scan_plan(const uint32_t* hist, uint32_t* cursor, int64_t nbuckets, int bbits, int nstates,
+	const StateDesc* states, const int32_t* cost_order, StatePlan* plans, PlanHeader* hdr, hipStream_t st);
+hipError_t launch_scatter(int64_t n, const uint32_t* bkey, uint32_t* cursor, uint32_t* perm, hipStream_t st);
+hipError_t launch_reset_counter(PlanHeader* hdr, hipStream_t st);
+hipError_t launch_sweep(const void* kern, int grid, int waves, size_t lds, const SweepArgs& args, hipStream_t st);
+hipError_t launch_reduce(const StateDesc* states, const StatePlan* plans, const uint32_t* perm,
+	const double* partial, double* y, int64_t ldy, int nstates, hipStream_t st);
+
+} // namespace hddm_lb

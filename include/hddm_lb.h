@@ -1,0 +1,185 @@
+/*
+ * hddm_lb.h -- flat C ABI of the MI355X (gfx950) LinearBasis interpolation engine.
+ *
+ * Every entry point takes plain pointers and sizes; no C++ or torch types cross
+ * this boundary. Device-side entry points take device pointers and a
+ * hipStream_t passed as void*, are stream-ordered and return a status code;
+ * nothing aborts the process. hddm_lb_last_error() returns the text of the
+ * last failure on the calling thread.
+ *
+ * The reference interfaces these entry points replace (paths relative to the
+ * reference checkout; LB = LinearBasis):
+ *   - include/interpolator.h:31-33   Interpolator::interpolate(device, data, istate, x, DofPerNode, value)
+ *       -> hddm_lb_forward / hddm_lb_forward_host (one point = npoints 1)
+ *   - include/interpolator.h:35-37   Interpolator::interpolate(device, data, x[], DofPerNode, value[])
+ *       -> hddm_lb_forward with a state id per point
+ *   - LB/gold/src/InterpolateArray.cpp:8-43 and LB/cuda/src/InterpolateArray.cu:239-267
+ *       (the extern "C" kernels the Interpolator dispatches to) -> hddm_lb_forward
+ *   - LB/gold/src/Data.cpp:322-595 DataDense::load (.plt reader)
+ *       -> hddm_lb_plt_read + hddm_lb_upload_state
+ *   - LB/cuda/src/Data.cu:944-948 (setXPS/setChains/setSurplus device upload)
+ *       -> hddm_lb_upload_state
+ *   - LB/cuda/src/Devices.cu:13-47 (GPU count) -> hddm_lb_device_count
+ * The adjoint, the batched point API, the hipGraph plan and the grid generator
+ * have no counterpart in the reference.
+ */
+#ifndef HDDM_LB_H
+#define HDDM_LB_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hddm_lb_ctx hddm_lb_ctx;
+typedef struct hddm_lb_plan hddm_lb_plan;
+typedef struct hddm_lb_grid hddm_lb_grid;
+
+/* status codes */
+#define HDDM_LB_SUCCESS               0
+#define HDDM_LB_ERROR_INVALID_VALUE   1
+#define HDDM_LB_ERROR_NO_DEVICE       2
+#define HDDM_LB_ERROR_OUT_OF_MEMORY   3
+#define HDDM_LB_ERROR_HIP             4
+#define HDDM_LB_ERROR_STATE           5
+#define HDDM_LB_ERROR_IO              6
+#define HDDM_LB_ERROR_FORMAT          7
+#define HDDM_LB_ERROR_UNSUPPORTED     8
+
+/* arithmetic of the contraction (the basis is always formed in FP64) */
+#define HDDM_LB_FP64    0  /* FP64 VALU contraction, FP64 accumulation */
+#define HDDM_LB_F16X2   1  /* MFMA f16: basis hi+lo x scaled surplus hi+lo, 3 products */
+#define HDDM_LB_BF16X3  2  /* MFMA bf16: 3-term splits, 6 products */
+#define HDDM_LB_F32     3  /* MFMA f32: basis f32 x surplus f32 */
+
+/* options for hddm_lb_set_option */
+#define HDDM_LB_OPT_AUTOTUNE      1  /* 1: time tile candidates on first use of a (D, nno, dofs) key */
+#define HDDM_LB_OPT_SEGMENT_NODES 2  /* node-range length of one work item (0: automatic) */
+#define HDDM_LB_OPT_WAVES_PER_WG  3  /* waves per workgroup (1, 2, 4, 8; 0: automatic) */
+#define HDDM_LB_OPT_SORT_POINTS   4  /* 1 (default): spatially sort points before the sweep */
+#define HDDM_LB_OPT_TIMING        5  /* 1: time the sweep kernels with hipEvents (hddm_lb_last_sweep_ms) */
+#define HDDM_LB_OPT_SLAB_TILES    6  /* dof tiles of 32 per sweep pass (1, 2; 0: automatic) */
+#define HDDM_LB_OPT_FLUSH_LOG2    7  /* log2 of the FP32->FP64 flush window in 16-node chunks (-1: default 3) */
+
+/* ---- devices and contexts ---- */
+
+/* Number of visible HIP devices; 0 (and success) when there is none. */
+int hddm_lb_device_count(int* count);
+
+/* One context per device per host thread; allocates nothing until states are uploaded. */
+int hddm_lb_create(int device, hddm_lb_ctx** ctx);
+int hddm_lb_destroy(hddm_lb_ctx* ctx);
+
+/* Text of the last error on this thread (never NULL). */
+const char* hddm_lb_last_error(void);
+
+int hddm_lb_set_option(hddm_lb_ctx* ctx, int option, int64_t value);
+
+/* Upload one state from the reference's dense representation: for node k and
+ * dim d, index_i[k*ld_index + d] is i (0 for level 1, 2^(l-1) otherwise) and
+ * index_j[k*ld_index + d] is j, exactly as the gold Matrix<int> holds them
+ * (LB/gold/src/Data.cpp:474-494); surplus is row-major [nno x ld_surplus]
+ * FP64. Host pointers. All states of a context share dim (the reference's
+ * rule, LB/gold/src/Data.cpp:424-429); dofs may differ per state. Replacing
+ * an uploaded state is allowed. */
+int hddm_lb_upload_state(hddm_lb_ctx* ctx, int istate, int dim, int nno, int dofs,
+	const int32_t* index_i, const int32_t* index_j, int64_t ld_index,
+	const double* surplus, int64_t ld_surplus);
+
+int hddm_lb_state_info(hddm_lb_ctx* ctx, int istate,
+	int* dim, int* nno, int* dofs, int* max_factors);
+
+/* Forward interpolation of npoints points (device pointers):
+ *   x         [npoints x ldx] FP64 coordinates in [0, 1]
+ *   state_ids [npoints] int32 state of each point, or NULL for state 0
+ *   y         [npoints x ldy] FP64; row p receives dofs(state_ids[p]) values
+ * Stream-ordered on `stream` (a hipStream_t, NULL for the default stream). */
+int hddm_lb_forward(hddm_lb_ctx* ctx, int64_t npoints,
+	const double* x, int64_t ldx, const int32_t* state_ids,
+	double* y, int64_t ldy, int precision, void* stream);
+
+/* Same contract with host pointers; synchronous. This is what the legacy
+ * per-point Interpolator::interpolate maps onto. */
+int hddm_lb_forward_host(hddm_lb_ctx* ctx, int64_t npoints,
+	const double* x, int64_t ldx, const int32_t* state_ids,
+	double* y, int64_t ldy, int precision);
+
+/* Adjoint (device pointers): given gbar [npoints x ldg] FP32,
+ *   dx        [npoints x lddx] FP32  = sum_k (gbar . S_k) dB_k/dx
+ *   dsurplus  FP32, per state s at element offset hddm_lb_surplus_offset(s),
+ *             rows in the uploaded node order, row length dofs(s):
+ *             dS_k = sum_p B_k(x_p) gbar_p   (overwritten)
+ * dx or dsurplus may be NULL to skip that output. */
+int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
+	const double* x, int64_t ldx, const int32_t* state_ids,
+	const float* gbar, int64_t ldg, float* dx, int64_t lddx,
+	float* dsurplus, int precision, void* stream);
+
+int hddm_lb_surplus_offset(hddm_lb_ctx* ctx, int istate, int64_t* offset, int64_t* total);
+
+/* ---- hipGraph plans: a forward with fixed pointers and sizes, captured once
+ * and replayed; every launch of the forward sequence is in the graph. ---- */
+int hddm_lb_plan_create(hddm_lb_ctx* ctx, int64_t npoints,
+	const double* x, int64_t ldx, const int32_t* state_ids,
+	double* y, int64_t ldy, int precision, void* stream, hddm_lb_plan** plan);
+int hddm_lb_plan_launch(hddm_lb_plan* plan, void* stream);
+int hddm_lb_plan_destroy(hddm_lb_plan* plan);
+
+/* ---- measurement ---- */
+/* Time in ms between hipEvents recorded on the forward's stream right before
+ * the first and right after the last sweep kernel of the last hddm_lb_forward
+ * on this context, when HDDM_LB_OPT_TIMING is set; -1 otherwise. Waits for
+ * the second event. */
+double hddm_lb_last_sweep_ms(hddm_lb_ctx* ctx);
+
+/* ---- host-side node layout, for inspection without a device ---- */
+typedef struct hddm_lb_layout hddm_lb_layout;
+/* Builds the prefix-trie layout hddm_lb_upload_state would upload. Node record
+ * n (trie position) holds rec_words 32-bit words: word 0 = F | s << 8 (active
+ * factors, factors shared with record n-1), then for depth t < F the factor id
+ * and the first record after the run of records sharing the first t+1 factors. */
+int hddm_lb_layout_build(int dim, int nno, int dofs,
+	const int32_t* index_i, const int32_t* index_j, int64_t ld_index,
+	const double* surplus, int64_t ld_surplus, hddm_lb_layout** layout);
+int hddm_lb_layout_info(const hddm_lb_layout* layout, int* max_factors, int* rec_words,
+	int* nfactors, int* nchunks, int* dofs_pad);
+int hddm_lb_layout_records(const hddm_lb_layout* layout, uint32_t* records);
+int hddm_lb_layout_factors(const hddm_lb_layout* layout, double* i, double* negj, int32_t* dim);
+int hddm_lb_layout_order(const hddm_lb_layout* layout, int32_t* order);
+int hddm_lb_layout_free(hddm_lb_layout* layout);
+
+/* ---- .plt surplus files (LB/gold/src/Data.cpp:13-595) ---- */
+/* Reads a compressed or uncompressed (binary or text) .plt file into newly
+ * allocated dense arrays (free with hddm_lb_free). nagents plays the role of
+ * the config's nagents: an uncompressed file whose first int differs from it
+ * is read as text, and a dim mismatch is an error, as in the reference. */
+int hddm_lb_plt_read(const char* path, int nagents, int* dim, int* nno, int* dofs,
+	int* level, int32_t** index_i, int32_t** index_j, double** surplus);
+/* Compressed writer: the layout the reference reads at LB/gold/src/Data.cpp:50-190,525-572. */
+int hddm_lb_plt_write_compressed(const char* path, int dim, int nno, int dofs, int level,
+	const int32_t* index_i, const int32_t* index_j, const double* surplus);
+/* Uncompressed writer from levels and indices (binary = 1, text = 0). */
+int hddm_lb_plt_write_uncompressed(const char* path, int binary, int dim, int nno, int dofs,
+	int level, const int32_t* levels, const int32_t* indices, const double* surplus);
+void hddm_lb_free(void* p);
+
+/* ---- seeded sparse grids (synthetic inputs) ---- */
+/* Regular grid: all nodes with sum_d (l_d - 1) <= start_sum, then rounds of
+ * random refinement (SplitMix64, seed): every unrefined node with
+ * sum_d (l_d - 1) < max_level - 1 is refined with probability p_refine
+ * (all its hierarchical children in every dim are added) until the grid holds
+ * at least target_nno nodes (0: no target) or nothing is left to refine. */
+int hddm_lb_grid_generate(int dim, int start_sum, int max_level, double p_refine,
+	uint64_t seed, int64_t target_nno, hddm_lb_grid** grid);
+int hddm_lb_grid_nno(const hddm_lb_grid* grid, int64_t* nno);
+/* Dense (i, j) as uploaded, and (level, index) as stored in uncompressed .plt files. */
+int hddm_lb_grid_index(const hddm_lb_grid* grid, int32_t* index_i, int32_t* index_j);
+int hddm_lb_grid_levels(const hddm_lb_grid* grid, int32_t* levels, int32_t* indices);
+int hddm_lb_grid_free(hddm_lb_grid* grid);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HDDM_LB_H */

@@ -1,0 +1,220 @@
+"""GPU parity of the forward sweep against the CPU oracle (gold restatement).
+
+Tolerances (normwise per dof column, max |dy| / max |y_gold|): FP64 <= 1e-12,
+every mixed-precision MFMA path <= 1e-6, as the target states.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import hddm_lb
+import oracle
+from conftest import GOLDEN
+from gpu_util import ALL_PRECISIONS, TOL, normwise_rel_err, run_forward
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a device"
+    c = hddm_lb.Context(0)
+    yield c
+    c.close()
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def test_kat_on_device(kat):
+    st = kat["state"]
+    c = hddm_lb.Context(0)
+    c.upload_state(0, st["index_i"], st["index_j"], st["surplus"])
+    X = np.array([kat["x"]])
+    yo = oracle.forward(st["index_i"], st["index_j"], st["surplus"], X)
+    for prec in ALL_PRECISIONS:
+        say("kat precision", prec)
+        y = run_forward(c, X, precision=prec)
+        np.testing.assert_allclose(y[0], kat["expected"], rtol=0, atol=kat["epsilon"])
+        assert normwise_rel_err(y, yo) <= TOL[prec], prec
+    c.close()
+
+
+@pytest.mark.parametrize("prec", ALL_PRECISIONS)
+def test_config1_regular_d4(ctx, prec):
+    """Plain FP64 configuration: regular D=4 grid sum(l-1) <= 4 (401 nodes), dofs=8, 4096 points."""
+    ii, jj, lv, ix = hddm_lb.generate_grid(4, 4, 5, 1.0, 1, 0)
+    S = hddm_lb.synthetic_surplus(lv, 8, 11)
+    ctx.upload_state(0, ii, jj, S)
+    X = hddm_lb.synthetic_points(4096, 4, 12)
+    say("config1 precision", prec)
+    y = run_forward(ctx, X, precision=prec)
+    yo = oracle.forward(ii, jj, S, X)
+    err = normwise_rel_err(y, yo)
+    say("  normwise rel err", err)
+    assert err <= TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ALL_PRECISIONS)
+def test_config2_grid_d16_subset(ctx, prec):
+    """D=16 adaptive grid of ~60k nodes, dofs=32, on a 1536-point subset."""
+    ii, jj, lv, ix = hddm_lb.generate_grid(16, 2, 7, 0.3, 1, 60000)
+    S = hddm_lb.synthetic_surplus(lv, 32, 2)
+    ctx.upload_state(0, ii, jj, S)
+    X = hddm_lb.synthetic_points(1536, 16, 3)
+    say("config2 subset precision", prec)
+    y = run_forward(ctx, X, precision=prec)
+    yo = oracle.forward(ii, jj, S, X)
+    err = normwise_rel_err(y, yo)
+    say("  normwise rel err", err)
+    assert err <= TOL[prec]
+
+
+@pytest.mark.parametrize("n", [1, 2, 63, 65, 200])
+def test_ragged_point_counts(ctx, n):
+    ii, jj, lv, ix = hddm_lb.generate_grid(6, 2, 5, 0.5, 5, 3000)
+    S = hddm_lb.synthetic_surplus(lv, 20, 6)
+    ctx.upload_state(0, ii, jj, S)
+    X = hddm_lb.synthetic_points(n, 6, 7)
+    yo = oracle.forward(ii, jj, S, X)
+    for prec in (hddm_lb.FP64, hddm_lb.F16X2):
+        say("ragged n", n, "precision", prec)
+        y = run_forward(ctx, X, precision=prec)
+        assert normwise_rel_err(y, yo) <= TOL[prec]
+
+
+def test_points_on_support_edges(ctx):
+    ii, jj, lv, ix = hddm_lb.generate_grid(5, 3, 5, 0.6, 8, 2000)
+    S = hddm_lb.synthetic_surplus(lv, 6, 9)
+    ctx.upload_state(0, ii, jj, S)
+    rng = np.random.default_rng(1)
+    grid_vals = np.array([0.0, 1.0, 0.5, 0.25, 0.75, 0.125, 0.875, 1.0 / 3])
+    X = grid_vals[rng.integers(0, len(grid_vals), size=(300, 5))]
+    yo = oracle.forward(ii, jj, S, X)
+    for prec in ALL_PRECISIONS:
+        say("edges precision", prec)
+        y = run_forward(ctx, X, precision=prec)
+        assert normwise_rel_err(y, yo) <= TOL[prec]
+
+
+def test_fp64_matches_gold_basis_bitwise_single_node(ctx):
+    """One node per grid (plus root): y = S_root + B_k S_k computed with gold's
+    operation order must agree to the last bit in FP64."""
+    ii = np.array([[0, 0, 0], [4, 8, 2]], np.int32)
+    jj = np.array([[0, 0, 0], [1, 5, 0]], np.int32)
+    S = np.array([[0.0], [1.0]])
+    ctx.upload_state(0, ii, jj, S)
+    X = hddm_lb.synthetic_points(256, 3, 4)
+    y = run_forward(ctx, X, precision=hddm_lb.FP64)
+    yo = oracle.forward(ii, jj, S, X)
+    assert np.array_equal(y, yo)
+
+
+def test_multistate_mixed_dofs(ctx):
+    """Several states, per-state node counts and dofs (16, 32, 64, 40), points
+    assigned by a seeded categorical draw, one launch."""
+    c = hddm_lb.Context(0)
+    specs = [(300, 16), (2500, 32), (800, 64), (5000, 40), (150, 16)]
+    states = []
+    for s, (target, dofs) in enumerate(specs):
+        ii, jj, lv, ix = hddm_lb.generate_grid(8, 2, 6, 0.3, 100 + s, target)
+        S = hddm_lb.synthetic_surplus(lv, dofs, 200 + s)
+        c.upload_state(s, ii, jj, S)
+        states.append((ii, jj, S))
+    rng = np.random.default_rng(5)
+    n = 3000
+    sid = rng.choice(len(specs), size=n, p=[0.1, 0.3, 0.2, 0.3, 0.1]).astype(np.int32)
+    X = hddm_lb.synthetic_points(n, 8, 6)
+    for prec in ALL_PRECISIONS:
+        say("multistate precision", prec)
+        y = run_forward(c, X, sid, precision=prec, ldy=64)
+        for s, (ii, jj, S) in enumerate(states):
+            m = sid == s
+            yo = oracle.forward(ii, jj, S, X[m])
+            assert normwise_rel_err(y[m, :S.shape[1]], yo) <= TOL[prec], (prec, s)
+            if S.shape[1] < 64:
+                assert np.all(np.isnan(y[m, S.shape[1]:]))   # beyond dofs: untouched
+    c.close()
+
+
+def test_invalid_state_ids_leave_rows_untouched(ctx):
+    ii, jj, lv, ix = hddm_lb.generate_grid(4, 3, 4, 0.5, 1, 0)
+    S = hddm_lb.synthetic_surplus(lv, 4, 1)
+    ctx.upload_state(0, ii, jj, S)
+    X = hddm_lb.synthetic_points(100, 4, 2)
+    sid = np.zeros(100, np.int32)
+    sid[::3] = 7
+    sid[1::5] = -1
+    y = run_forward(ctx, X, sid, precision=hddm_lb.FP64)
+    bad = (sid != 0)
+    assert np.all(np.isnan(y[bad]))
+    yo = oracle.forward(ii, jj, S, X[~bad])
+    assert normwise_rel_err(y[~bad], yo) <= 1e-12
+
+
+@pytest.mark.parametrize("prec", [hddm_lb.FP64, hddm_lb.F16X2])
+def test_node_segments_match_unsplit(ctx, prec):
+    """Work items split over node segments (partials + fixed-order reduce) agree with the oracle."""
+    c = hddm_lb.Context(0)
+    ii, jj, lv, ix = hddm_lb.generate_grid(10, 2, 6, 0.3, 4, 20000)
+    S = hddm_lb.synthetic_surplus(lv, 48, 5)
+    c.upload_state(0, ii, jj, S)
+    X = hddm_lb.synthetic_points(700, 10, 6)
+    yo = oracle.forward(ii, jj, S, X)
+    y1 = run_forward(c, X, precision=prec)
+    c.set_option(hddm_lb.OPT_SEGMENT_NODES, 1000)
+    say("segments precision", prec)
+    y2 = run_forward(c, X, precision=prec)
+    assert normwise_rel_err(y1, yo) <= TOL[prec]
+    assert normwise_rel_err(y2, yo) <= TOL[prec]
+    c.close()
+
+
+def test_results_independent_of_point_order(ctx):
+    """Per-point results do not depend on which points share a wave."""
+    ii, jj, lv, ix = hddm_lb.generate_grid(12, 2, 6, 0.3, 9, 8000)
+    S = hddm_lb.synthetic_surplus(lv, 32, 1)
+    ctx.upload_state(0, ii, jj, S)
+    X = hddm_lb.synthetic_points(2000, 12, 2)
+    perm = np.random.default_rng(0).permutation(2000)
+    for prec in (hddm_lb.FP64, hddm_lb.F16X2):
+        a = run_forward(ctx, X, precision=prec)
+        b = run_forward(ctx, X[perm], precision=prec)
+        assert np.array_equal(a[perm], b)
+
+
+def test_graph_plan_replays_match_eager(ctx):
+    import torch
+    ii, jj, lv, ix = hddm_lb.generate_grid(8, 2, 6, 0.3, 3, 5000)
+    S = hddm_lb.synthetic_surplus(lv, 32, 3)
+    ctx.upload_state(0, ii, jj, S)
+    X = hddm_lb.synthetic_points(5000, 8, 4)
+    x = torch.from_numpy(X).cuda()
+    eager = run_forward(ctx, X, precision=hddm_lb.F16X2)
+    y = torch.full((5000, 32), np.nan, dtype=torch.float64, device="cuda")
+    stream = torch.cuda.Stream()
+    plan = ctx.plan(x, y, None, hddm_lb.F16X2, stream)
+    for rep in range(4):
+        say("graph replay", rep)
+        y.fill_(np.nan)
+        torch.cuda.synchronize()
+        plan.launch(stream)
+        stream.synchronize()
+        assert np.array_equal(y.cpu().numpy(), eager)
+        _ = run_forward(ctx, X[:100], precision=hddm_lb.FP64)  # eager work between replays
+    del plan
+
+
+def test_forward_host_path(ctx, kat):
+    st = kat["state"]
+    c = hddm_lb.Context(0)
+    c.upload_state(0, st["index_i"], st["index_j"], st["surplus"])
+    y = np.zeros((1, 118))
+    c.forward_host(np.array([kat["x"]]), y, precision=hddm_lb.FP64)
+    np.testing.assert_allclose(y[0], kat["expected"], rtol=0, atol=kat["epsilon"])
+    c.close()

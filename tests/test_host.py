@@ -1,0 +1,252 @@
+"""Host logic without a device: .plt I/O, grid generator, trie layout, C-ABI symbols."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import hddm_lb
+import oracle
+from conftest import GOLDEN, PKG, ROOT, tiny_grid
+
+
+# ------------------------------------------------------------------ .plt
+
+def test_plt_fixture_layout(kat):
+    st = kat["state"]
+    ii, jj = st["index_i"], st["index_j"]
+    act = (ii != 0) | (jj != 0)
+    assert act.sum() == 118 and act.sum(axis=1).max() == 1
+    assert set(zip(ii[act].tolist(), jj[act].tolist())) == {(2, 0), (2, 2)}
+
+
+def test_plt_roundtrip_all_formats(tmp_path):
+    ii, jj, lv, ix = hddm_lb.generate_grid(7, 2, 5, 0.5, 3, 800)
+    S = hddm_lb.synthetic_surplus(lv, 9, 4)
+    S[::7, 3] = 0.0  # exercise the sparse surplus encoding
+    pc = str(tmp_path / "c.plt")
+    hddm_lb.write_plt_compressed(pc, ii, jj, S, level=5)
+    r = hddm_lb.read_plt(pc, 7)
+    assert (r["dim"], r["nno"], r["dofs"], r["level"]) == (7, ii.shape[0], 9, 5)
+    assert np.array_equal(r["index_i"], ii) and np.array_equal(r["index_j"], jj) and np.array_equal(r["surplus"], S)
+    for binary in (True, False):
+        pu = str(tmp_path / ("u%d.plt" % binary))
+        hddm_lb.write_plt_uncompressed(pu, lv, ix, S, binary=binary, level=5)
+        r = hddm_lb.read_plt(pu, 7)
+        assert np.array_equal(r["index_i"], ii) and np.array_equal(r["index_j"], jj)
+        assert np.array_equal(r["surplus"], S)
+
+
+def test_plt_wide_dims_use_u16_columns(tmp_path):
+    # dim > 255 switches the index IA/JA width to 2 bytes
+    ii, jj, lv, ix = hddm_lb.generate_grid(300, 1, 2, 0.0, 1, 0)
+    S = hddm_lb.synthetic_surplus(lv, 2, 1)
+    p = str(tmp_path / "w.plt")
+    hddm_lb.write_plt_compressed(p, ii, jj, S)
+    r = hddm_lb.read_plt(p, 300)
+    assert np.array_equal(r["index_i"], ii) and np.array_equal(r["surplus"], S)
+
+
+def test_plt_errors(tmp_path):
+    with pytest.raises(hddm_lb.HddmLbError) as e:
+        hddm_lb.read_plt(str(tmp_path / "missing.plt"), 3)
+    assert e.value.code == 6
+    ii, jj, lv, ix = hddm_lb.generate_grid(3, 2, 3, 0.0, 1, 0)
+    S = hddm_lb.synthetic_surplus(lv, 2, 1)
+    p = str(tmp_path / "c.plt")
+    hddm_lb.write_plt_compressed(p, ii, jj, S)
+    with pytest.raises(hddm_lb.HddmLbError) as e:      # nagents mismatch
+        hddm_lb.read_plt(p, 4)
+    assert e.value.code == 7
+    data = open(p, "rb").read()
+    t = str(tmp_path / "t.plt")
+    open(t, "wb").write(data[:-5])                      # truncated
+    with pytest.raises(hddm_lb.HddmLbError):
+        hddm_lb.read_plt(t, 3)
+    # a JA (dim) entry out of range
+    bad = bytearray(data)
+    nnz = int.from_bytes(bad[10 + 16 + 5: 10 + 16 + 9], "little")
+    ja0 = 10 + 16 + 9 + 4 * nnz + (ii.shape[0] + 1)
+    bad[ja0] = 9
+    open(t, "wb").write(bytes(bad))
+    with pytest.raises(hddm_lb.HddmLbError) as e:
+        hddm_lb.read_plt(t, 3)
+    assert e.value.code == 7 and "column" in str(e.value)
+    # uncompressed with a level 0 entry
+    lv2 = lv.copy()
+    lv2[1, 0] = 0
+    u = str(tmp_path / "u.plt")
+    hddm_lb.write_plt_uncompressed(u, lv2, ix, S)
+    with pytest.raises(hddm_lb.HddmLbError) as e:
+        hddm_lb.read_plt(u, 3)
+    assert e.value.code == 7
+
+
+def test_plt_level_transform():
+    """Uncompressed levels map to i = 2 << (l-2), j = index - 1, j = 0 where i == 0."""
+    import tempfile
+    lv = np.array([[1, 2, 3], [4, 1, 2]], np.int32)
+    ix = np.array([[7, 3, 2], [6, 9, 1]], np.int32)
+    S = np.ones((2, 1))
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "l.plt")
+        hddm_lb.write_plt_uncompressed(p, lv, ix, S)
+        r = hddm_lb.read_plt(p, 3)
+    assert r["index_i"].tolist() == [[0, 2, 4], [8, 0, 2]]
+    assert r["index_j"].tolist() == [[0, 2, 1], [5, 0, 0]]
+
+
+# ------------------------------------------------------------------ grids
+
+def test_regular_grid_counts():
+    # D=4, sum(l-1) <= 4: 401 nodes and 896 active factors
+    ii, jj, lv, ix = hddm_lb.generate_grid(4, 4, 5, 1.0, 1, 0)
+    assert ii.shape == (401, 4)
+    assert int(((ii != 0) | (jj != 0)).sum()) == 896
+    # closed form: sum over multi-indices of prod m(l), m = 1, 2, 2, 4, 8, ...
+    m = {1: 1, 2: 2, 3: 2, 4: 4, 5: 8}
+    import itertools
+    tot = sum(int(np.prod([m[l] for l in ls])) for ls in itertools.product(range(1, 6), repeat=4)
+              if sum(l - 1 for l in ls) <= 4)
+    assert tot == 401
+    # D=59, sum <= 3 regular: 281,077 nodes
+    g = hddm_lb.generate_grid(59, 3, 4, 1.0, 1, 0)
+    assert g[0].shape[0] == 281077
+
+
+def test_adaptive_grid_properties():
+    a = hddm_lb.generate_grid(16, 2, 7, 0.3, 1, 60000)
+    b = hddm_lb.generate_grid(16, 2, 7, 0.3, 1, 60000)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    ii, jj, lv, ix = a
+    assert 60000 <= ii.shape[0] < 60000 + 64
+    assert (lv - 1).sum(axis=1).max() <= 6
+    # unique nodes; level/index consistent with (i, j)
+    keys = {tuple(r) for r in np.concatenate([ii, jj], axis=1).tolist()}
+    assert len(keys) == ii.shape[0]
+    assert np.array_equal(np.where(lv > 1, 1 << (lv - 1), 0), ii)
+    assert np.array_equal(np.where(lv > 1, ix - 1, 0), jj)
+    # valid 1-D nodes: level 2 -> j in {0, 2}; level >= 3 -> odd j
+    l2 = lv == 2
+    assert set(np.unique(jj[l2]).tolist()) <= {0, 2}
+    hi = lv >= 3
+    assert np.all(jj[hi] % 2 == 1)
+    c = hddm_lb.generate_grid(16, 2, 7, 0.3, 2, 60000)
+    assert not np.array_equal(c[0][:5000], a[0][:5000]) or not np.array_equal(c[1], a[1])
+
+
+# ------------------------------------------------------------------ layout
+
+def walk_layout(L, X):
+    """CPU emulation of one wave's trie walk (the sweep kernel's algorithm) over
+    the layout records for the points X [64, dim]: returns y [64, dofs] and the
+    fraction of records whose factors were evaluated."""
+    n = X.shape[0]
+    stack = [np.ones(n)] + [None] * L.fmax
+    y = np.zeros((n, L.dofs))
+    skip_end, fresh, visited = 0, True, 0
+    S = L.surplus_sorted
+    k = 0
+    while k < L.nno:
+        if k < skip_end:
+            k = skip_end
+            continue
+        r = L.records[k]
+        F, sh = int(r[0] & 0xff), int((r[0] >> 8) & 0xff)
+        if fresh:
+            sh, fresh = 0, False
+        visited += 1
+        b = np.ones(n)
+        pruned = False
+        for t in range(sh, F):
+            f = int(r[2 + 2 * t])
+            tt = X[:, L.fac_dim[f]] * L.fac_i[f] + L.fac_negj[f]
+            p = stack[t] * np.maximum(1.0 - np.abs(tt), 0.0)
+            stack[t + 1] = p
+            b = p
+            if not p.any():
+                skip_end = int(r[3 + 2 * t])
+                pruned = True
+                break
+        if pruned:
+            k = skip_end
+            continue
+        y += b[:, None] * S[k][None, :]
+        k += 1
+    return y, visited / L.nno
+
+
+@pytest.mark.parametrize("dim,start,level,target,seed", [(4, 4, 5, 0, 1), (8, 2, 5, 2000, 2), (16, 2, 7, 6000, 3)])
+def test_layout_trie_walk_matches_oracle(dim, start, level, target, seed):
+    ii, jj, lv, ix = hddm_lb.generate_grid(dim, start, level, 0.3, seed, target)
+    S = hddm_lb.synthetic_surplus(lv, 5, seed)
+    L = hddm_lb.Layout(ii, jj, S)
+    assert sorted(L.order.tolist()) == list(range(ii.shape[0]))
+    L.surplus_sorted = S[L.order]
+    # records: lexicographic factor sequences, correct shared-prefix lengths
+    seqs = []
+    for k in range(L.nno):
+        r = L.records[k]
+        F = int(r[0] & 0xff)
+        seqs.append(tuple(int(r[2 + 2 * t]) for t in range(F)))
+        act = np.nonzero((ii[L.order[k]] != 0) | (jj[L.order[k]] != 0))[0]
+        assert F == len(act)
+        assert [int(L.fac_dim[f]) for f in seqs[-1]] == act.tolist()
+    assert seqs == sorted(seqs)
+    for k in range(1, L.nno):
+        s = int((L.records[k][0] >> 8) & 0xff)
+        a, b = seqs[k - 1], seqs[k]
+        m = 0
+        while m < min(len(a), len(b)) and a[m] == b[m]:
+            m += 1
+        assert s == m
+    # subtree ends
+    for k in range(L.nno):
+        for t in range(len(seqs[k])):
+            e = int(L.records[k][3 + 2 * t])
+            assert all(seqs[q][:t + 1] == seqs[k][:t + 1] for q in range(k, e))
+            assert e == L.nno or seqs[e][:t + 1] != seqs[k][:t + 1]
+    # walks with pruning reproduce the oracle on a clustered and a random block
+    rng = np.random.default_rng(seed)
+    X = rng.random((4096, dim))
+    key = np.zeros(4096, np.int64)
+    for d in range(dim):
+        key = key * 2 + (X[:, d] >= 0.5)
+    Xs = X[np.argsort(key, kind="stable")]
+    for blk in (Xs[:64], X[:64]):
+        yw, frac = walk_layout(L, blk)
+        yo = oracle.forward(ii, jj, S, blk)
+        np.testing.assert_allclose(yw, yo, rtol=1e-12, atol=1e-12 * np.abs(yo).max())
+
+
+def test_layout_rejects_bad_input():
+    with pytest.raises(hddm_lb.HddmLbError) as e:
+        hddm_lb.Layout(np.zeros((0, 3), np.int32), np.zeros((0, 3), np.int32), np.zeros((0, 2)))
+    assert e.value.code == 1
+
+
+# ------------------------------------------------------------------ C ABI
+
+def declared_functions(header):
+    text = open(header).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\**\s*(hddm_lb_[a-z_0-9]+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions(os.path.join(ROOT, "include", "hddm_lb.h"))
+    assert len(names) >= 30
+    lib = ctypes.CDLL(hddm_lb.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_no_device_paths_fail_cleanly():
+    n = hddm_lb.device_count()
+    if n > 0:
+        pytest.skip("a device is visible")
+    with pytest.raises(hddm_lb.HddmLbError) as e:
+        hddm_lb.Context(0)
+    assert e.value.code == 2 and "no HIP device" in str(e.value)
