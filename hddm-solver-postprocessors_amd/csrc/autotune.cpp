@@ -27,7 +27,6 @@ int autotune(hddm_lb_ctx* ctx, int64_t npoints, const double* x, int64_t ldx, co
 	std::vector<TuneConfig> cands;
 	for (int waves : { 1, 2, 4 })
 	{
-		if ((size_t)waves * ctx->dim * kXRowBytes > 64 * 1024) continue;
 		if (ctx->opt_waves && waves != ctx->opt_waves) continue;
 		for (int slab : { 1, 2 })
 		{

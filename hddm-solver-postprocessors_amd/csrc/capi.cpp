@@ -5,7 +5,7 @@
 #include "layout_build.h"
 #include "context.h"
 
-#include <hip/hip_runtime.h>
+#include <hip/hip_runtime_api.h>
 
 #include <algorithm>
 #include <cmath>
@@ -92,7 +92,7 @@ hddm_lb_ctx::~hddm_lb_ctx()
 	states.clear();
 	scratch.release();
 	for (void* p : { (void*)d_states, (void*)d_cost_order, (void*)d_state_ok,
-		(void*)d_xstage, (void*)d_ystage, (void*)d_sidstage, (void*)d_wbuf })
+		(void*)d_xstage, (void*)d_ystage, (void*)d_sidstage, (void*)d_wbuf, (void*)d_stats })
 		dev_free(p);
 	if (ev0) (void)hipEventDestroy(ev0);
 	if (ev1) (void)hipEventDestroy(ev1);
@@ -123,10 +123,11 @@ extern "C" int hddm_lb_create(int device, hddm_lb_ctx** out)
 	DeviceGuard g(device);
 	std::unique_ptr<hddm_lb_ctx> ctx(new hddm_lb_ctx);
 	ctx->device = device;
-	hipDeviceProp_t prop;
-	HIP_TRY(hipGetDeviceProperties(&prop, device));
-	ctx->num_cus = prop.multiProcessorCount;
-	ctx->max_lds = (size_t)prop.maxSharedMemoryPerMultiProcessor;
+	int cus = 0, lds = 0;
+	HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+	HIP_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device));
+	ctx->num_cus = cus;
+	ctx->max_lds = (size_t)lds;
 	HIP_TRY(hipEventCreate(&ctx->ev0));
 	HIP_TRY(hipEventCreate(&ctx->ev1));
 	*out = ctx.release();
@@ -158,6 +159,7 @@ extern "C" int hddm_lb_set_option(hddm_lb_ctx* ctx, int option, int64_t value)
 		if (!(value == 0 || value == 1 || value == 2))
 			return set_error(HDDM_LB_ERROR_INVALID_VALUE, "set_option: slab tiles must be 0, 1 or 2");
 		ctx->opt_slab = (int)value; break;
+	case HDDM_LB_OPT_STATS: ctx->opt_stats = value != 0; break;
 	case HDDM_LB_OPT_FLUSH_LOG2:
 		if (value < 0 || value > 20) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "set_option: flush log2 in [0, 20]");
 		ctx->opt_wlog = (int)value; break;
@@ -180,8 +182,8 @@ extern "C" int hddm_lb_upload_state(hddm_lb_ctx* ctx, int istate, int dim, int n
 	if (loaded_others && dim != ctx->dim)
 		return set_error(HDDM_LB_ERROR_INVALID_VALUE,
 			"upload_state: # of dimensions (%d) mismatches another state (%d)", dim, ctx->dim);
-	if ((size_t)dim * kXRowBytes > 160 * 1024)
-		return set_error(HDDM_LB_ERROR_UNSUPPORTED, "upload_state: dim %d exceeds the LDS stage (at most 320)", dim);
+	if (xbanks_for(dim) == 0)
+		return set_error(HDDM_LB_ERROR_UNSUPPORTED, "upload_state: dim %d exceeds 64 (x is held in registers)", dim);
 
 	HostLayout L;
 	int rc = build_layout(dim, nno, dofs, index_i, index_j, ld_index, surplus, ld_surplus, L);
@@ -194,7 +196,7 @@ extern "C" int hddm_lb_upload_state(hddm_lb_ctx* ctx, int istate, int dim, int n
 		void* p = dev_alloc(bytes, rc);
 		if (!p) return nullptr;
 		slot->bufs.push_back(p);
-		if (bytes)
+		if (bytes && src)
 		{
 			hipError_t e = hipMemcpy(p, src, bytes, hipMemcpyHostToDevice);
 			if (e != hipSuccess) rc = hip_fail(e, "hipMemcpy(upload)", __FILE__, __LINE__);
@@ -204,9 +206,10 @@ extern "C" int hddm_lb_upload_state(hddm_lb_ctx* ctx, int istate, int dim, int n
 	StateDesc& d = slot->desc;
 	std::memset(&d, 0, sizeof(d));
 	d.dim = dim; d.nno = nno; d.dofs = dofs; d.dofs_pad = L.dofs_pad;
-	d.fmax = L.fmax; d.rec_words = L.rec_words; d.nchunks = L.nchunks;
-	d.rec = (const uint32_t*)up(L.rec.data(), L.rec.size() * sizeof(uint32_t));
-	d.fac = (const Factor*)up(L.fac.data(), L.fac.size() * sizeof(Factor));
+	d.nops = (int)L.skip.size() - 1; d.nchunks = L.nchunks; d.max_depth = L.fmax;
+	d.nseg = 1;
+	d.ops = (const Op*)up(L.ops.data(), L.ops.size() * sizeof(Op));
+	d.skip = (const uint32_t*)up(L.skip.data(), L.skip.size() * sizeof(uint32_t));
 	d.s64 = (const double*)up(L.s64.data(), L.s64.size() * sizeof(double));
 	d.s16 = (const uint16_t*)up(L.s16.data(), L.s16.size() * sizeof(uint16_t));
 	d.sbf = (const uint16_t*)up(L.sbf.data(), L.sbf.size() * sizeof(uint16_t));
@@ -215,6 +218,10 @@ extern "C" int hddm_lb_upload_state(hddm_lb_ctx* ctx, int istate, int dim, int n
 	d.order = (const int32_t*)up(L.order.data(), L.order.size() * sizeof(int32_t));
 	d.lomask = (const uint64_t*)up(L.lomask.data(), L.lomask.size() * sizeof(uint64_t));
 	d.himask = (const uint64_t*)up(L.himask.data(), L.himask.size() * sizeof(uint64_t));
+	slot->d_seg_op = (int32_t*)up(nullptr, sizeof(int32_t) * ((size_t)nno + 1));
+	d.seg_op = slot->d_seg_op;
+	slot->shared = std::move(L.shared);
+	slot->opstart = std::move(L.opstart);
 	if (rc) return rc;
 	slot->loaded = true;
 
@@ -236,7 +243,7 @@ extern "C" int hddm_lb_state_info(hddm_lb_ctx* ctx, int istate, int* dim, int* n
 	if (dim) *dim = d.dim;
 	if (nno) *nno = d.nno;
 	if (dofs) *dofs = d.dofs;
-	if (max_factors) *max_factors = d.fmax;
+	if (max_factors) *max_factors = d.max_depth;
 	return HDDM_LB_SUCCESS;
 }
 
@@ -278,20 +285,33 @@ int prepare_tables(hddm_lb_ctx* ctx)
 	for (int s = 0; s < ns; s++)
 	{
 		std::memset(&descs[s], 0, sizeof(StateDesc));
-		if (!ctx->states[s] || !ctx->states[s]->loaded) { descs[s].nseg = 1; descs[s].seg_len = 16; descs[s].dofs_pad = 32; continue; }
-		StateDesc d = ctx->states[s]->desc;
-		int64_t seg = ctx->opt_seg_nodes;
-		if (seg <= 0) seg = d.nno;
-		seg = (seg + kChunk - 1) / kChunk * kChunk;
-		if (seg > d.nno) seg = (d.nno + kChunk - 1) / kChunk * kChunk;
-		d.seg_len = (int)seg;
-		d.nseg = (int)((d.nno + seg - 1) / seg);
+		if (!ctx->states[s] || !ctx->states[s]->loaded) { descs[s].nseg = 1; descs[s].dofs_pad = 32; continue; }
+		StateSlot& slot = *ctx->states[s];
+		StateDesc d = slot.desc;
+		const int64_t want = ctx->opt_seg_nodes > 0 ? ctx->opt_seg_nodes : 0;
+		if (slot.seg_len != want)
+		{
+			// Segment boundaries at top-level runs, so each segment's first op is at depth 0.
+			std::vector<int32_t> segop;
+			segop.push_back(0);
+			if (want > 0)
+			{
+				int64_t last = 0;
+				for (int n = 1; n < d.nno; n++)
+					if (n - last >= want && slot.shared[n] == 0) { segop.push_back(slot.opstart[n]); last = n; }
+			}
+			segop.push_back(slot.opstart[d.nno]);
+			HIP_TRY(hipMemcpy(slot.d_seg_op, segop.data(), sizeof(int32_t) * segop.size(), hipMemcpyHostToDevice));
+			slot.desc.nseg = (int)segop.size() - 1;
+			slot.seg_len = want;
+		}
+		d = slot.desc;
 		descs[s] = d;
 		ok[s] = 1;
 		ctx->dt_needed = std::max(ctx->dt_needed, d.dofs_pad / 32);
-		ctx->fmax_all = std::max(ctx->fmax_all, d.fmax);
+		ctx->fmax_all = std::max(ctx->fmax_all, d.max_depth);
 		if (d.nseg > 1) ctx->max_nseg_dp = std::max<int64_t>(ctx->max_nseg_dp, (int64_t)d.nseg * d.dofs_pad);
-		cost.push_back(std::make_pair(-(double)d.seg_len * d.dofs_pad, s));
+		cost.push_back(std::make_pair(-(double)d.nops / d.nseg * d.dofs_pad, s));
 	}
 	std::stable_sort(cost.begin(), cost.end());
 	std::vector<int32_t> order;
@@ -364,9 +384,8 @@ TuneConfig default_config(hddm_lb_ctx* ctx, int prec)
 {
 	TuneConfig c;
 	c.waves = ctx->opt_waves ? ctx->opt_waves : 4;
-	while (c.waves > 1 && (size_t)c.waves * ctx->dim * kXRowBytes > 64 * 1024) c.waves /= 2;
 	int slab = ctx->opt_slab;
-	if (!slab) slab = (prec == HDDM_LB_FP64) ? std::min(2, ctx->dt_needed) : 1;
+	if (!slab || prec == HDDM_LB_FP64) slab = 1;
 	c.slab = slab;
 	c.wlog = ctx->opt_wlog >= 0 ? ctx->opt_wlog : 3;
 	return c;
@@ -376,6 +395,9 @@ TuneConfig default_config(hddm_lb_ctx* ctx, int prec)
 int enqueue_forward(hddm_lb_ctx* ctx, Scratch& sc, int64_t npoints, const double* x, int64_t ldx,
 	const int32_t* sid, double* y, int64_t ldy, int prec, hipStream_t st, const TuneConfig& cfg, bool timed)
 {
+	hipStreamCaptureStatus cap_status = hipStreamCaptureStatusNone;
+	HIP_TRY(hipStreamIsCapturing(st, &cap_status));
+	const bool in_capture = cap_status != hipStreamCaptureStatusNone;
 	const int ns = ctx->tables_n;
 	const int bb = bucket_bits(npoints, ns, ctx->opt_sort);
 	const int64_t nb = (int64_t)(ns + 1) << bb;
@@ -385,15 +407,13 @@ int enqueue_forward(hddm_lb_ctx* ctx, Scratch& sc, int64_t npoints, const double
 		sc.plans, sc.hdr, st));
 	HIP_TRY(launch_scatter(npoints, sc.bkey, sc.cursor, sc.perm, st));
 
-	const void* kern = sweep_kernel_ptr(prec, cfg.slab, ctx->fmax_all);
-	if (!kern) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "forward: unknown precision %d", prec);
-	const size_t lds = (size_t)cfg.waves * ctx->dim * kXRowBytes;
-	if (lds > 64 * 1024)
-		HIP_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+	const void* kern = sweep_kernel_ptr(prec, cfg.slab, xbanks_for(ctx->dim), ctx->fmax_all);
+	if (!kern) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "forward: no sweep kernel for precision %d, dim %d", prec, ctx->dim);
+	const size_t lds = sweep_lds_bytes(cfg.waves, ctx->fmax_all);
 	int per_cu = 0;
 	HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, cfg.waves * kWave, lds));
 	if (per_cu < 1)
-		return set_error(HDDM_LB_ERROR_UNSUPPORTED, "forward: sweep kernel does not fit a CU (dim %d, %d waves)", ctx->dim, cfg.waves);
+		return set_error(HDDM_LB_ERROR_UNSUPPORTED, "forward: sweep kernel does not fit a CU (%d waves)", cfg.waves);
 	const int grid = per_cu * ctx->num_cus;
 
 	SweepArgs a;
@@ -401,6 +421,17 @@ int enqueue_forward(hddm_lb_ctx* ctx, Scratch& sc, int64_t npoints, const double
 	a.states = ctx->d_states; a.plans = sc.plans; a.cost_order = ctx->d_cost_order; a.hdr = sc.hdr;
 	a.perm = sc.perm; a.x = x; a.ldx = ldx; a.y = y; a.ldy = ldy; a.partial = sc.partial;
 	a.nstates = ns; a.dim = ctx->dim; a.wlog = cfg.wlog;
+	if (ctx->opt_stats && !in_capture)
+	{
+		if (!ctx->d_stats)
+		{
+			int rc = 0;
+			ctx->d_stats = (unsigned long long*)dev_alloc(4 * sizeof(unsigned long long), rc);
+			if (rc) return rc;
+		}
+		HIP_TRY(launch_fill_u32((uint32_t*)ctx->d_stats, 8, 0u, st));
+		a.stats = ctx->d_stats;
+	}
 	const int nslab = (ctx->dt_needed + cfg.slab - 1) / cfg.slab;
 	if (timed) HIP_TRY(hipEventRecord(ctx->ev0, st));
 	for (int sl = 0; sl < nslab; sl++)
@@ -448,6 +479,16 @@ extern "C" int hddm_lb_forward(hddm_lb_ctx* ctx, int64_t npoints,
 		return rc;
 	ctx->last_timed = ctx->opt_timing;
 	return enqueue_forward(ctx, ctx->scratch, npoints, x, ldx, state_ids, y, ldy, precision, (hipStream_t)stream, cfg, ctx->opt_timing);
+}
+
+extern "C" int hddm_lb_last_stats(hddm_lb_ctx* ctx, uint64_t* stats)
+{
+	if (!ctx || !stats) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "last_stats: NULL argument");
+	if (!ctx->d_stats) return set_error(HDDM_LB_ERROR_STATE, "last_stats: HDDM_LB_OPT_STATS was not set for a forward");
+	DeviceGuard g(ctx->device);
+	HIP_TRY(hipDeviceSynchronize());
+	HIP_TRY(hipMemcpy(stats, ctx->d_stats, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost));
+	return HDDM_LB_SUCCESS;
 }
 
 extern "C" double hddm_lb_last_sweep_ms(hddm_lb_ctx* ctx)

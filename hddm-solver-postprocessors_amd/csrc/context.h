@@ -1,7 +1,7 @@
 // Host-side context of one device.
 #pragma once
 
-#include <hip/hip_runtime.h>
+#include <hip/hip_runtime_api.h>
 
 #include <map>
 #include <memory>
@@ -29,6 +29,10 @@ struct StateSlot
 	bool loaded = false;
 	StateDesc desc;              // device pointers, seg fields filled per call
 	std::vector<void*> bufs;     // device allocations owned by the slot
+	std::vector<uint8_t> shared; // host copy: factors shared with the previous node
+	std::vector<int32_t> opstart;// host copy: first op of each node
+	int32_t* d_seg_op = nullptr; // device segment table (capacity nno + 1)
+	int64_t seg_len = -1;        // segment length the table was built for
 	~StateSlot();
 	void release();
 };
@@ -96,7 +100,9 @@ struct hddm_lb_ctx
 	// adjoint scratch
 	float* d_wbuf = nullptr; size_t cap_wbuf = 0;
 
-	// timing of the sweep
+	// measurement
+	bool opt_stats = false;
+	unsigned long long* d_stats = nullptr;
 	hipEvent_t ev0 = nullptr, ev1 = nullptr;
 	bool last_timed = false;
 

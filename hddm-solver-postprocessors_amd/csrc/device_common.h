@@ -16,6 +16,16 @@ __device__ __forceinline__ T ldc(const T* p)
 	return *(const __attribute__((address_space(4))) T*)p;
 }
 
+// A 16-byte op through one scalar load.
+__device__ __forceinline__ Op ldc_op(const Op* p)
+{
+	typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+	const u4v v = *(const __attribute__((address_space(4))) u4v*)p;
+	Op op;
+	op.dim = v[0]; op.ctrl = v[1]; op.ihi = v[2]; op.njhi = v[3];
+	return op;
+}
+
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 __device__ __forceinline__ uint64_t uniform64(uint64_t v)

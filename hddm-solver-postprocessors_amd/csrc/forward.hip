@@ -20,6 +20,7 @@
 //                    (FP64 VALU, or MFMA on 16-node chunks with split operands)
 //   6. reduce        sums node-segment partials of split states (fixed order)
 #include "device_common.h"
+#include <type_traits>
 #include "kernels.h"
 
 namespace hddm_lb {
@@ -117,54 +118,6 @@ __global__ void scatter_kernel(int64_t n, const uint32_t* __restrict__ bkey,
 
 namespace {
 
-// Trie walk state of one wave.
-template <int FMAX>
-struct Walker
-{
-	double stack[FMAX + 1];   // stack[t]: product of the first t factors of the current path
-	int skip_end;             // nodes < skip_end are pruned
-	bool fresh;               // the next evaluated node must rebuild its whole prefix
-};
-
-// Evaluates node n (not pruned by skip_end). Returns false and advances
-// w.skip_end when a prefix product is zero on every lane; otherwise b is B_n.
-template <int FMAX>
-__device__ __forceinline__ bool eval_node(Walker<FMAX>& w, const uint32_t* rec, int rw,
-	const Factor* fac, const double* xs, int lane, int n, double& b)
-{
-	const uint32_t* r = rec + (size_t)n * rw;
-	const uint32_t w0 = ldc(r);
-	const int F = (int)(w0 & 0xffu);
-	int sh = (int)((w0 >> 8) & 0xffu);
-	if (w.fresh) { sh = 0; w.fresh = false; }
-	b = 1.0;
-#pragma unroll
-	for (int t = 0; t < FMAX; t++)
-	{
-		if (t >= sh && t < F)
-		{
-			const uint32_t fid = ldc(r + 2 + 2 * t);
-			const Factor* f = fac + fid;
-			const double fi = ldc(&f->i);
-			const double fnj = ldc(&f->negj);
-			const uint32_t xoff = ldc(&f->xoff);
-			const double xv = xs[(xoff >> 3) + lane];
-			// 1 - |x*i - j| as a multiply then an add, like the reference
-			const double tt = __dadd_rn(__dmul_rn(xv, fi), fnj);
-			const double phi = fmax(__dadd_rn(1.0, -fabs(tt)), 0.0);
-			const double p = __dmul_rn(w.stack[t], phi);
-			w.stack[t + 1] = p;
-			b = p;
-			if (!wave_any(p != 0.0))
-			{
-				w.skip_end = (int)ldc(r + 3 + 2 * t);
-				return false;
-			}
-		}
-	}
-	return true;
-}
-
 __device__ __forceinline__ uint32_t pack_h2(float a, float b)
 {
 	h2_t v;
@@ -191,6 +144,11 @@ __device__ __forceinline__ V as_vec8(uint32_t a, uint32_t b, uint32_t c, uint32_
 {
 	u4_t u = { a, b, c, d };
 	return __builtin_bit_cast(V, u);
+}
+
+__device__ __forceinline__ double f64_from_high(uint32_t hi)
+{
+	return __builtin_bit_cast(double, (uint64_t)hi << 32);
 }
 
 struct ItemInfo
@@ -228,88 +186,366 @@ __device__ __forceinline__ bool next_item(const SweepArgs& a, int lane, ItemInfo
 	return true;
 }
 
+// Coordinates of the lane's point are held in XB register banks of 16
+// doubles (separate arrays); a bank is indexed with a wave-uniform dim
+// (s_set_gpr_idx), never through scratch.
+template <int XB>
+__device__ __forceinline__ double xget(const double (&b0)[16], const double (&b1)[16], const double (&b2)[16],
+	const double (&b3)[16], uint32_t dim)
+{
+	const uint32_t idx = dim & 15u;
+	if constexpr (XB == 1) return b0[idx];
+	else
+	{
+		const uint32_t bank = dim >> 4;
+		if (bank == 0) return b0[idx];
+		if constexpr (XB == 2) return b1[idx];
+		else
+		{
+			if (bank == 1) return b1[idx];
+			if (bank == 2) return b2[idx];
+			return b3[idx];
+		}
+	}
+}
+
+// S^T operands (the MFMA A side) of one 16-node chunk, loaded ahead of use.
+template <int PREC, int DT>
+struct SOperands
+{
+	typedef typename std::conditional<PREC == HDDM_LB_F16X2, h8_t,
+		typename std::conditional<PREC == HDDM_LB_BF16X3, b8_t, float>::type>::type V;
+	static constexpr int kTerms = PREC == HDDM_LB_F16X2 ? 2 : (PREC == HDDM_LB_BF16X3 ? 3 : 8);
+	V A[DT][kTerms];
+
+	__device__ __forceinline__ void load(const StateDesc* sd, int chunk, int DP, int dt0, int ndt, int lane)
+	{
+		const int row = lane & 31;
+		const int khalf = lane >> 5;
+#pragma unroll
+		for (int dt = 0; dt < DT; dt++)
+			if (dt < ndt)
+			{
+				const int dr = (dt0 + dt) * 32 + row;
+				if constexpr (PREC == HDDM_LB_F16X2)
+				{
+					const uint16_t* sc = ldc(&sd->s16) + (size_t)chunk * 2 * DP * kChunk;
+					A[dt][0] = *(const h8_t*)(sc + ((size_t)dr) * kChunk + khalf * 8);
+					A[dt][1] = *(const h8_t*)(sc + ((size_t)DP + dr) * kChunk + khalf * 8);
+				}
+				else if constexpr (PREC == HDDM_LB_BF16X3)
+				{
+					const uint16_t* sc = ldc(&sd->sbf) + (size_t)chunk * 3 * DP * kChunk;
+#pragma unroll
+					for (int term = 0; term < 3; term++)
+						A[dt][term] = *(const b8_t*)(sc + ((size_t)term * DP + dr) * kChunk + khalf * 8);
+				}
+				else
+				{
+					const float* ap = ldc(&sd->s32) + (size_t)chunk * DP * kChunk + ((size_t)dr * 2 + khalf) * 8;
+#pragma unroll
+					for (int q = 0; q < 8; q++) A[dt][q] = ap[q];
+				}
+			}
+	}
+};
+
+// MFMA contraction of one 16-node chunk: acc[t][dt] (points 32t.., dofs
+// 32(dt0+dt)..) += S^T[dofs x 16 nodes] * B^T[16 nodes x 32 points].
+template <int PREC, int DT>
+__device__ __forceinline__ void contract_chunk(const SOperands<PREC, DT>& so, int ndt,
+	const float (&b)[kChunk], f16_t (&acc)[2][DT])
+{
+	if constexpr (PREC == HDDM_LB_F16X2)
+	{
+		uint32_t hi[8], lo[8];
+#pragma unroll
+		for (int m = 0; m < 8; m++)
+		{
+			const float f0 = b[2 * m], f1 = b[2 * m + 1];
+			hi[m] = pack_h2(f0, f1);
+			lo[m] = pack_h2(f0 - lo_half_f16(hi[m]), f1 - hi_half_f16(hi[m]));
+		}
+		// Lanes 0-31 own points 0-31, lanes 32-63 points 32-63; each holds
+		// nodes 0-15 of its point. The B operand of points 0-31 needs nodes
+		// 0-7 in lanes 0-31 and nodes 8-15 in lanes 32-63: one half-swap.
+#pragma unroll
+		for (int m = 0; m < 4; m++)
+		{
+			auto rh = __builtin_amdgcn_permlane32_swap(hi[m], hi[m + 4], false, false);
+			hi[m] = rh[0]; hi[m + 4] = rh[1];
+			auto rl = __builtin_amdgcn_permlane32_swap(lo[m], lo[m + 4], false, false);
+			lo[m] = rl[0]; lo[m + 4] = rl[1];
+		}
+		const h8_t Bh0 = as_vec8<h8_t>(hi[0], hi[1], hi[2], hi[3]);
+		const h8_t Bh1 = as_vec8<h8_t>(hi[4], hi[5], hi[6], hi[7]);
+		const h8_t Bl0 = as_vec8<h8_t>(lo[0], lo[1], lo[2], lo[3]);
+		const h8_t Bl1 = as_vec8<h8_t>(lo[4], lo[5], lo[6], lo[7]);
+#pragma unroll
+		for (int dt = 0; dt < DT; dt++)
+			if (dt < ndt)
+			{
+				const h8_t Ah = so.A[dt][0], Al = so.A[dt][1];
+				acc[0][dt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Al, Bh0, acc[0][dt], 0, 0, 0);
+				acc[0][dt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bl0, acc[0][dt], 0, 0, 0);
+				acc[0][dt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bh0, acc[0][dt], 0, 0, 0);
+				acc[1][dt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Al, Bh1, acc[1][dt], 0, 0, 0);
+				acc[1][dt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bl1, acc[1][dt], 0, 0, 0);
+				acc[1][dt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bh1, acc[1][dt], 0, 0, 0);
+			}
+	}
+	else if constexpr (PREC == HDDM_LB_BF16X3)
+	{
+		uint32_t t1[8], t2[8], t3[8];
+#pragma unroll
+		for (int m = 0; m < 8; m++)
+		{
+			const float f0 = b[2 * m], f1 = b[2 * m + 1];
+			t1[m] = pack_b2(f0, f1);
+			const float r0 = f0 - lo_half_bf16(t1[m]), r1 = f1 - hi_half_bf16(t1[m]);
+			t2[m] = pack_b2(r0, r1);
+			t3[m] = pack_b2(r0 - lo_half_bf16(t2[m]), r1 - hi_half_bf16(t2[m]));
+		}
+#pragma unroll
+		for (int m = 0; m < 4; m++)
+		{
+			auto r1 = __builtin_amdgcn_permlane32_swap(t1[m], t1[m + 4], false, false);
+			t1[m] = r1[0]; t1[m + 4] = r1[1];
+			auto r2 = __builtin_amdgcn_permlane32_swap(t2[m], t2[m + 4], false, false);
+			t2[m] = r2[0]; t2[m + 4] = r2[1];
+			auto r3 = __builtin_amdgcn_permlane32_swap(t3[m], t3[m + 4], false, false);
+			t3[m] = r3[0]; t3[m + 4] = r3[1];
+		}
+		b8_t B[2][3];
+		B[0][0] = as_vec8<b8_t>(t1[0], t1[1], t1[2], t1[3]);
+		B[1][0] = as_vec8<b8_t>(t1[4], t1[5], t1[6], t1[7]);
+		B[0][1] = as_vec8<b8_t>(t2[0], t2[1], t2[2], t2[3]);
+		B[1][1] = as_vec8<b8_t>(t2[4], t2[5], t2[6], t2[7]);
+		B[0][2] = as_vec8<b8_t>(t3[0], t3[1], t3[2], t3[3]);
+		B[1][2] = as_vec8<b8_t>(t3[4], t3[5], t3[6], t3[7]);
+#pragma unroll
+		for (int dt = 0; dt < DT; dt++)
+			if (dt < ndt)
+			{
+#pragma unroll
+				for (int t = 0; t < 2; t++)
+				{
+					// smallest products first
+					acc[t][dt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(so.A[dt][2], B[t][0], acc[t][dt], 0, 0, 0);
+					acc[t][dt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(so.A[dt][1], B[t][1], acc[t][dt], 0, 0, 0);
+					acc[t][dt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(so.A[dt][0], B[t][2], acc[t][dt], 0, 0, 0);
+					acc[t][dt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(so.A[dt][1], B[t][0], acc[t][dt], 0, 0, 0);
+					acc[t][dt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(so.A[dt][0], B[t][1], acc[t][dt], 0, 0, 0);
+					acc[t][dt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(so.A[dt][0], B[t][0], acc[t][dt], 0, 0, 0);
+				}
+			}
+	}
+	else  // HDDM_LB_F32: v_mfma_f32_32x32x2_f32, K = 2 nodes per step
+	{
+		float B0[8], B1[8];
+#pragma unroll
+		for (int q = 0; q < 8; q++)
+		{
+			auto r = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, b[2 * q]),
+				__builtin_bit_cast(uint32_t, b[2 * q + 1]), false, false);
+			B0[q] = __builtin_bit_cast(float, (uint32_t)r[0]);
+			B1[q] = __builtin_bit_cast(float, (uint32_t)r[1]);
+		}
+#pragma unroll
+		for (int dt = 0; dt < DT; dt++)
+			if (dt < ndt)
+			{
+#pragma unroll
+				for (int q = 0; q < 8; q++)
+				{
+					acc[0][dt] = __builtin_amdgcn_mfma_f32_32x32x2f32(so.A[dt][q], B0[q], acc[0][dt], 0, 0, 0);
+					acc[1][dt] = __builtin_amdgcn_mfma_f32_32x32x2f32(so.A[dt][q], B1[q], acc[1][dt], 0, 0, 0);
+				}
+			}
+	}
+}
+
 } // namespace
 
-template <int PREC, int DT, int FMAX>
+// Walk state of one wave. P[t] (the product of the first t factors of the
+// current trie path) lives in the wave's LDS stage; the two most recent
+// products stay in registers, so only a move up the trie reads LDS.
+template <int XB, typename BT>
+struct Walker
+{
+	const Op* ops;
+	const uint32_t* skip;
+	double* P;          // [PM][64] in LDS
+	BT* B;              // [16][64] in LDS: basis values of the current chunk
+	double base_prev;   // P[last_t]
+	double cur;         // P[last_t + 1]
+	int last_t;
+	int q, q1;
+	Op op;
+	uint32_t sk;
+	int chunk;          // chunk whose positions B holds (-1: none)
+	uint32_t mask;      // positions of B written in this chunk
+	uint32_t nops, nprunes;  // counts for measurement
+
+	__device__ __forceinline__ void init(const StateDesc* sd, int seg, double* Pl, BT* Bl, int lane)
+	{
+		ops = ldc(&sd->ops);
+		skip = ldc(&sd->skip);
+		const int32_t* seg_op = ldc(&sd->seg_op);
+		q = ldc(seg_op + seg);
+		q1 = ldc(seg_op + seg + 1);
+		P = Pl;
+		B = Bl;
+		P[lane] = 1.0;
+		base_prev = 1.0;
+		cur = 1.0;
+		last_t = -1;
+		op = ldc_op(ops + q);
+		sk = ldc(skip + q);
+		chunk = -1;
+		mask = 0;
+		nops = 0;
+		nprunes = 0;
+	}
+
+	// Walks ops, writing basis values of nodes of `chunk` into B. Stops at the
+	// first surviving node of another chunk, returned in (node, val) and not
+	// yet stored, or at the end of the segment (returns false).
+	__device__ __forceinline__ bool run(int lane, const double (&xb0)[16], const double (&xb1)[16],
+		const double (&xb2)[16], const double (&xb3)[16], int& node, double& val)
+	{
+		while (q < q1)
+		{
+			// prefetch the sequential successor (ops and skip carry a sentinel)
+			const Op nop = ldc_op(ops + q + 1);
+			const uint32_t nsk = ldc(skip + q + 1);
+			const int t = (int)(op.ctrl & 15u);
+			double bt;
+			if (t == last_t + 1) bt = cur;
+			else if (t == last_t) bt = base_prev;
+			else bt = P[t * kWave + lane];
+			const double tt = fma(xget<XB>(xb0, xb1, xb2, xb3, op.dim), f64_from_high(op.ihi), f64_from_high(op.njhi));
+			const double p = bt * fmax(1.0 - fabs(tt), 0.0);
+			P[(t + 1) * kWave + lane] = p;
+			base_prev = bt;
+			cur = p;
+			last_t = t;
+			nops++;
+			if (!wave_any(p != 0.0))
+			{
+				nprunes++;
+				q = (int)sk;
+				op = ldc_op(ops + q);
+				sk = ldc(skip + q);
+				continue;
+			}
+			const bool last = (op.ctrl & kOpLast) != 0;
+			const int n = (int)(op.ctrl >> kOpNodeShift);
+			q++;
+			op = nop;
+			sk = nsk;
+			if (last)
+			{
+				if ((n >> 4) != chunk) { node = n; val = p; return true; }
+				B[(n & 15) * kWave + lane] = (BT)p;
+				mask |= 1u << (n & 15);
+			}
+		}
+		return false;
+	}
+
+	__device__ __forceinline__ void start_chunk(int node, double val, int lane)
+	{
+		chunk = node >> 4;
+		mask = 1u << (node & 15);
+		B[(node & 15) * kWave + lane] = (BT)val;
+	}
+};
+
+template <int PREC, int DT, int XB, int PM>
 __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 {
-	extern __shared__ double lds_x[];
+	extern __shared__ double lds[];
 	const int lane = threadIdx.x & 63;
 	const int wave = threadIdx.x >> 6;
-	double* xs = lds_x + (size_t)wave * a.dim * kWave;
+	// per wave: P stack [PM][64] doubles, then the chunk buffer [16][64]
+	// (doubles for the FP64 contraction, floats for the MFMA paths)
+	typedef typename std::conditional<PREC == HDDM_LB_FP64, double, float>::type BT;
+	double* Pl = lds + (size_t)wave * (PM * kWave + kChunk * kWave);
+	BT* Bl = (BT*)(Pl + PM * kWave);
 
 	ItemInfo it;
 	while (next_item(a, lane, it))
 	{
 		const StateDesc* sd = a.states + it.s;
-		const int myp = lane < it.np ? lane : it.np - 1;
-		const uint32_t orig = a.perm[it.start + it.blk * kWave + myp];
-		const double* xrow = a.x + (int64_t)orig * a.ldx;
-		for (int d = 0; d < a.dim; d++) xs[d * kWave + lane] = xrow[d];
-
-		const int nno = ldc(&sd->nno);
 		const int dofs = ldc(&sd->dofs);
 		const int DP = ldc(&sd->dofs_pad);
-		const int rw = ldc(&sd->rec_words);
-		const int seg_len = ldc(&sd->seg_len);
 		const int nseg = ldc(&sd->nseg);
-		const uint32_t* rec = ldc(&sd->rec);
-		const Factor* fac = ldc(&sd->fac);
 		const int dt0 = a.dt_base;
 		const int ndt = min(DT, (DP >> 5) - dt0);
 		if (ndt <= 0) continue;  // the state has no dofs in this pass
-		const int n_begin = it.seg * seg_len;
-		const int n_end = min(nno, n_begin + seg_len);
 
-		Walker<FMAX> w;
-		w.stack[0] = 1.0;
-		w.skip_end = n_begin;
-		w.fresh = true;
+		const int myp = lane < it.np ? lane : it.np - 1;
+		const uint32_t orig = a.perm[it.start + it.blk * kWave + myp];
+		const double* xrow = a.x + (int64_t)orig * a.ldx;
+		double xb0[16], xb1[16], xb2[16], xb3[16];
+#pragma unroll
+		for (int d = 0; d < 16; d++)
+		{
+			xb0[d] = xrow[d < a.dim ? d : 0];
+			xb1[d] = XB > 1 ? xrow[16 + d < a.dim ? 16 + d : 0] : 0.0;
+			xb2[d] = XB > 2 ? xrow[32 + d < a.dim ? 32 + d : 0] : 0.0;
+			xb3[d] = XB > 2 ? xrow[48 + d < a.dim ? 48 + d : 0] : 0.0;
+		}
+#pragma unroll
+		for (int k = 0; k < kChunk; k++) Bl[k * kWave + lane] = (BT)0;
 
-		// Output row of lane's point (or of point q for the MFMA layouts).
+		Walker<XB, BT> w;
+		w.init(sd, it.seg, Pl, Bl, lane);
+		uint32_t stat_chunks = 0;
+
 		const bool split = nseg > 1;
 		double* const part = a.partial + ldc(&a.plans[it.s].partial_base);
 
 		if constexpr (PREC == HDDM_LB_FP64)
 		{
-			const double* s64 = ldc(&sd->s64);
-			double acc[DT * 32];
+			const double* s64 = ldc(&sd->s64) + 32 * dt0;
+			double acc[32];
 #pragma unroll
-			for (int c = 0; c < DT * 32; c++) acc[c] = 0.0;
-			for (int n = n_begin; n < n_end; )
+			for (int c = 0; c < 32; c++) acc[c] = 0.0;
+			for (;;)
 			{
-				if (n < w.skip_end) { n = w.skip_end; continue; }
-				double b;
-				if (!eval_node<FMAX>(w, rec, rw, fac, xs, lane, n, b)) { n = w.skip_end; continue; }
-				const double* srow = s64 + (size_t)n * DP + 32 * dt0;
+				int node = 0;
+				double val = 0.0;
+				const bool more = w.run(lane, xb0, xb1, xb2, xb3, node, val);
+				// FP64 contraction of the finished chunk, positions in ascending order
+				if (a.stats && w.chunk >= 0) stat_chunks++;
+				for (uint32_t m = w.mask; m; m &= m - 1)
+				{
+					const int k = __builtin_ctz(m);
+					const double b = Bl[k * kWave + lane];
+					const double* srow = s64 + (size_t)(w.chunk * kChunk + k) * DP;
 #pragma unroll
-				for (int dt = 0; dt < DT; dt++)
-					if (dt < ndt)
-					{
-#pragma unroll
-						for (int c = 0; c < 32; c++)
-							acc[dt * 32 + c] = fma(b, ldc(srow + dt * 32 + c), acc[dt * 32 + c]);
-					}
-				n++;
+					for (int c = 0; c < 32; c++) acc[c] = fma(b, ldc(srow + c), acc[c]);
+				}
+				if (!more) break;
+				w.start_chunk(node, val, lane);
 			}
 			if (lane < it.np)
 			{
-				double* out;
-				if (split)
-					out = part + ((int64_t)it.seg * it.count + it.blk * kWave + lane) * DP;
-				else
-					out = a.y + (int64_t)orig * a.ldy;
+				double* out = split ? part + ((int64_t)it.seg * it.count + it.blk * kWave + lane) * DP
+					: a.y + (int64_t)orig * a.ldy;
 				out += 32 * dt0;
 				const int dlim = dofs - 32 * dt0;
 #pragma unroll
-				for (int dt = 0; dt < DT; dt++)
-#pragma unroll
-					for (int c = 0; c < 32; c++)
-						if (dt * 32 + c < dlim) out[dt * 32 + c] = acc[dt * 32 + c];
+				for (int c = 0; c < 32; c++)
+					if (c < dlim) out[c] = acc[c];
 			}
 		}
 		else
 		{
+			// FP32 accumulation in the MFMA accumulators inside fixed windows
+			// of 2^wlog chunks, folded into FP64 at window changes.
 			f16_t acc[2][DT];
 			double acc64[2][DT][16];
 #pragma unroll
@@ -319,158 +555,36 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 #pragma unroll
 					for (int r = 0; r < 16; r++) { acc[t][dt][r] = 0.0f; acc64[t][dt][r] = 0.0; }
 			int cur_win = -1;
-			const int row = lane & 31;
-			const int khalf = lane >> 5;
-
-			for (int c0 = n_begin; c0 < n_end; c0 += kChunk)
+			SOperands<PREC, DT> so;
+			for (;;)
 			{
-				if (w.skip_end >= c0 + kChunk) { c0 = (w.skip_end & ~(kChunk - 1)) - kChunk; continue; }
-				double b[kChunk];
-				bool any = false;
-#pragma unroll
-				for (int k = 0; k < kChunk; k++)
+				int node = 0;
+				double val = 0.0;
+				const bool more = w.run(lane, xb0, xb1, xb2, xb3, node, val);
+				if (w.chunk >= 0)
 				{
-					b[k] = 0.0;
-					const int n = c0 + k;
-					if (n < n_end && n >= w.skip_end)
+					if (a.stats) stat_chunks++;
+					const int win = w.chunk >> a.wlog;
+					if (win != cur_win)
 					{
-						double v;
-						if (eval_node<FMAX>(w, rec, rw, fac, xs, lane, n, v)) { b[k] = v; any = true; }
+#pragma unroll
+						for (int t = 0; t < 2; t++)
+#pragma unroll
+							for (int dt = 0; dt < DT; dt++)
+#pragma unroll
+								for (int r = 0; r < 16; r++) { acc64[t][dt][r] += (double)acc[t][dt][r]; acc[t][dt][r] = 0.0f; }
+						cur_win = win;
 					}
+					float b[kChunk];
+#pragma unroll
+					for (int k = 0; k < kChunk; k++) { b[k] = Bl[k * kWave + lane]; Bl[k * kWave + lane] = 0.0f; }
+					contract_chunk<PREC, DT>(so, ndt, b, acc);
 				}
-				if (!any) continue;
-
-				const int chunk = c0 / kChunk;
-				const int win = chunk >> a.wlog;
-				if (win != cur_win)
-				{
-#pragma unroll
-					for (int t = 0; t < 2; t++)
-#pragma unroll
-						for (int dt = 0; dt < DT; dt++)
-#pragma unroll
-							for (int r = 0; r < 16; r++) { acc64[t][dt][r] += (double)acc[t][dt][r]; acc[t][dt][r] = 0.0f; }
-					cur_win = win;
-				}
-
-				if constexpr (PREC == HDDM_LB_F16X2)
-				{
-					uint32_t hi[8], lo[8];
-#pragma unroll
-					for (int m = 0; m < 8; m++)
-					{
-						const float f0 = (float)b[2 * m], f1 = (float)b[2 * m + 1];
-						hi[m] = pack_h2(f0, f1);
-						lo[m] = pack_h2(f0 - lo_half_f16(hi[m]), f1 - hi_half_f16(hi[m]));
-					}
-#pragma unroll
-					for (int m = 0; m < 4; m++)
-					{
-						auto rh = __builtin_amdgcn_permlane32_swap(hi[m], hi[m + 4], false, false);
-						hi[m] = rh[0]; hi[m + 4] = rh[1];
-						auto rl = __builtin_amdgcn_permlane32_swap(lo[m], lo[m + 4], false, false);
-						lo[m] = rl[0]; lo[m + 4] = rl[1];
-					}
-					const h8_t Bh0 = as_vec8<h8_t>(hi[0], hi[1], hi[2], hi[3]);
-					const h8_t Bh1 = as_vec8<h8_t>(hi[4], hi[5], hi[6], hi[7]);
-					const h8_t Bl0 = as_vec8<h8_t>(lo[0], lo[1], lo[2], lo[3]);
-					const h8_t Bl1 = as_vec8<h8_t>(lo[4], lo[5], lo[6], lo[7]);
-					const uint16_t* sc = ldc(&sd->s16) + (size_t)chunk * 2 * DP * kChunk;
-#pragma unroll
-					for (int dt = 0; dt < DT; dt++)
-						if (dt < ndt)
-						{
-							const int dr = (dt0 + dt) * 32 + row;
-							const h8_t Ah = *(const h8_t*)(sc + ((size_t)dr) * kChunk + khalf * 8);
-							const h8_t Al = *(const h8_t*)(sc + ((size_t)DP + dr) * kChunk + khalf * 8);
-							acc[0][dt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Al, Bh0, acc[0][dt], 0, 0, 0);
-							acc[0][dt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bl0, acc[0][dt], 0, 0, 0);
-							acc[0][dt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bh0, acc[0][dt], 0, 0, 0);
-							acc[1][dt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Al, Bh1, acc[1][dt], 0, 0, 0);
-							acc[1][dt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bl1, acc[1][dt], 0, 0, 0);
-							acc[1][dt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bh1, acc[1][dt], 0, 0, 0);
-						}
-				}
-				else if constexpr (PREC == HDDM_LB_BF16X3)
-				{
-					uint32_t t1[8], t2[8], t3[8];
-#pragma unroll
-					for (int m = 0; m < 8; m++)
-					{
-						const float f0 = (float)b[2 * m], f1 = (float)b[2 * m + 1];
-						t1[m] = pack_b2(f0, f1);
-						const float r0 = f0 - lo_half_bf16(t1[m]), r1 = f1 - hi_half_bf16(t1[m]);
-						t2[m] = pack_b2(r0, r1);
-						t3[m] = pack_b2(r0 - lo_half_bf16(t2[m]), r1 - hi_half_bf16(t2[m]));
-					}
-#pragma unroll
-					for (int m = 0; m < 4; m++)
-					{
-						auto r1 = __builtin_amdgcn_permlane32_swap(t1[m], t1[m + 4], false, false);
-						t1[m] = r1[0]; t1[m + 4] = r1[1];
-						auto r2 = __builtin_amdgcn_permlane32_swap(t2[m], t2[m + 4], false, false);
-						t2[m] = r2[0]; t2[m + 4] = r2[1];
-						auto r3 = __builtin_amdgcn_permlane32_swap(t3[m], t3[m + 4], false, false);
-						t3[m] = r3[0]; t3[m + 4] = r3[1];
-					}
-					b8_t B[2][3];
-					B[0][0] = as_vec8<b8_t>(t1[0], t1[1], t1[2], t1[3]);
-					B[1][0] = as_vec8<b8_t>(t1[4], t1[5], t1[6], t1[7]);
-					B[0][1] = as_vec8<b8_t>(t2[0], t2[1], t2[2], t2[3]);
-					B[1][1] = as_vec8<b8_t>(t2[4], t2[5], t2[6], t2[7]);
-					B[0][2] = as_vec8<b8_t>(t3[0], t3[1], t3[2], t3[3]);
-					B[1][2] = as_vec8<b8_t>(t3[4], t3[5], t3[6], t3[7]);
-					const uint16_t* sc = ldc(&sd->sbf) + (size_t)chunk * 3 * DP * kChunk;
-#pragma unroll
-					for (int dt = 0; dt < DT; dt++)
-						if (dt < ndt)
-						{
-							const int dr = (dt0 + dt) * 32 + row;
-							b8_t A[3];
-#pragma unroll
-							for (int term = 0; term < 3; term++)
-								A[term] = *(const b8_t*)(sc + ((size_t)term * DP + dr) * kChunk + khalf * 8);
-#pragma unroll
-							for (int t = 0; t < 2; t++)
-							{
-								// smallest products first
-								acc[t][dt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[2], B[t][0], acc[t][dt], 0, 0, 0);
-								acc[t][dt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[1], B[t][1], acc[t][dt], 0, 0, 0);
-								acc[t][dt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[0], B[t][2], acc[t][dt], 0, 0, 0);
-								acc[t][dt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[1], B[t][0], acc[t][dt], 0, 0, 0);
-								acc[t][dt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[0], B[t][1], acc[t][dt], 0, 0, 0);
-								acc[t][dt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[0], B[t][0], acc[t][dt], 0, 0, 0);
-							}
-						}
-				}
-				else  // HDDM_LB_F32
-				{
-					float B0[8], B1[8];
-#pragma unroll
-					for (int q = 0; q < 8; q++)
-					{
-						auto r = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, (float)b[2 * q]),
-							__builtin_bit_cast(uint32_t, (float)b[2 * q + 1]), false, false);
-						B0[q] = __builtin_bit_cast(float, (uint32_t)r[0]);
-						B1[q] = __builtin_bit_cast(float, (uint32_t)r[1]);
-					}
-					const float* sc = ldc(&sd->s32) + (size_t)chunk * DP * kChunk;
-#pragma unroll
-					for (int dt = 0; dt < DT; dt++)
-						if (dt < ndt)
-						{
-							const float* ap = sc + (((size_t)((dt0 + dt) * 32 + row)) * 2 + khalf) * 8;
-							float A[8];
-#pragma unroll
-							for (int q = 0; q < 8; q++) A[q] = ap[q];
-#pragma unroll
-							for (int q = 0; q < 8; q++)
-							{
-								acc[0][dt] = __builtin_amdgcn_mfma_f32_32x32x2f32(A[q], B0[q], acc[0][dt], 0, 0, 0);
-								acc[1][dt] = __builtin_amdgcn_mfma_f32_32x32x2f32(A[q], B1[q], acc[1][dt], 0, 0, 0);
-							}
-						}
-				}
+				if (!more) break;
+				w.start_chunk(node, val, lane);
+				// issue the surplus operands of the new chunk now; they are
+				// consumed after the walk has finished the chunk
+				so.load(sd, w.chunk, DP, dt0, ndt, lane);
 			}
 #pragma unroll
 			for (int t = 0; t < 2; t++)
@@ -482,14 +596,15 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 			// C layout of v_mfma_f32_32x32x*: register r of lane l holds
 			// row 8*(r/4) + 4*(l/32) + r%4 (the dof) and column l%32 (the point).
 			const double* iscale = ldc(&sd->iscale);
+			const int row = lane & 31, khalf = lane >> 5;
 #pragma unroll
 			for (int t = 0; t < 2; t++)
 			{
-				const int q = 32 * t + row;
-				const uint32_t oq = (uint32_t)__shfl((int)orig, q, 64);
-				if (q < it.np)
+				const int qp = 32 * t + row;
+				const uint32_t oq = (uint32_t)__shfl((int)orig, qp, 64);
+				if (qp < it.np)
 				{
-					double* out = split ? part + ((int64_t)it.seg * it.count + it.blk * kWave + q) * DP
+					double* out = split ? part + ((int64_t)it.seg * it.count + it.blk * kWave + qp) * DP
 						: a.y + (int64_t)oq * a.ldy;
 #pragma unroll
 					for (int dt = 0; dt < DT; dt++)
@@ -510,7 +625,20 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 				}
 			}
 		}
+		if (a.stats && lane == 0)
+		{
+			atomicAdd(&a.stats[0], (unsigned long long)w.nops);
+			atomicAdd(&a.stats[1], (unsigned long long)w.nprunes);
+			atomicAdd(&a.stats[2], (unsigned long long)stat_chunks);
+			atomicAdd(&a.stats[3], 1ull);
+		}
 	}
+}
+
+size_t sweep_lds_bytes(int waves, int depth)
+{
+	const int pm = depth < 8 ? 8 : 16;
+	return (size_t)waves * ((size_t)pm + kChunk) * kWave * 8;
 }
 
 __global__ void reset_counter_kernel(PlanHeader* hdr)
@@ -540,40 +668,45 @@ __global__ void reduce_partials_kernel(const StateDesc* __restrict__ states, con
 
 // ---- launchers ----
 
-template <int PREC, int DT, int FMAX>
-static const void* sweep_ptr() { return (const void*)&sweep_kernel<PREC, DT, FMAX>; }
+template <int PREC, int DT, int XB, int PM>
+static const void* sweep_ptr() { return (const void*)&sweep_kernel<PREC, DT, XB, PM>; }
 
-const void* sweep_kernel_ptr(int prec, int dt, int fmax)
+template <int PREC, int DT, int XB>
+static const void* sweep_pm(int depth)
 {
-#define HDDM_FM(P, D) \
-	switch (fmax) { \
-	case 1: case 2: return sweep_ptr<P, D, 2>(); \
-	case 3: return sweep_ptr<P, D, 3>(); \
-	case 4: return sweep_ptr<P, D, 4>(); \
-	case 5: case 6: return sweep_ptr<P, D, 6>(); \
-	case 7: case 8: return sweep_ptr<P, D, 8>(); \
-	default: return sweep_ptr<P, D, 15>(); }
-#define HDDM_DT(P) \
-	if (dt == 1) { HDDM_FM(P, 1) } else { HDDM_FM(P, 2) }
+	return depth < 8 ? sweep_ptr<PREC, DT, XB, 8>() : sweep_ptr<PREC, DT, XB, 16>();
+}
+
+template <int PREC, int DT>
+static const void* sweep_xb(int xbanks, int depth)
+{
+	switch (xbanks)
+	{
+	case 1: return sweep_pm<PREC, DT, 1>(depth);
+	case 2: return sweep_pm<PREC, DT, 2>(depth);
+	case 3: case 4: return sweep_pm<PREC, DT, 4>(depth);
+	default: return nullptr;
+	}
+}
+
+const void* sweep_kernel_ptr(int prec, int dt, int xbanks, int depth)
+{
 	switch (prec)
 	{
-	case HDDM_LB_FP64: HDDM_DT(HDDM_LB_FP64)
-	case HDDM_LB_F16X2: HDDM_DT(HDDM_LB_F16X2)
-	case HDDM_LB_BF16X3: HDDM_DT(HDDM_LB_BF16X3)
-	case HDDM_LB_F32: HDDM_DT(HDDM_LB_F32)
+	case HDDM_LB_FP64: return sweep_xb<HDDM_LB_FP64, 1>(xbanks, depth);
+	case HDDM_LB_F16X2: return dt == 1 ? sweep_xb<HDDM_LB_F16X2, 1>(xbanks, depth) : sweep_xb<HDDM_LB_F16X2, 2>(xbanks, depth);
+	case HDDM_LB_BF16X3: return dt == 1 ? sweep_xb<HDDM_LB_BF16X3, 1>(xbanks, depth) : sweep_xb<HDDM_LB_BF16X3, 2>(xbanks, depth);
+	case HDDM_LB_F32: return dt == 1 ? sweep_xb<HDDM_LB_F32, 1>(xbanks, depth) : sweep_xb<HDDM_LB_F32, 2>(xbanks, depth);
 	}
-#undef HDDM_DT
-#undef HDDM_FM
 	return nullptr;
 }
 
-int fmax_bucket(int fmax)
+int xbanks_for(int dim)
 {
-	if (fmax <= 2) return 2;
-	if (fmax <= 4) return fmax;
-	if (fmax <= 6) return 6;
-	if (fmax <= 8) return 8;
-	return 15;
+	if (dim <= 16) return 1;
+	if (dim <= 32) return 2;
+	if (dim <= 64) return 4;
+	return 0;
 }
 
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st)

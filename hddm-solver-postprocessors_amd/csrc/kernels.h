@@ -1,7 +1,7 @@
 // Host-visible declarations of the HIP kernels and their launchers.
 #pragma once
 
-#include <hip/hip_runtime.h>
+#include <hip/hip_runtime_api.h>
 #include <stdint.h>
 
 #include "hddm_lb.h"
@@ -23,13 +23,17 @@ struct SweepArgs
 	double* partial;           // node-segment partials of split states
 	int nstates;
 	int dim;
-	int wlog;                  // log2 of the FP64 flush window, in 16-node chunks
+	int wlog;                  // log2 of the FP32 accumulation window, in 16-node chunks
 	int dt_base;               // first dof tile of 32 handled by this pass
+	unsigned long long* stats; // optional: ops evaluated, prunes, chunks contracted, items
 };
 
-// Kernel symbol of the sweep instantiation for (precision, dof tiles of 32, max factors).
-const void* sweep_kernel_ptr(int prec, int dt, int fmax);
-int fmax_bucket(int fmax);
+// Kernel symbol of the sweep instantiation for (precision, dof tiles of 32
+// per pass, register banks of 16 coordinates, max node depth); NULL when
+// unsupported. FP64 always runs one dof tile per pass.
+const void* sweep_kernel_ptr(int prec, int dt, int xbanks, int depth);
+// Register banks needed for dim (0: dim above 64 is unsupported).
+int xbanks_for(int dim);
 
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st);
 hipError_t launch_bucket_key(int64_t n, const double* x, int64_t ldx, const int32_t* sid,
@@ -39,6 +43,8 @@ hipError_t launch_scan_plan(const uint32_t* hist, uint32_t* cursor, int64_t nbuc
 hipError_t launch_scatter(int64_t n, const uint32_t* bkey, uint32_t* cursor, uint32_t* perm, hipStream_t st);
 hipError_t launch_reset_counter(PlanHeader* hdr, hipStream_t st);
 hipError_t launch_sweep(const void* kern, int grid, int waves, size_t lds, const SweepArgs& args, hipStream_t st);
+// Dynamic LDS of the sweep: per wave the path stack and the chunk buffer.
+size_t sweep_lds_bytes(int waves, int depth);
 hipError_t launch_reduce(const StateDesc* states, const StatePlan* plans, const uint32_t* perm,
 	const double* partial, double* y, int64_t ldy, int nstates, hipStream_t st);
 

@@ -15,30 +15,31 @@ constexpr int kXRowBytes = kWave * 8;
 // Bits of the spatial sort key below the state id.
 constexpr int kMortonBits = 40;
 
-// One unique hat factor phi(x_dim; i, j) = 1 - |x_dim * i - j|.
-struct alignas(32) Factor
+// One factor evaluation of the prefix-trie walk ("op"), 16 bytes, read by
+// one scalar load. The ops of all nodes are stored in trie order; node n
+// contributes one op per factor it does not share with node n-1.
+//   dim    dimension of x the factor reads
+//   ctrl   depth t (bits 0-3) | last factor of its node (bit 4) | node index << 5
+//   ihi    high word of the FP64 value of i (the low word is 0)
+//   njhi   high word of the FP64 value of -j (the low word is 0)
+// The walk computes P[t+1] = P[t] * max(0, 1 - |x_dim * i - j|). When P[t+1]
+// is zero on every lane, the run of nodes sharing the node's first t+1
+// factors is skipped: skip[op] is the first op after that run.
+struct Op
 {
-	double i;        // 0 or 2^(l-1), any integer in general
-	double negj;     // -j
-	uint32_t xoff;   // dim * kXRowBytes: byte offset of x_dim in a wave's LDS stage
-	uint32_t dim;
-	uint32_t lomask_bit;  // reserved
-	uint32_t pad;
+	uint32_t dim, ctrl, ihi, njhi;
 };
+constexpr uint32_t kOpLast = 16u;
+constexpr int kOpNodeShift = 5;
 
-// Node record in prefix-trie order, stride rec_words 32-bit words:
-//   word 0: F (active factors) | s (factors shared with the previous node) << 8
-//   word 1: unused
-//   words 2 + 2t, 3 + 2t (t < fmax): factor id of depth t, and the index of the
-//           first node after the subtree that shares this node's first t+1 factors.
 struct StateDesc
 {
 	int dim, nno, dofs, dofs_pad;   // dofs_pad: dofs rounded up to 32
-	int fmax, rec_words, nchunks;
-	int seg_len, nseg;              // node segments: [seg*seg_len, (seg+1)*seg_len)
-	int pad0;
-	const uint32_t* rec;            // [nno * rec_words]
-	const Factor* fac;              // unique factors
+	int nops, nchunks;
+	int nseg, max_depth;            // node segments (work-item split), max factors per node
+	const Op* ops;                  // [nops + 1]
+	const uint32_t* skip;           // [nops]
+	const int32_t* seg_op;          // [nseg + 1] first op of each segment
 	const double* s64;              // [nno * dofs_pad] FP64 rows, trie order
 	const uint16_t* s16;            // [nchunks][2][dofs_pad][16] f16 hi, lo (scaled)
 	const uint16_t* sbf;            // [nchunks][3][dofs_pad][16] bf16 terms

@@ -68,6 +68,39 @@ double bf16_value(uint16_t b)
 
 } // namespace
 
+namespace {
+
+// High 32 bits of the FP64 value of v; requires the low 32 bits to be zero,
+// which holds for every integer of magnitude below 2^21.
+bool f64_high_word(int64_t v, uint32_t& hi)
+{
+	double d = (double)v;
+	uint64_t u;
+	std::memcpy(&u, &d, 8);
+	hi = (uint32_t)(u >> 32);
+	return (uint32_t)u == 0u;
+}
+
+} // namespace
+
+std::vector<int32_t> segment_nodes(const HostLayout& L, int64_t seg_len)
+{
+	std::vector<int32_t> b;
+	b.push_back(0);
+	if (seg_len > 0 && seg_len < L.nno)
+	{
+		int64_t last = 0;
+		for (int n = 1; n < L.nno; n++)
+			if (n - last >= seg_len && L.shared[n] == 0)
+			{
+				b.push_back(n);
+				last = n;
+			}
+	}
+	b.push_back(L.nno);
+	return b;
+}
+
 int build_layout(int dim, int nno, int dofs,
 	const int32_t* index_i, const int32_t* index_j, int64_t ld_index,
 	const double* surplus, int64_t ld_surplus, HostLayout& L)
@@ -75,104 +108,105 @@ int build_layout(int dim, int nno, int dofs,
 	if (dim < 1 || nno < 1 || dofs < 1 || ld_index < dim || ld_surplus < dofs || !index_i || !index_j || !surplus)
 		return set_error(HDDM_LB_ERROR_INVALID_VALUE,
 			"upload_state: need dim, nno, dofs >= 1, ld_index >= dim, ld_surplus >= dofs and non-NULL arrays");
-	if ((int64_t)dim * kXRowBytes > (int64_t)UINT32_MAX)
-		return set_error(HDDM_LB_ERROR_UNSUPPORTED, "upload_state: dim %d too large", dim);
+	if (nno >= (1 << (32 - kOpNodeShift)))
+		return set_error(HDDM_LB_ERROR_UNSUPPORTED, "upload_state: nno %d exceeds %d", nno, 1 << (32 - kOpNodeShift));
 
 	L.dim = dim; L.nno = nno; L.dofs = dofs;
 	L.dofs_pad = (dofs + 31) / 32 * 32;
 	L.nchunks = (nno + kChunk - 1) / kChunk;
 
-	// Unique factors, ids ordered by (dim, i, j) so id order is factor order.
-	std::map<FactorKey, int> ids;
+	// Active factors of every node in ascending dim order (the order in which
+	// the gold kernel multiplies them), as (dim, i, j) triples.
+	std::vector<std::vector<FactorKey> > seq(nno);
 	int fmax = 0;
 	for (int k = 0; k < nno; k++)
 	{
-		int f = 0;
 		for (int d = 0; d < dim; d++)
 		{
 			int32_t i = index_i[(int64_t)k * ld_index + d], j = index_j[(int64_t)k * ld_index + d];
 			if (!i && !j) continue;
-			ids.emplace(FactorKey{ d, i, j }, 0);
-			f++;
+			uint32_t h;
+			if (!f64_high_word(i, h) || !f64_high_word(-(int64_t)j, h))
+				return set_error(HDDM_LB_ERROR_UNSUPPORTED,
+					"upload_state: node %d dim %d has (i, j) = (%d, %d); |i|, |j| < 2^21 are supported", k, d, i, j);
+			seq[k].push_back(FactorKey{ d, i, j });
 		}
-		fmax = std::max(fmax, f);
+		fmax = std::max(fmax, (int)seq[k].size());
 	}
 	if (fmax > 15)
 		return set_error(HDDM_LB_ERROR_UNSUPPORTED,
 			"upload_state: a node has %d active dims; at most 15 are supported", fmax);
 	L.fmax = std::max(fmax, 1);
-	L.rec_words = 2 + 2 * L.fmax;
-	L.fac.clear();
-	for (auto& kv : ids)
-	{
-		kv.second = (int)L.fac.size();
-		Factor f;
-		std::memset(&f, 0, sizeof(f));
-		f.i = (double)kv.first.i;
-		f.negj = -(double)kv.first.j;
-		f.dim = (uint32_t)kv.first.d;
-		f.xoff = (uint32_t)kv.first.d * (uint32_t)kXRowBytes;
-		L.fac.push_back(f);
-	}
 
-	// Factor-id sequence of every node.
-	std::vector<int32_t> seq((size_t)nno * L.fmax, -1);
-	std::vector<int> nf(nno, 0);
-	for (int k = 0; k < nno; k++)
-		for (int d = 0; d < dim; d++)
-		{
-			int32_t i = index_i[(int64_t)k * ld_index + d], j = index_j[(int64_t)k * ld_index + d];
-			if (!i && !j) continue;
-			seq[(size_t)k * L.fmax + nf[k]++] = ids[FactorKey{ d, i, j }];
-		}
-
-	// Lexicographic order of the sequences (a prefix sorts first).
+	// Lexicographic order of the factor sequences (a prefix sorts first).
 	L.order.resize(nno);
 	for (int k = 0; k < nno; k++) L.order[k] = k;
-	const int FM = L.fmax;
 	std::stable_sort(L.order.begin(), L.order.end(), [&](int a, int b) {
-		const int32_t* sa = &seq[(size_t)a * FM];
-		const int32_t* sb = &seq[(size_t)b * FM];
-		for (int t = 0; t < FM; t++)
-		{
-			if (t >= nf[a] || t >= nf[b]) return nf[a] < nf[b];
-			if (sa[t] != sb[t]) return sa[t] < sb[t];
-		}
-		return false;
+		return std::lexicographical_compare(seq[a].begin(), seq[a].end(), seq[b].begin(), seq[b].end());
 	});
 
 	// Shared-prefix lengths and subtree ends.
 	std::vector<int> F(nno), S(nno);
+	L.shared.assign(nno, 0);
 	for (int n = 0; n < nno; n++)
 	{
-		int k = L.order[n];
-		F[n] = nf[k];
+		const auto& cur = seq[L.order[n]];
+		F[n] = (int)cur.size();
 		int s = 0;
 		if (n > 0)
 		{
-			int kp = L.order[n - 1];
-			while (s < nf[k] && s < nf[kp] && seq[(size_t)k * FM + s] == seq[(size_t)kp * FM + s]) s++;
+			const auto& prev = seq[L.order[n - 1]];
+			while (s < (int)cur.size() && s < (int)prev.size() &&
+				!(cur[s] < prev[s]) && !(prev[s] < cur[s])) s++;
 		}
 		S[n] = s;
+		L.shared[n] = (uint8_t)s;
 	}
+	const int FM = L.fmax;
 	// end[n][t]: first node after n whose first t+1 factors differ from n's.
 	std::vector<int> end((size_t)nno * FM, 0);
 	for (int n = nno - 1; n >= 0; n--)
 		for (int t = 0; t < F[n]; t++)
 			end[(size_t)n * FM + t] = (n + 1 < nno && S[n + 1] >= t + 1) ? end[(size_t)(n + 1) * FM + t] : n + 1;
 
-	L.rec.assign((size_t)nno * L.rec_words, 0);
+	// Op stream: one op per factor a node does not share with its predecessor;
+	// the root (no active factor) gets one op with i = j = 0 (phi == 1).
+	L.opstart.assign(nno + 1, 0);
+	int64_t nops = 0;
 	for (int n = 0; n < nno; n++)
 	{
-		uint32_t* r = &L.rec[(size_t)n * L.rec_words];
-		r[0] = (uint32_t)F[n] | ((uint32_t)S[n] << 8);
-		int k = L.order[n];
-		for (int t = 0; t < F[n]; t++)
+		L.opstart[n] = (int32_t)nops;
+		nops += std::max(1, F[n] - S[n]);
+	}
+	L.opstart[nno] = (int32_t)nops;
+	if (nops >= ((int64_t)1 << 31))
+		return set_error(HDDM_LB_ERROR_UNSUPPORTED, "upload_state: too many trie edges (%lld)", (long long)nops);
+	L.ops.assign(nops + 1, Op{ 0, 0, 0, 0 });
+	L.skip.assign(nops + 1, 0);
+	for (int n = 0; n < nno; n++)
+	{
+		const auto& cur = seq[L.order[n]];
+		int64_t q = L.opstart[n];
+		if (F[n] == 0)
 		{
-			r[2 + 2 * t] = (uint32_t)seq[(size_t)k * FM + t];
-			r[3 + 2 * t] = (uint32_t)end[(size_t)n * FM + t];
+			L.ops[q] = Op{ 0u, kOpLast | ((uint32_t)n << kOpNodeShift), 0u, 0u };
+			L.skip[q] = (uint32_t)L.opstart[n + 1];
+			continue;
+		}
+		for (int t = S[n]; t < F[n]; t++, q++)
+		{
+			Op op;
+			op.dim = (uint32_t)cur[t].d;
+			op.ctrl = (uint32_t)t | (t == F[n] - 1 ? kOpLast : 0u) | ((uint32_t)n << kOpNodeShift);
+			f64_high_word(cur[t].i, op.ihi);
+			f64_high_word(-(int64_t)cur[t].j, op.njhi);
+			L.ops[q] = op;
+			L.skip[q] = (uint32_t)L.opstart[end[(size_t)n * FM + t]];
 		}
 	}
+	// Sentinels (never evaluated) keep a one-op read-ahead in bounds.
+	L.ops[nops] = Op{ 0u, 0u, 0u, 0u };
+	L.skip[nops] = (uint32_t)nops;
 
 	// Support masks: bit d of lomask (himask) when the node's support in dim d
 	// lies in [0, 1/2] ([1/2, 1]); dims >= 64 are left out (conservative).
@@ -263,32 +297,24 @@ extern "C" int hddm_lb_layout_build(int dim, int nno, int dofs,
 	return HDDM_LB_SUCCESS;
 }
 
-extern "C" int hddm_lb_layout_info(const hddm_lb_layout* l, int* fmax, int* rec_words,
-	int* nfactors, int* nchunks, int* dofs_pad)
+extern "C" int hddm_lb_layout_info(const hddm_lb_layout* l, int* max_factors, int* nops,
+	int* nchunks, int* dofs_pad)
 {
 	if (!l) return hddm_lb::set_error(HDDM_LB_ERROR_INVALID_VALUE, "layout_info: NULL layout");
-	if (fmax) *fmax = l->L.fmax;
-	if (rec_words) *rec_words = l->L.rec_words;
-	if (nfactors) *nfactors = (int)l->L.fac.size();
+	if (max_factors) *max_factors = l->L.fmax;
+	if (nops) *nops = (int)l->L.skip.size() - 1;
 	if (nchunks) *nchunks = l->L.nchunks;
 	if (dofs_pad) *dofs_pad = l->L.dofs_pad;
 	return HDDM_LB_SUCCESS;
 }
 
-extern "C" int hddm_lb_layout_records(const hddm_lb_layout* l, uint32_t* rec)
+extern "C" int hddm_lb_layout_ops(const hddm_lb_layout* l, uint32_t* ops, uint32_t* skip, int32_t* opstart)
 {
-	if (!l || !rec) return hddm_lb::set_error(HDDM_LB_ERROR_INVALID_VALUE, "layout_records: NULL argument");
-	std::memcpy(rec, l->L.rec.data(), l->L.rec.size() * sizeof(uint32_t));
-	return HDDM_LB_SUCCESS;
-}
-
-extern "C" int hddm_lb_layout_factors(const hddm_lb_layout* l, double* i, double* negj, int32_t* dim)
-{
-	if (!l || !i || !negj || !dim) return hddm_lb::set_error(HDDM_LB_ERROR_INVALID_VALUE, "layout_factors: NULL argument");
-	for (size_t f = 0; f < l->L.fac.size(); f++)
-	{
-		i[f] = l->L.fac[f].i; negj[f] = l->L.fac[f].negj; dim[f] = (int32_t)l->L.fac[f].dim;
-	}
+	if (!l || !ops || !skip || !opstart) return hddm_lb::set_error(HDDM_LB_ERROR_INVALID_VALUE, "layout_ops: NULL argument");
+	const size_t nops = l->L.skip.size() - 1;
+	std::memcpy(ops, l->L.ops.data(), nops * sizeof(hddm_lb::Op));
+	std::memcpy(skip, l->L.skip.data(), nops * sizeof(uint32_t));
+	std::memcpy(opstart, l->L.opstart.data(), l->L.opstart.size() * sizeof(int32_t));
 	return HDDM_LB_SUCCESS;
 }
 

@@ -10,9 +10,11 @@ namespace hddm_lb {
 
 struct HostLayout
 {
-	int dim = 0, nno = 0, dofs = 0, dofs_pad = 0, fmax = 0, rec_words = 0, nchunks = 0;
-	std::vector<uint32_t> rec;        // [nno * rec_words]
-	std::vector<Factor> fac;          // unique factors, ordered by (dim, i, j)
+	int dim = 0, nno = 0, dofs = 0, dofs_pad = 0, fmax = 0, nchunks = 0;
+	std::vector<Op> ops;              // [nops + 1], trie order, sentinel last
+	std::vector<uint32_t> skip;       // [nops]
+	std::vector<int32_t> opstart;     // [nno + 1] first op of each node
+	std::vector<uint8_t> shared;      // [nno] factors shared with the previous node
 	std::vector<double> s64;          // [nno * dofs_pad]
 	std::vector<uint16_t> s16;        // [nchunks][2][dofs_pad][16]
 	std::vector<uint16_t> sbf;        // [nchunks][3][dofs_pad][16]
@@ -21,6 +23,10 @@ struct HostLayout
 	std::vector<int32_t> order;       // trie position -> uploaded node index
 	std::vector<uint64_t> lomask, himask;  // [nno]
 };
+
+// Segment boundaries for a target length: nodes where a new top-level run
+// starts (shared prefix 0), so every segment's first op is at depth 0.
+std::vector<int32_t> segment_nodes(const HostLayout& L, int64_t seg_len);
 
 // Builds the prefix-trie layout. Returns an HDDM_LB_* status.
 int build_layout(int dim, int nno, int dofs,

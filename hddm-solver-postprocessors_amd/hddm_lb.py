@@ -26,7 +26,7 @@ FP64, F16X2, BF16X3, F32 = 0, 1, 2, 3
 PRECISIONS = {"fp64": FP64, "f16x2": F16X2, "bf16x3": BF16X3, "f32": F32}
 
 OPT_AUTOTUNE, OPT_SEGMENT_NODES, OPT_WAVES_PER_WG, OPT_SORT_POINTS = 1, 2, 3, 4
-OPT_TIMING, OPT_SLAB_TILES, OPT_FLUSH_LOG2 = 5, 6, 7
+OPT_TIMING, OPT_SLAB_TILES, OPT_FLUSH_LOG2, OPT_STATS = 5, 6, 7, 8
 
 SUCCESS = 0
 ERROR_NAMES = {1: "INVALID_VALUE", 2: "NO_DEVICE", 3: "OUT_OF_MEMORY", 4: "HIP",
@@ -41,6 +41,23 @@ class HddmLbError(RuntimeError):
 
 if not os.path.exists(LIB_PATH):
     raise ImportError("libhddm_lb.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+
+
+def _load_hip_runtime():
+    """Makes one HIP runtime global before the engine loads (the engine has no
+    NEEDED entry for one): torch's bundled runtime when torch is importable, so
+    engine kernels and torch tensors share a runtime; else the system ROCm."""
+    try:
+        import torch
+        cand = os.path.join(os.path.dirname(torch.__file__), "lib", "libamdhip64.so")
+        if os.path.exists(cand):
+            return ctypes.CDLL(cand, mode=ctypes.RTLD_GLOBAL)
+    except ImportError:
+        pass
+    return ctypes.CDLL("libamdhip64.so.7", mode=ctypes.RTLD_GLOBAL)
+
+
+HIP_RUNTIME = _load_hip_runtime()
 lib = ctypes.CDLL(LIB_PATH)
 
 _c = ctypes
@@ -64,6 +81,7 @@ _SIGS = {
     "hddm_lb_plan_launch": (_i32, [_vp, _vp]),
     "hddm_lb_plan_destroy": (_i32, [_vp]),
     "hddm_lb_last_sweep_ms": (_dbl, [_vp]),
+    "hddm_lb_last_stats": (_i32, [_vp, _vp]),
     "hddm_lb_plt_read": (_i32, [_c.c_char_p, _i32, _c.POINTER(_i32), _c.POINTER(_i32), _c.POINTER(_i32),
                                 _c.POINTER(_i32), _c.POINTER(_I32P), _c.POINTER(_I32P), _c.POINTER(_DP)]),
     "hddm_lb_plt_write_compressed": (_i32, [_c.c_char_p, _i32, _i32, _i32, _i32, _vp, _vp, _vp]),
@@ -75,9 +93,8 @@ _SIGS = {
     "hddm_lb_grid_levels": (_i32, [_vp, _vp, _vp]),
     "hddm_lb_grid_free": (_i32, [_vp]),
     "hddm_lb_layout_build": (_i32, [_i32, _i32, _i32, _vp, _vp, _i64, _vp, _i64, _c.POINTER(_vp)]),
-    "hddm_lb_layout_info": (_i32, [_vp] + [_c.POINTER(_i32)] * 5),
-    "hddm_lb_layout_records": (_i32, [_vp, _vp]),
-    "hddm_lb_layout_factors": (_i32, [_vp, _vp, _vp, _vp]),
+    "hddm_lb_layout_info": (_i32, [_vp] + [_c.POINTER(_i32)] * 4),
+    "hddm_lb_layout_ops": (_i32, [_vp, _vp, _vp, _vp]),
     "hddm_lb_layout_order": (_i32, [_vp, _vp]),
     "hddm_lb_layout_free": (_i32, [_vp]),
 }
@@ -193,7 +210,8 @@ def write_plt_uncompressed(path, levels, indices, surplus, binary=True, level=0)
 # ---------------------------------------------------------------- layout
 
 class Layout:
-    """Host-side prefix-trie layout of a state (no device needed)."""
+    """Host-side prefix-trie layout of a state (no device needed): nodes in
+    trie order and the op stream of the walk (see include/hddm_lb.h)."""
 
     def __init__(self, index_i, index_j, surplus):
         ii = np.ascontiguousarray(index_i, np.int32)
@@ -204,15 +222,20 @@ class Layout:
         h = _vp()
         _check(lib.hddm_lb_layout_build(dim, nno, self.dofs, _ptr(ii), _ptr(jj), dim, _ptr(ss), self.dofs, _c.byref(h)))
         self._h = h
-        vals = [_i32() for _ in range(5)]
+        vals = [_i32() for _ in range(4)]
         _check(lib.hddm_lb_layout_info(h, *[_c.byref(v) for v in vals]))
-        self.fmax, self.rec_words, self.nfactors, self.nchunks, self.dofs_pad = [v.value for v in vals]
-        self.records = np.zeros((nno, self.rec_words), np.uint32)
-        _check(lib.hddm_lb_layout_records(h, _ptr(self.records)))
-        self.fac_i = np.zeros(self.nfactors, np.float64)
-        self.fac_negj = np.zeros(self.nfactors, np.float64)
-        self.fac_dim = np.zeros(self.nfactors, np.int32)
-        _check(lib.hddm_lb_layout_factors(h, _ptr(self.fac_i), _ptr(self.fac_negj), _ptr(self.fac_dim)))
+        self.fmax, self.nops, self.nchunks, self.dofs_pad = [v.value for v in vals]
+        ops = np.zeros((self.nops, 4), np.uint32)
+        self.skip = np.zeros(self.nops, np.uint32)
+        self.opstart = np.zeros(nno + 1, np.int32)
+        _check(lib.hddm_lb_layout_ops(h, _ptr(ops), _ptr(self.skip), _ptr(self.opstart)))
+        self.op_dim = ops[:, 0].astype(np.int64)
+        ctrl = ops[:, 1]
+        self.op_depth = (ctrl & 15).astype(np.int64)
+        self.op_last = (ctrl >> 4) & 1 == 1
+        self.op_node = (ctrl >> 5).astype(np.int64)
+        self.op_i = (ops[:, 2].astype(np.uint64) << np.uint64(32)).view(np.float64)
+        self.op_negj = (ops[:, 3].astype(np.uint64) << np.uint64(32)).view(np.float64)
         self.order = np.zeros(nno, np.int32)
         _check(lib.hddm_lb_layout_order(h, _ptr(self.order)))
 
@@ -296,6 +319,12 @@ class Context:
 
     def last_sweep_ms(self):
         return lib.hddm_lb_last_sweep_ms(self._h)
+
+    def last_stats(self):
+        """(ops evaluated, subtrees pruned, chunks contracted, work items) of the last forward."""
+        st = np.zeros(4, np.uint64)
+        _check(lib.hddm_lb_last_stats(self._h, _ptr(st)))
+        return tuple(int(v) for v in st)
 
     def plan(self, x, y, state_ids=None, precision=FP64, stream=None):
         return Plan(self, x, y, state_ids, precision, stream)

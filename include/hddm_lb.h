@@ -56,11 +56,12 @@ typedef struct hddm_lb_grid hddm_lb_grid;
 /* options for hddm_lb_set_option */
 #define HDDM_LB_OPT_AUTOTUNE      1  /* 1: time tile candidates on first use of a (D, nno, dofs) key */
 #define HDDM_LB_OPT_SEGMENT_NODES 2  /* node-range length of one work item (0: automatic) */
-#define HDDM_LB_OPT_WAVES_PER_WG  3  /* waves per workgroup (1, 2, 4, 8; 0: automatic) */
+#define HDDM_LB_OPT_WAVES_PER_WG  3  /* waves per workgroup (1, 2, 4; 0: automatic) */
 #define HDDM_LB_OPT_SORT_POINTS   4  /* 1 (default): spatially sort points before the sweep */
 #define HDDM_LB_OPT_TIMING        5  /* 1: time the sweep kernels with hipEvents (hddm_lb_last_sweep_ms) */
 #define HDDM_LB_OPT_SLAB_TILES    6  /* dof tiles of 32 per sweep pass (1, 2; 0: automatic) */
 #define HDDM_LB_OPT_FLUSH_LOG2    7  /* log2 of the FP32->FP64 flush window in 16-node chunks (-1: default 3) */
+#define HDDM_LB_OPT_STATS         8  /* 1: count walk ops, prunes, contracted chunks and items (hddm_lb_last_stats) */
 
 /* ---- devices and contexts ---- */
 
@@ -132,20 +133,28 @@ int hddm_lb_plan_destroy(hddm_lb_plan* plan);
  * on this context, when HDDM_LB_OPT_TIMING is set; -1 otherwise. Waits for
  * the second event. */
 double hddm_lb_last_sweep_ms(hddm_lb_ctx* ctx);
+/* With HDDM_LB_OPT_STATS set: counts of the last forward, summed over all
+ * work items: stats[0] ops evaluated, [1] subtrees pruned, [2] 16-node chunks
+ * contracted, [3] work items. Waits for the forward to finish. */
+int hddm_lb_last_stats(hddm_lb_ctx* ctx, uint64_t* stats);
 
 /* ---- host-side node layout, for inspection without a device ---- */
 typedef struct hddm_lb_layout hddm_lb_layout;
-/* Builds the prefix-trie layout hddm_lb_upload_state would upload. Node record
- * n (trie position) holds rec_words 32-bit words: word 0 = F | s << 8 (active
- * factors, factors shared with record n-1), then for depth t < F the factor id
- * and the first record after the run of records sharing the first t+1 factors. */
+/* Builds the layout hddm_lb_upload_state would upload: nodes in prefix-trie
+ * order (lexicographic by their ascending-dim factor sequences) and the op
+ * stream of the walk, one op per factor a node does not share with its
+ * predecessor. Op q is 4 words: dim, ctrl = depth | (last factor of its node)
+ * << 4 | node << 5, and the high words of the FP64 values of i and -j (low
+ * words 0); skip[q] is the first op after the run of nodes sharing op q's
+ * prefix; opstart[n] (n <= nno) is node n's first op. */
 int hddm_lb_layout_build(int dim, int nno, int dofs,
 	const int32_t* index_i, const int32_t* index_j, int64_t ld_index,
 	const double* surplus, int64_t ld_surplus, hddm_lb_layout** layout);
-int hddm_lb_layout_info(const hddm_lb_layout* layout, int* max_factors, int* rec_words,
-	int* nfactors, int* nchunks, int* dofs_pad);
-int hddm_lb_layout_records(const hddm_lb_layout* layout, uint32_t* records);
-int hddm_lb_layout_factors(const hddm_lb_layout* layout, double* i, double* negj, int32_t* dim);
+int hddm_lb_layout_info(const hddm_lb_layout* layout, int* max_factors, int* nops,
+	int* nchunks, int* dofs_pad);
+/* ops: 4 * nops words; skip: nops; opstart: nno + 1 */
+int hddm_lb_layout_ops(const hddm_lb_layout* layout, uint32_t* ops, uint32_t* skip, int32_t* opstart);
+/* order[n]: uploaded index of the node at trie position n */
 int hddm_lb_layout_order(const hddm_lb_layout* layout, int32_t* order);
 int hddm_lb_layout_free(hddm_lb_layout* layout);
 

@@ -139,43 +139,28 @@ def test_adaptive_grid_properties():
 
 # ------------------------------------------------------------------ layout
 
-def walk_layout(L, X):
-    """CPU emulation of one wave's trie walk (the sweep kernel's algorithm) over
-    the layout records for the points X [64, dim]: returns y [64, dofs] and the
-    fraction of records whose factors were evaluated."""
+def walk_layout(L, X, S_sorted):
+    """CPU emulation of one wave's walk over the op stream (the sweep kernel's
+    algorithm) for the points X [64, dim]: returns y [64, dofs] and the number
+    of ops evaluated."""
     n = X.shape[0]
-    stack = [np.ones(n)] + [None] * L.fmax
+    P = [np.ones(n) for _ in range(16)]
     y = np.zeros((n, L.dofs))
-    skip_end, fresh, visited = 0, True, 0
-    S = L.surplus_sorted
-    k = 0
-    while k < L.nno:
-        if k < skip_end:
-            k = skip_end
+    q, evals = 0, 0
+    while q < L.nops:
+        t = L.op_depth[q]
+        tt = X[:, L.op_dim[q]] * L.op_i[q] + L.op_negj[q]
+        p = P[t] * np.maximum(1.0 - np.abs(tt), 0.0)
+        P[t + 1] = p
+        evals += 1
+        if not p.any():
+            assert L.skip[q] > q
+            q = int(L.skip[q])
             continue
-        r = L.records[k]
-        F, sh = int(r[0] & 0xff), int((r[0] >> 8) & 0xff)
-        if fresh:
-            sh, fresh = 0, False
-        visited += 1
-        b = np.ones(n)
-        pruned = False
-        for t in range(sh, F):
-            f = int(r[2 + 2 * t])
-            tt = X[:, L.fac_dim[f]] * L.fac_i[f] + L.fac_negj[f]
-            p = stack[t] * np.maximum(1.0 - np.abs(tt), 0.0)
-            stack[t + 1] = p
-            b = p
-            if not p.any():
-                skip_end = int(r[3 + 2 * t])
-                pruned = True
-                break
-        if pruned:
-            k = skip_end
-            continue
-        y += b[:, None] * S[k][None, :]
-        k += 1
-    return y, visited / L.nno
+        if L.op_last[q]:
+            y += p[:, None] * S_sorted[L.op_node[q]][None, :]
+        q += 1
+    return y, evals
 
 
 @pytest.mark.parametrize("dim,start,level,target,seed", [(4, 4, 5, 0, 1), (8, 2, 5, 2000, 2), (16, 2, 7, 6000, 3)])
@@ -183,31 +168,31 @@ def test_layout_trie_walk_matches_oracle(dim, start, level, target, seed):
     ii, jj, lv, ix = hddm_lb.generate_grid(dim, start, level, 0.3, seed, target)
     S = hddm_lb.synthetic_surplus(lv, 5, seed)
     L = hddm_lb.Layout(ii, jj, S)
-    assert sorted(L.order.tolist()) == list(range(ii.shape[0]))
-    L.surplus_sorted = S[L.order]
-    # records: lexicographic factor sequences, correct shared-prefix lengths
-    seqs = []
-    for k in range(L.nno):
-        r = L.records[k]
-        F = int(r[0] & 0xff)
-        seqs.append(tuple(int(r[2 + 2 * t]) for t in range(F)))
-        act = np.nonzero((ii[L.order[k]] != 0) | (jj[L.order[k]] != 0))[0]
-        assert F == len(act)
-        assert [int(L.fac_dim[f]) for f in seqs[-1]] == act.tolist()
+    nno = ii.shape[0]
+    assert sorted(L.order.tolist()) == list(range(nno))
+    act = (ii != 0) | (jj != 0)
+    # factor sequences of the nodes in trie order are sorted; ops cover exactly the unshared factors
+    seqs = [tuple((d, int(ii[k, d]), int(jj[k, d])) for d in np.nonzero(act[k])[0]) for k in L.order]
     assert seqs == sorted(seqs)
-    for k in range(1, L.nno):
-        s = int((L.records[k][0] >> 8) & 0xff)
-        a, b = seqs[k - 1], seqs[k]
-        m = 0
-        while m < min(len(a), len(b)) and a[m] == b[m]:
-            m += 1
-        assert s == m
-    # subtree ends
-    for k in range(L.nno):
-        for t in range(len(seqs[k])):
-            e = int(L.records[k][3 + 2 * t])
-            assert all(seqs[q][:t + 1] == seqs[k][:t + 1] for q in range(k, e))
-            assert e == L.nno or seqs[e][:t + 1] != seqs[k][:t + 1]
+    for n in range(nno):
+        s = 0
+        if n > 0:
+            while s < min(len(seqs[n]), len(seqs[n - 1])) and seqs[n][s] == seqs[n - 1][s]:
+                s += 1
+        ops = range(L.opstart[n], L.opstart[n + 1])
+        if len(seqs[n]) == 0:
+            assert len(ops) == 1 and L.op_i[ops[0]] == 0 and L.op_last[ops[0]]
+            continue
+        assert [L.op_depth[q] for q in ops] == list(range(s, len(seqs[n])))
+        for q, t in zip(ops, range(s, len(seqs[n]))):
+            d, i, j = seqs[n][t]
+            assert L.op_dim[q] == d and L.op_i[q] == i and L.op_negj[q] == -j and L.op_node[q] == n
+            assert L.op_last[q] == (t == len(seqs[n]) - 1)
+            # skip: first op of the first node after the run sharing the first t+1 factors
+            e = n + 1
+            while e < nno and seqs[e][:t + 1] == seqs[n][:t + 1]:
+                e += 1
+            assert L.skip[q] == L.opstart[e]
     # walks with pruning reproduce the oracle on a clustered and a random block
     rng = np.random.default_rng(seed)
     X = rng.random((4096, dim))
@@ -216,7 +201,7 @@ def test_layout_trie_walk_matches_oracle(dim, start, level, target, seed):
         key = key * 2 + (X[:, d] >= 0.5)
     Xs = X[np.argsort(key, kind="stable")]
     for blk in (Xs[:64], X[:64]):
-        yw, frac = walk_layout(L, blk)
+        yw, evals = walk_layout(L, blk, S[L.order])
         yo = oracle.forward(ii, jj, S, blk)
         np.testing.assert_allclose(yw, yo, rtol=1e-12, atol=1e-12 * np.abs(yo).max())
 
@@ -237,7 +222,7 @@ def declared_functions(header):
 
 def test_library_exports_every_declared_symbol():
     names = declared_functions(os.path.join(ROOT, "include", "hddm_lb.h"))
-    assert len(names) >= 30
+    assert len(names) >= 29
     lib = ctypes.CDLL(hddm_lb.LIB_PATH)
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
