@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""Headline benchmark of the LinearBasis interpolation engine on one MI355X.
+
+Workload: single-state forward, adaptive sparse grid D=16 (seeded refinement
+p=0.3 from level sum <= 2, max level 7, stopped at 60,018 nodes), dofs=32,
+1,048,576 points x ~ U[0,1)^16, all resident in HBM before timing.
+
+A step is one forward over the whole batch. The headline path is the
+mixed-precision MFMA sweep (f16 hi/lo split operands, FP32 accumulation
+folded into FP64); the plain FP64 GPU sweep (same walk, FP64 contraction) is
+timed beside it. Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--precision f16x2]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "hddm-solver-postprocessors_amd"))
+
+METRIC = "point·dof evals/sec on 1 MI355X, mixed/MFMA vs FP64 plain at ≤1e-6 rel err"
+PEAK_TFLOPS = {"f16x2": 2500.0, "bf16x3": 2500.0, "f32": 157.3, "fp64": 78.6}
+DTYPE = {"f16x2": "f16", "bf16x3": "bf16", "f32": "f32", "fp64": "f64"}
+
+
+def build_workload(npoints):
+    import hddm_lb
+    ii, jj, lv, _ = hddm_lb.generate_grid(16, 2, 7, 0.3, 1, 60000)
+    S = hddm_lb.synthetic_surplus(lv, 32, 2)
+    X = hddm_lb.synthetic_points(npoints, 16, 3)
+    return ii, jj, S, X
+
+
+def time_path(ctx, x, y, prec, steps, warmup):
+    """Wall time per step over `steps` forwards, and the sweep kernels' mean
+    duration from hipEvents recorded on the same stream in the same steps."""
+    import torch
+    import hddm_lb
+    for _ in range(warmup):
+        ctx.forward(x, y, None, prec)
+    torch.cuda.synchronize()
+    ctx.set_option(hddm_lb.OPT_TIMING, 1)
+    ctx.sweep_time_summary()  # drop the warmup events
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.forward(x, y, None, prec)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    n, tot, mn, mx = ctx.sweep_time_summary()
+    ctx.set_option(hddm_lb.OPT_TIMING, 0)
+    return (t1 - t0) * 1e3 / steps, tot / max(n, 1), mn, mx
+
+
+def normwise_err(y, yref):
+    import numpy as np
+    den = np.abs(yref).max()
+    return float(np.abs(y - yref).max() / den) if den > 0 else float(np.abs(y - yref).max())
+
+
+def cpu_baseline(ii, jj, S, X, ygpu, target_s):
+    """The reference's x86 xps/chains algorithm (the oracle's port of it) on
+    the host cores, on a bounded prefix of the same points; also the checker
+    for the GPU results on that prefix (the port against the FP64 gold
+    restatement on its first 256 points)."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    st = oracle.X86State(ii, jj, S)
+    probe = 256
+    t0 = time.perf_counter()
+    st.forward(X[:probe], threads)
+    dt = time.perf_counter() - t0
+    m = int(min(X.shape[0], max(probe, probe * target_s / max(dt, 1e-6))))
+    t0 = time.perf_counter()
+    yc = st.forward(X[:m], threads)
+    dt = time.perf_counter() - t0
+    gold = oracle.forward(ii, jj, S, X[:probe])
+    errs = {k: normwise_err(v[:m], yc) for k, v in ygpu.items()}
+    return {
+        "value": m * S.shape[1] / dt,
+        "unit": "point·dof/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "first %d of the %d points, x86 xps/chains algorithm (FP64), OpenMP static over points, %.1f s"
+                  % (m, X.shape[0], dt),
+    }, errs, normwise_err(yc[:probe], gold)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--precision", default="f16x2", choices=["f16x2", "bf16x3", "f32"])
+    ap.add_argument("--points", type=int, default=1 << 20)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target length of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and error check")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (a forward shards by points; "
+              "run one process per GPU on its own share). Use --gpus 1.", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+    import hddm_lb
+
+    precs = {"f16x2": hddm_lb.F16X2, "bf16x3": hddm_lb.BF16X3, "f32": hddm_lb.F32, "fp64": hddm_lb.FP64}
+    ii, jj, S, X = build_workload(args.points)
+    nno, dofs, N = ii.shape[0], S.shape[1], X.shape[0]
+    factors = int(np.count_nonzero((ii != 0) | (jj != 0)))
+    ctx = hddm_lb.Context(0)
+    ctx.upload_state(0, ii, jj, S)
+    x = torch.from_numpy(X).cuda()
+    y = torch.zeros((N, dofs), dtype=torch.float64, device="cuda")
+
+    res, ygpu = {}, {}
+    for name in (args.precision, "fp64"):
+        ms, sweep_ms, mn, mx = time_path(ctx, x, y, precs[name], args.steps, args.warmup)
+        res[name] = (ms, sweep_ms, mn, mx)
+        ygpu[name] = y.cpu().numpy()
+    ms, sweep_ms, _, _ = res[args.precision]
+
+    # algorithmic work of one launch: dense contraction 2*nno*dofs plus
+    # 4 flops per active factor, per point
+    flops = N * (2.0 * nno * dofs + 4.0 * factors)
+    achieved = flops / (sweep_ms * 1e-3) / 1e12
+    peak = PEAK_TFLOPS[args.precision]
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("sweep_%s_d16_60k_1m" % args.precision)
+
+    out = {
+        "metric": METRIC,
+        "value": N * dofs / (ms * 1e-3),
+        "unit": "point·dof/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": DTYPE[args.precision],
+        "data": "synthetic (seeded grid, surpluses U[-1,1]*2^-sum(l), points U[0,1)^16)",
+        "config": {"workload": "single-state forward, adaptive sparse grid D=16, %d nodes, dofs=%d, %d points"
+                               % (nno, dofs, N),
+                   "dim": 16, "nno": nno, "dofs": dofs, "points": N, "precision": args.precision},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": traffic,
+                     "kernel": "sweep_kernel (%s)" % args.precision, "kernel_ms": sweep_ms,
+                     "flops_per_launch": flops},
+        "fp64_plain": {"value": N * dofs / (res["fp64"][0] * 1e-3), "ms_per_step": res["fp64"][0],
+                       "kernel_ms": res["fp64"][1],
+                       "roofline_frac": flops / (res["fp64"][1] * 1e-3) / 1e12 / PEAK_TFLOPS["fp64"]},
+        "mixed_over_fp64": res["fp64"][0] / ms,
+    }
+    if not args.no_cpu:
+        cb, errs, port_err = cpu_baseline(ii, jj, S, X, ygpu, args.cpu_seconds)
+        out["cpu_baseline"] = cb
+        out["max_rel_err"] = {k: v for k, v in errs.items()}
+        out["max_rel_err"]["cpu_port_vs_gold"] = port_err
+    else:
+        out["cpu_baseline"] = None
+    ctx.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

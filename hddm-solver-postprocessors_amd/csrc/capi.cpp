@@ -94,8 +94,8 @@ hddm_lb_ctx::~hddm_lb_ctx()
 	for (void* p : { (void*)d_states, (void*)d_cost_order, (void*)d_state_ok,
 		(void*)d_xstage, (void*)d_ystage, (void*)d_sidstage, (void*)d_wbuf, (void*)d_stats })
 		dev_free(p);
-	if (ev0) (void)hipEventDestroy(ev0);
-	if (ev1) (void)hipEventDestroy(ev1);
+	for (hipEvent_t e : tev0) (void)hipEventDestroy(e);
+	for (hipEvent_t e : tev1) (void)hipEventDestroy(e);
 	if (own_stream) (void)hipStreamDestroy(own_stream);
 }
 
@@ -128,8 +128,14 @@ extern "C" int hddm_lb_create(int device, hddm_lb_ctx** out)
 	HIP_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device));
 	ctx->num_cus = cus;
 	ctx->max_lds = (size_t)lds;
-	HIP_TRY(hipEventCreate(&ctx->ev0));
-	HIP_TRY(hipEventCreate(&ctx->ev1));
+	for (int i = 0; i < hddm_lb_ctx::kTimingRing; i++)
+	{
+		hipEvent_t e0 = nullptr, e1 = nullptr;
+		HIP_TRY(hipEventCreate(&e0));
+		ctx->tev0.push_back(e0);
+		HIP_TRY(hipEventCreate(&e1));
+		ctx->tev1.push_back(e1);
+	}
 	*out = ctx.release();
 	return HDDM_LB_SUCCESS;
 }
@@ -404,12 +410,12 @@ int enqueue_forward(hddm_lb_ctx* ctx, Scratch& sc, int64_t npoints, const double
 	HIP_TRY(launch_fill_u32(sc.hist, nb, 0u, st));
 	HIP_TRY(launch_bucket_key(npoints, x, ldx, sid, ctx->d_state_ok, ns, ctx->dim, bb, sc.bkey, sc.hist, st));
 	HIP_TRY(launch_scan_plan(sc.hist, sc.cursor, nb, bb, ns, ctx->d_states, ctx->d_cost_order,
-		sc.plans, sc.hdr, st));
+		sc.plans, sc.hdr, cfg.waves, st));
 	HIP_TRY(launch_scatter(npoints, sc.bkey, sc.cursor, sc.perm, st));
 
 	const void* kern = sweep_kernel_ptr(prec, cfg.slab, xbanks_for(ctx->dim), ctx->fmax_all);
 	if (!kern) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "forward: no sweep kernel for precision %d, dim %d", prec, ctx->dim);
-	const size_t lds = sweep_lds_bytes(cfg.waves, ctx->fmax_all);
+	const size_t lds = sweep_lds_bytes(prec, cfg.waves, ctx->fmax_all);
 	int per_cu = 0;
 	HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, cfg.waves * kWave, lds));
 	if (per_cu < 1)
@@ -433,14 +439,19 @@ int enqueue_forward(hddm_lb_ctx* ctx, Scratch& sc, int64_t npoints, const double
 		a.stats = ctx->d_stats;
 	}
 	const int nslab = (ctx->dt_needed + cfg.slab - 1) / cfg.slab;
-	if (timed) HIP_TRY(hipEventRecord(ctx->ev0, st));
+	const int slot = (int)(ctx->timed_calls % hddm_lb_ctx::kTimingRing);
+	if (timed) HIP_TRY(hipEventRecord(ctx->tev0[slot], st));
 	for (int sl = 0; sl < nslab; sl++)
 	{
 		a.dt_base = sl * cfg.slab;
 		if (sl > 0) HIP_TRY(launch_reset_counter(sc.hdr, st));
 		HIP_TRY(launch_sweep(kern, grid, cfg.waves, lds, a, st));
 	}
-	if (timed) HIP_TRY(hipEventRecord(ctx->ev1, st));
+	if (timed)
+	{
+		HIP_TRY(hipEventRecord(ctx->tev1[slot], st));
+		ctx->timed_calls++;
+	}
 	if (ctx->max_nseg_dp > 0)
 		HIP_TRY(launch_reduce(ctx->d_states, sc.plans, sc.perm, sc.partial, y, ldy, ns, st));
 	return HDDM_LB_SUCCESS;
@@ -493,12 +504,38 @@ extern "C" int hddm_lb_last_stats(hddm_lb_ctx* ctx, uint64_t* stats)
 
 extern "C" double hddm_lb_last_sweep_ms(hddm_lb_ctx* ctx)
 {
-	if (!ctx || !ctx->last_timed) return -1.0;
+	if (!ctx || !ctx->last_timed || ctx->timed_calls == 0) return -1.0;
 	DeviceGuard g(ctx->device);
-	if (hipEventSynchronize(ctx->ev1) != hipSuccess) { (void)hipGetLastError(); return -1.0; }
+	const int slot = (int)((ctx->timed_calls - 1) % hddm_lb_ctx::kTimingRing);
+	if (hipEventSynchronize(ctx->tev1[slot]) != hipSuccess) { (void)hipGetLastError(); return -1.0; }
 	float ms = -1.0f;
-	if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) != hipSuccess) { (void)hipGetLastError(); return -1.0; }
+	if (hipEventElapsedTime(&ms, ctx->tev0[slot], ctx->tev1[slot]) != hipSuccess) { (void)hipGetLastError(); return -1.0; }
 	return (double)ms;
+}
+
+extern "C" int hddm_lb_sweep_time_summary(hddm_lb_ctx* ctx, int64_t* ncalls, double* total_ms,
+	double* min_ms, double* max_ms)
+{
+	if (!ctx || !ncalls || !total_ms) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "sweep_time_summary: NULL argument");
+	DeviceGuard g(ctx->device);
+	const int64_t first = std::max(ctx->timed_read, ctx->timed_calls - hddm_lb_ctx::kTimingRing);
+	double tot = 0.0, mn = 0.0, mx = 0.0;
+	for (int64_t c = first; c < ctx->timed_calls; c++)
+	{
+		const int slot = (int)(c % hddm_lb_ctx::kTimingRing);
+		HIP_TRY(hipEventSynchronize(ctx->tev1[slot]));
+		float ms = 0.0f;
+		HIP_TRY(hipEventElapsedTime(&ms, ctx->tev0[slot], ctx->tev1[slot]));
+		tot += ms;
+		mn = c == first ? ms : std::min(mn, (double)ms);
+		mx = c == first ? ms : std::max(mx, (double)ms);
+	}
+	*ncalls = ctx->timed_calls - first;
+	*total_ms = tot;
+	if (min_ms) *min_ms = mn;
+	if (max_ms) *max_ms = mx;
+	ctx->timed_read = ctx->timed_calls;
+	return HDDM_LB_SUCCESS;
 }
 
 extern "C" int hddm_lb_forward_host(hddm_lb_ctx* ctx, int64_t npoints,

@@ -103,7 +103,11 @@ struct hddm_lb_ctx
 	// measurement
 	bool opt_stats = false;
 	unsigned long long* d_stats = nullptr;
-	hipEvent_t ev0 = nullptr, ev1 = nullptr;
+	// ring of event pairs bracketing the sweep kernels of timed forwards
+	static constexpr int kTimingRing = 256;
+	std::vector<hipEvent_t> tev0, tev1;
+	int64_t timed_calls = 0;      // pairs recorded since create
+	int64_t timed_read = 0;       // pairs already summarised
 	bool last_timed = false;
 
 	// autotuner cache: (prec, dim, total nodes, max dofs, log2 npoints) -> config

@@ -11,10 +11,13 @@
 //                    interleaved expansion); histogram by atomics
 //   3. scan_plan     one workgroup: exclusive scan of the histogram, per-state
 //                    point ranges and the work-item plan (items = state x
-//                    64-point block x node segment, states in cost order)
+//                    group of G 64-point blocks x node segment, states in
+//                    cost order)
 //   4. scatter       counting-sort scatter: perm[sorted position] = point
-//   5. sweep         persistent waves pop items; one point per lane; the wave
-//                    walks the state's prefix trie of nodes, pruning every
+//   5. sweep         persistent workgroups of G waves pop items; one point per
+//                    lane, one block per wave. The workgroup streams the
+//                    segment's op stream through double-buffered LDS tiles;
+//                    each wave walks the prefix trie of nodes, pruning every
 //                    subtree whose prefix product is zero on all 64 lanes, and
 //                    contracts the surviving nodes against the surplus
 //                    (FP64 VALU, or MFMA on 16-node chunks with split operands)
@@ -57,7 +60,7 @@ __global__ void bucket_key_kernel(int64_t n, const double* __restrict__ x, int64
 __global__ void __launch_bounds__(1024) scan_plan_kernel(const uint32_t* __restrict__ hist,
 	uint32_t* __restrict__ cursor, int64_t nbuckets, int bbits, int nstates,
 	const StateDesc* __restrict__ states, const int32_t* __restrict__ cost_order,
-	StatePlan* __restrict__ plans, PlanHeader* __restrict__ hdr)
+	StatePlan* __restrict__ plans, PlanHeader* __restrict__ hdr, int group)
 {
 	__shared__ uint32_t part[1024];
 	const int t = threadIdx.x;
@@ -96,7 +99,7 @@ __global__ void __launch_bounds__(1024) scan_plan_kernel(const uint32_t* __restr
 			int s = cost_order[i];
 			int nseg = states[s].nseg;
 			plans[s].item_base = item;
-			item += plans[s].nblk * nseg;
+			item += (plans[s].nblk + group - 1) / group * nseg;
 			plans[s].partial_base = pbase;
 			if (nseg > 1) pbase += (int64_t)nseg * plans[s].count * states[s].dofs_pad;
 		}
@@ -151,39 +154,33 @@ __device__ __forceinline__ double f64_from_high(uint32_t hi)
 	return __builtin_bit_cast(double, (uint64_t)hi << 32);
 }
 
+// One work item: G neighbouring 64-point blocks of one state (one block per
+// wave of the workgroup) against one node segment of that state.
 struct ItemInfo
 {
 	int s, seg;
-	int64_t blk, start, count;
-	int np;
+	int64_t grp, start, count;
 };
 
-__device__ __forceinline__ bool next_item(const SweepArgs& a, int lane, ItemInfo& it)
+__device__ __forceinline__ void decode_item(const SweepArgs& a, int64_t item, int G, ItemInfo& it)
 {
-	uint64_t item = 0;
-	if (lane == 0) item = atomicAdd(&a.hdr->counter, 1ull);
-	item = uniform64(item);
-	const int64_t nitems = ldc(&a.hdr->nitems);
-	if ((int64_t)item >= nitems) return false;
 	int s = 0;
-	int64_t base = 0, nblk = 1;
+	int64_t base = 0, ngrp = 1;
 	for (int i = 0; i < a.nstates; i++)
 	{
 		const int si = ldc(&a.cost_order[i]);
 		const int64_t ib = ldc(&a.plans[si].item_base);
 		const int64_t nb = ldc(&a.plans[si].nblk);
+		const int64_t ng = (nb + G - 1) / G;
 		const int ns = ldc(&a.states[si].nseg);
-		if ((int64_t)item < ib + nb * ns) { s = si; base = ib; nblk = nb; break; }
+		if (item < ib + ng * ns) { s = si; base = ib; ngrp = ng; break; }
 	}
-	const int64_t local = (int64_t)item - base;
+	const int64_t local = item - base;
 	it.s = s;
-	it.seg = (int)(local / nblk);
-	it.blk = local - (int64_t)it.seg * nblk;
+	it.seg = (int)(local / ngrp);
+	it.grp = local - (int64_t)it.seg * ngrp;
 	it.start = ldc(&a.plans[s].start);
 	it.count = ldc(&a.plans[s].count);
-	const int64_t rem = it.count - it.blk * kWave;
-	it.np = rem < kWave ? (int)rem : kWave;
-	return true;
 }
 
 // Coordinates of the lane's point are held in XB register banks of 16
@@ -367,64 +364,84 @@ __device__ __forceinline__ void contract_chunk(const SOperands<PREC, DT>& so, in
 
 } // namespace
 
+// Ops per LDS tile of the op stream; the workgroup double-buffers tiles.
+constexpr int kTileOps = 512;
+
+// Workgroup LDS: [item slot 16 B][ops 2 x kTileOps x 16 B][skip 2 x kTileOps x 4 B]
+// then per wave the path stack [PM][64] doubles and the chunk buffer [16][64] BT.
+constexpr size_t kLdsHeader = 16;
+constexpr size_t kLdsTiles = 2 * (size_t)kTileOps * (sizeof(Op) + sizeof(uint32_t));
+
+// Issues the copy of ops [tb, tb + kTileOps) and their skips into one tile
+// buffer, spread over the G waves of the workgroup (one 64-op row per wave
+// and instruction). Rows past the end of the stream read its sentinel.
+__device__ __forceinline__ void load_tile(const Op* gops, const uint32_t* gskip, int nops, int tb,
+	Op* tops, uint32_t* tskip, int wave, int G, int lane)
+{
+	for (int r = wave; r < kTileOps / kWave; r += G)
+	{
+		int q = tb + r * kWave + lane;
+		q = q < nops ? q : nops;
+		__builtin_amdgcn_global_load_lds(gops + q, tops + r * kWave, 16, 0, 0);
+		__builtin_amdgcn_global_load_lds(gskip + q, tskip + r * kWave, 4, 0, 0);
+	}
+}
+
 // Walk state of one wave. P[t] (the product of the first t factors of the
 // current trie path) lives in the wave's LDS stage; the two most recent
-// products stay in registers, so only a move up the trie reads LDS.
+// products stay in registers, so only a move up the trie reads LDS. Ops are
+// read from the workgroup's LDS tile with a wave-uniform address.
 template <int XB, typename BT>
 struct Walker
 {
-	const Op* ops;
-	const uint32_t* skip;
 	double* P;          // [PM][64] in LDS
 	BT* B;              // [16][64] in LDS: basis values of the current chunk
 	double base_prev;   // P[last_t]
 	double cur;         // P[last_t + 1]
 	int last_t;
 	int q, q1;
-	Op op;
-	uint32_t sk;
 	int chunk;          // chunk whose positions B holds (-1: none)
 	uint32_t mask;      // positions of B written in this chunk
 	uint32_t nops, nprunes;  // counts for measurement
 
-	__device__ __forceinline__ void init(const StateDesc* sd, int seg, double* Pl, BT* Bl, int lane)
+	__device__ __forceinline__ void init(int q0, int qend, double* Pl, BT* Bl, int lane)
 	{
-		ops = ldc(&sd->ops);
-		skip = ldc(&sd->skip);
-		const int32_t* seg_op = ldc(&sd->seg_op);
-		q = ldc(seg_op + seg);
-		q1 = ldc(seg_op + seg + 1);
+		q = q0;
+		q1 = qend;
 		P = Pl;
 		B = Bl;
 		P[lane] = 1.0;
 		base_prev = 1.0;
 		cur = 1.0;
 		last_t = -1;
-		op = ldc_op(ops + q);
-		sk = ldc(skip + q);
 		chunk = -1;
 		mask = 0;
 		nops = 0;
 		nprunes = 0;
 	}
 
-	// Walks ops, writing basis values of nodes of `chunk` into B. Stops at the
-	// first surviving node of another chunk, returned in (node, val) and not
-	// yet stored, or at the end of the segment (returns false).
-	__device__ __forceinline__ bool run(int lane, const double (&xb0)[16], const double (&xb1)[16],
-		const double (&xb2)[16], const double (&xb3)[16], int& node, double& val)
+	// Walks ops below tend (the end of the resident tile or of the segment),
+	// writing basis values of nodes of `chunk` into B. Stops at the first
+	// surviving node of another chunk, returned in (node, val) and not yet
+	// stored (returns true), or when q reaches tend (returns false).
+	__device__ __forceinline__ bool run(const Op* tops, const uint32_t* tskip, int tb, int tend, int lane,
+		const double (&xb0)[16], const double (&xb1)[16], const double (&xb2)[16], const double (&xb3)[16],
+		int& node, double& val)
 	{
-		while (q < q1)
+		while (q < tend)
 		{
-			// prefetch the sequential successor (ops and skip carry a sentinel)
-			const Op nop = ldc_op(ops + q + 1);
-			const uint32_t nsk = ldc(skip + q + 1);
-			const int t = (int)(op.ctrl & 15u);
+			const int sl = q - tb;
+			const u4_t ov = *(const u4_t*)(tops + sl);
+			const uint32_t dim = __builtin_amdgcn_readfirstlane(ov[0]);
+			const uint32_t ctrl = __builtin_amdgcn_readfirstlane(ov[1]);
+			const uint32_t ihi = __builtin_amdgcn_readfirstlane(ov[2]);
+			const uint32_t njhi = __builtin_amdgcn_readfirstlane(ov[3]);
+			const int t = (int)(ctrl & 15u);
 			double bt;
 			if (t == last_t + 1) bt = cur;
 			else if (t == last_t) bt = base_prev;
 			else bt = P[t * kWave + lane];
-			const double tt = fma(xget<XB>(xb0, xb1, xb2, xb3, op.dim), f64_from_high(op.ihi), f64_from_high(op.njhi));
+			const double tt = fma(xget<XB>(xb0, xb1, xb2, xb3, dim), f64_from_high(ihi), f64_from_high(njhi));
 			const double p = bt * fmax(1.0 - fabs(tt), 0.0);
 			P[(t + 1) * kWave + lane] = p;
 			base_prev = bt;
@@ -434,18 +451,13 @@ struct Walker
 			if (!wave_any(p != 0.0))
 			{
 				nprunes++;
-				q = (int)sk;
-				op = ldc_op(ops + q);
-				sk = ldc(skip + q);
+				q = (int)__builtin_amdgcn_readfirstlane(tskip[sl]);
 				continue;
 			}
-			const bool last = (op.ctrl & kOpLast) != 0;
-			const int n = (int)(op.ctrl >> kOpNodeShift);
 			q++;
-			op = nop;
-			sk = nsk;
-			if (last)
+			if (ctrl & kOpLast)
 			{
+				const int n = (int)(ctrl >> kOpNodeShift);
 				if ((n >> 4) != chunk) { node = n; val = p; return true; }
 				B[(n & 15) * kWave + lane] = (BT)p;
 				mask |= 1u << (n & 15);
@@ -465,47 +477,67 @@ struct Walker
 template <int PREC, int DT, int XB, int PM>
 __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 {
-	extern __shared__ double lds[];
+	extern __shared__ __attribute__((aligned(16))) double lds[];
 	const int lane = threadIdx.x & 63;
-	const int wave = threadIdx.x >> 6;
-	// per wave: P stack [PM][64] doubles, then the chunk buffer [16][64]
-	// (doubles for the FP64 contraction, floats for the MFMA paths)
+	// wave-uniform values the compiler cannot see as uniform by itself
+	const int wave = uniform((int)(threadIdx.x >> 6));
+	const int G = uniform((int)(blockDim.x >> 6));
 	typedef typename std::conditional<PREC == HDDM_LB_FP64, double, float>::type BT;
-	double* Pl = lds + (size_t)wave * (PM * kWave + kChunk * kWave);
-	BT* Bl = (BT*)(Pl + PM * kWave);
+	char* const lbase = (char*)lds;
+	int64_t* const item_slot = (int64_t*)lbase;
+	Op* const tile_ops = (Op*)(lbase + kLdsHeader);
+	uint32_t* const tile_skip = (uint32_t*)(lbase + kLdsHeader + 2 * (size_t)kTileOps * sizeof(Op));
+	double* const Pl = (double*)(lbase + kLdsHeader + kLdsTiles + (size_t)wave * (PM * kWave * 8 + kChunk * kWave * sizeof(BT)));
+	BT* const Bl = (BT*)(Pl + PM * kWave);
+	const int64_t nitems = ldc(&a.hdr->nitems);
 
-	ItemInfo it;
-	while (next_item(a, lane, it))
+	for (;;)
 	{
+		if (threadIdx.x == 0) *item_slot = (int64_t)atomicAdd(&a.hdr->counter, 1ull);
+		__syncthreads();
+		const int64_t item = uniform64((uint64_t)*item_slot);
+		if (item >= nitems) break;
+		ItemInfo it;
+		decode_item(a, item, G, it);
 		const StateDesc* sd = a.states + it.s;
 		const int dofs = ldc(&sd->dofs);
 		const int DP = ldc(&sd->dofs_pad);
 		const int nseg = ldc(&sd->nseg);
+		const int nops_s = ldc(&sd->nops);
+		const Op* const gops = ldc(&sd->ops);
+		const uint32_t* const gskip = ldc(&sd->skip);
+		const int32_t* const seg_op = ldc(&sd->seg_op);
+		const int q0 = ldc(seg_op + it.seg), q1 = ldc(seg_op + it.seg + 1);
 		const int dt0 = a.dt_base;
 		const int ndt = min(DT, (DP >> 5) - dt0);
-		if (ndt <= 0) continue;  // the state has no dofs in this pass
 
-		const int myp = lane < it.np ? lane : it.np - 1;
-		const uint32_t orig = a.perm[it.start + it.blk * kWave + myp];
+		// this wave's block; waves past the state's last block only help load tiles
+		const int64_t blk = it.grp * G + wave;
+		const int64_t rem = it.count - blk * kWave;
+		const int np = ndt <= 0 ? 0 : (rem <= 0 ? 0 : (rem < kWave ? (int)rem : kWave));
+		const int myp = lane < np ? lane : (np > 0 ? np - 1 : 0);
+		const uint32_t orig = np > 0 ? a.perm[it.start + blk * kWave + myp] : 0u;
 		const double* xrow = a.x + (int64_t)orig * a.ldx;
 		double xb0[16], xb1[16], xb2[16], xb3[16];
 #pragma unroll
 		for (int d = 0; d < 16; d++)
 		{
-			xb0[d] = xrow[d < a.dim ? d : 0];
-			xb1[d] = XB > 1 ? xrow[16 + d < a.dim ? 16 + d : 0] : 0.0;
-			xb2[d] = XB > 2 ? xrow[32 + d < a.dim ? 32 + d : 0] : 0.0;
-			xb3[d] = XB > 2 ? xrow[48 + d < a.dim ? 48 + d : 0] : 0.0;
+			xb0[d] = np > 0 ? xrow[d < a.dim ? d : 0] : 0.0;
+			xb1[d] = XB > 1 && np > 0 ? xrow[16 + d < a.dim ? 16 + d : 0] : 0.0;
+			xb2[d] = XB > 2 && np > 0 ? xrow[32 + d < a.dim ? 32 + d : 0] : 0.0;
+			xb3[d] = XB > 2 && np > 0 ? xrow[48 + d < a.dim ? 48 + d : 0] : 0.0;
 		}
 #pragma unroll
 		for (int k = 0; k < kChunk; k++) Bl[k * kWave + lane] = (BT)0;
 
 		Walker<XB, BT> w;
-		w.init(sd, it.seg, Pl, Bl, lane);
+		w.init(q0, np > 0 ? q1 : q0, Pl, Bl, lane);
 		uint32_t stat_chunks = 0;
-
 		const bool split = nseg > 1;
 		double* const part = a.partial + ldc(&a.plans[it.s].partial_base);
+
+		load_tile(gops, gskip, nops_s, q0, tile_ops, tile_skip, wave, G, lane);
+		__syncthreads();
 
 		if constexpr (PREC == HDDM_LB_FP64)
 		{
@@ -513,27 +545,51 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 			double acc[32];
 #pragma unroll
 			for (int c = 0; c < 32; c++) acc[c] = 0.0;
-			for (;;)
+			int k = 0;
+			for (int tb = q0; tb < q1; tb += kTileOps, k ^= 1)
 			{
-				int node = 0;
-				double val = 0.0;
-				const bool more = w.run(lane, xb0, xb1, xb2, xb3, node, val);
-				// FP64 contraction of the finished chunk, positions in ascending order
-				if (a.stats && w.chunk >= 0) stat_chunks++;
+				if (tb + kTileOps < q1)
+					load_tile(gops, gskip, nops_s, tb + kTileOps, tile_ops + (k ^ 1) * kTileOps,
+						tile_skip + (k ^ 1) * kTileOps, wave, G, lane);
+				const int tend = min(tb + kTileOps, w.q1);
+				for (;;)
+				{
+					int node = 0;
+					double val = 0.0;
+					if (!w.run(tile_ops + k * kTileOps, tile_skip + k * kTileOps, tb, tend, lane,
+							xb0, xb1, xb2, xb3, node, val)) break;
+					// FP64 contraction of the finished chunk, positions in ascending order
+					if (w.chunk >= 0)
+					{
+						if (a.stats) stat_chunks++;
+						for (uint32_t m = w.mask; m; m &= m - 1)
+						{
+							const int kk = __builtin_ctz(m);
+							const double b = Bl[kk * kWave + lane];
+							const double* srow = s64 + (size_t)(w.chunk * kChunk + kk) * DP;
+#pragma unroll
+							for (int c = 0; c < 32; c++) acc[c] = fma(b, ldc(srow + c), acc[c]);
+						}
+					}
+					w.start_chunk(node, val, lane);
+				}
+				__syncthreads();
+			}
+			if (w.chunk >= 0)
+			{
+				if (a.stats) stat_chunks++;
 				for (uint32_t m = w.mask; m; m &= m - 1)
 				{
-					const int k = __builtin_ctz(m);
-					const double b = Bl[k * kWave + lane];
-					const double* srow = s64 + (size_t)(w.chunk * kChunk + k) * DP;
+					const int kk = __builtin_ctz(m);
+					const double b = Bl[kk * kWave + lane];
+					const double* srow = s64 + (size_t)(w.chunk * kChunk + kk) * DP;
 #pragma unroll
 					for (int c = 0; c < 32; c++) acc[c] = fma(b, ldc(srow + c), acc[c]);
 				}
-				if (!more) break;
-				w.start_chunk(node, val, lane);
 			}
-			if (lane < it.np)
+			if (lane < np)
 			{
-				double* out = split ? part + ((int64_t)it.seg * it.count + it.blk * kWave + lane) * DP
+				double* out = split ? part + ((int64_t)it.seg * it.count + blk * kWave + lane) * DP
 					: a.y + (int64_t)orig * a.ldy;
 				out += 32 * dt0;
 				const int dlim = dofs - 32 * dt0;
@@ -556,36 +612,46 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 					for (int r = 0; r < 16; r++) { acc[t][dt][r] = 0.0f; acc64[t][dt][r] = 0.0; }
 			int cur_win = -1;
 			SOperands<PREC, DT> so;
-			for (;;)
-			{
-				int node = 0;
-				double val = 0.0;
-				const bool more = w.run(lane, xb0, xb1, xb2, xb3, node, val);
-				if (w.chunk >= 0)
+			auto contract = [&]() {
+				if (a.stats) stat_chunks++;
+				const int win = w.chunk >> a.wlog;
+				if (win != cur_win)
 				{
-					if (a.stats) stat_chunks++;
-					const int win = w.chunk >> a.wlog;
-					if (win != cur_win)
-					{
 #pragma unroll
-						for (int t = 0; t < 2; t++)
+					for (int t = 0; t < 2; t++)
 #pragma unroll
-							for (int dt = 0; dt < DT; dt++)
+						for (int dt = 0; dt < DT; dt++)
 #pragma unroll
-								for (int r = 0; r < 16; r++) { acc64[t][dt][r] += (double)acc[t][dt][r]; acc[t][dt][r] = 0.0f; }
-						cur_win = win;
-					}
-					float b[kChunk];
-#pragma unroll
-					for (int k = 0; k < kChunk; k++) { b[k] = Bl[k * kWave + lane]; Bl[k * kWave + lane] = 0.0f; }
-					contract_chunk<PREC, DT>(so, ndt, b, acc);
+							for (int r = 0; r < 16; r++) { acc64[t][dt][r] += (double)acc[t][dt][r]; acc[t][dt][r] = 0.0f; }
+					cur_win = win;
 				}
-				if (!more) break;
-				w.start_chunk(node, val, lane);
-				// issue the surplus operands of the new chunk now; they are
-				// consumed after the walk has finished the chunk
-				so.load(sd, w.chunk, DP, dt0, ndt, lane);
+				float b[kChunk];
+#pragma unroll
+				for (int kk = 0; kk < kChunk; kk++) { b[kk] = Bl[kk * kWave + lane]; Bl[kk * kWave + lane] = 0.0f; }
+				contract_chunk<PREC, DT>(so, ndt, b, acc);
+			};
+			int k = 0;
+			for (int tb = q0; tb < q1; tb += kTileOps, k ^= 1)
+			{
+				if (tb + kTileOps < q1)
+					load_tile(gops, gskip, nops_s, tb + kTileOps, tile_ops + (k ^ 1) * kTileOps,
+						tile_skip + (k ^ 1) * kTileOps, wave, G, lane);
+				const int tend = min(tb + kTileOps, w.q1);
+				for (;;)
+				{
+					int node = 0;
+					double val = 0.0;
+					if (!w.run(tile_ops + k * kTileOps, tile_skip + k * kTileOps, tb, tend, lane,
+							xb0, xb1, xb2, xb3, node, val)) break;
+					if (w.chunk >= 0) contract();
+					w.start_chunk(node, val, lane);
+					// issue the surplus operands of the new chunk now; they are
+					// consumed after the walk has finished the chunk
+					so.load(sd, w.chunk, DP, dt0, ndt, lane);
+				}
+				__syncthreads();
 			}
+			if (w.chunk >= 0) contract();
 #pragma unroll
 			for (int t = 0; t < 2; t++)
 #pragma unroll
@@ -602,9 +668,9 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 			{
 				const int qp = 32 * t + row;
 				const uint32_t oq = (uint32_t)__shfl((int)orig, qp, 64);
-				if (qp < it.np)
+				if (qp < np)
 				{
-					double* out = split ? part + ((int64_t)it.seg * it.count + it.blk * kWave + qp) * DP
+					double* out = split ? part + ((int64_t)it.seg * it.count + blk * kWave + qp) * DP
 						: a.y + (int64_t)oq * a.ldy;
 #pragma unroll
 					for (int dt = 0; dt < DT; dt++)
@@ -625,7 +691,7 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 				}
 			}
 		}
-		if (a.stats && lane == 0)
+		if (a.stats && lane == 0 && np > 0)
 		{
 			atomicAdd(&a.stats[0], (unsigned long long)w.nops);
 			atomicAdd(&a.stats[1], (unsigned long long)w.nprunes);
@@ -635,10 +701,11 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 	}
 }
 
-size_t sweep_lds_bytes(int waves, int depth)
+size_t sweep_lds_bytes(int prec, int waves, int depth)
 {
-	const int pm = depth < 8 ? 8 : 16;
-	return (size_t)waves * ((size_t)pm + kChunk) * kWave * 8;
+	const size_t pm = depth < 8 ? 8 : 16;
+	const size_t bt = prec == HDDM_LB_FP64 ? 8 : 4;
+	return kLdsHeader + kLdsTiles + (size_t)waves * (pm * kWave * 8 + kChunk * kWave * bt);
 }
 
 __global__ void reset_counter_kernel(PlanHeader* hdr)
@@ -729,10 +796,10 @@ hipError_t launch_bucket_key(int64_t n, const double* x, int64_t ldx, const int3
 }
 
 hipError_t launch_scan_plan(const uint32_t* hist, uint32_t* cursor, int64_t nbuckets, int bbits, int nstates,
-	const StateDesc* states, const int32_t* cost_order, StatePlan* plans, PlanHeader* hdr, hipStream_t st)
+	const StateDesc* states, const int32_t* cost_order, StatePlan* plans, PlanHeader* hdr, int group, hipStream_t st)
 {
 	hipLaunchKernelGGL(scan_plan_kernel, dim3(1), dim3(1024), 0, st, hist, cursor, nbuckets, bbits, nstates,
-		states, cost_order, plans, hdr);
+		states, cost_order, plans, hdr, group);
 	return hipGetLastError();
 }
 

@@ -39,12 +39,13 @@ hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st);
 hipError_t launch_bucket_key(int64_t n, const double* x, int64_t ldx, const int32_t* sid,
 	const uint8_t* state_ok, int nstates, int dim, int bbits, uint32_t* bkey, uint32_t* hist, hipStream_t st);
 hipError_t launch_scan_plan(const uint32_t* hist, uint32_t* cursor, int64_t nbuckets, int bbits, int nstates,
-	const StateDesc* states, const int32_t* cost_order, StatePlan* plans, PlanHeader* hdr, hipStream_t st);
+	const StateDesc* states, const int32_t* cost_order, StatePlan* plans, PlanHeader* hdr, int group, hipStream_t st);
 hipError_t launch_scatter(int64_t n, const uint32_t* bkey, uint32_t* cursor, uint32_t* perm, hipStream_t st);
 hipError_t launch_reset_counter(PlanHeader* hdr, hipStream_t st);
 hipError_t launch_sweep(const void* kern, int grid, int waves, size_t lds, const SweepArgs& args, hipStream_t st);
-// Dynamic LDS of the sweep: per wave the path stack and the chunk buffer.
-size_t sweep_lds_bytes(int waves, int depth);
+// Dynamic LDS of the sweep: the workgroup's op tiles, then per wave the path
+// stack and the chunk buffer.
+size_t sweep_lds_bytes(int prec, int waves, int depth);
 hipError_t launch_reduce(const StateDesc* states, const StatePlan* plans, const uint32_t* perm,
 	const double* partial, double* y, int64_t ldy, int nstates, hipStream_t st);
 

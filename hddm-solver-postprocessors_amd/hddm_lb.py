@@ -82,6 +82,7 @@ _SIGS = {
     "hddm_lb_plan_destroy": (_i32, [_vp]),
     "hddm_lb_last_sweep_ms": (_dbl, [_vp]),
     "hddm_lb_last_stats": (_i32, [_vp, _vp]),
+    "hddm_lb_sweep_time_summary": (_i32, [_vp, _vp, _vp, _vp, _vp]),
     "hddm_lb_plt_read": (_i32, [_c.c_char_p, _i32, _c.POINTER(_i32), _c.POINTER(_i32), _c.POINTER(_i32),
                                 _c.POINTER(_i32), _c.POINTER(_I32P), _c.POINTER(_I32P), _c.POINTER(_DP)]),
     "hddm_lb_plt_write_compressed": (_i32, [_c.c_char_p, _i32, _i32, _i32, _i32, _vp, _vp, _vp]),
@@ -241,9 +242,9 @@ class Layout:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
+        if h and lib is not None:
             lib.hddm_lb_layout_free(h)
-            self._h = None
+        self._h = None
 
 
 # ---------------------------------------------------------------- context
@@ -267,9 +268,9 @@ class Context:
         self.states = {}
 
     def close(self):
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and lib is not None:
             lib.hddm_lb_destroy(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         self.close()
@@ -320,6 +321,13 @@ class Context:
     def last_sweep_ms(self):
         return lib.hddm_lb_last_sweep_ms(self._h)
 
+    def sweep_time_summary(self):
+        """(calls, total ms, min ms, max ms) of the sweep kernels of timed forwards since the last call."""
+        n = ctypes.c_int64(0)
+        tot, mn, mx = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_double(0)
+        _check(lib.hddm_lb_sweep_time_summary(self._h, ctypes.byref(n), ctypes.byref(tot), ctypes.byref(mn), ctypes.byref(mx)))
+        return n.value, tot.value, mn.value, mx.value
+
     def last_stats(self):
         """(ops evaluated, subtrees pruned, chunks contracted, work items) of the last forward."""
         st = np.zeros(4, np.uint64)
@@ -344,6 +352,6 @@ class Plan:
         _check(lib.hddm_lb_plan_launch(self._h, _stream_ptr(stream)))
 
     def __del__(self):
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and lib is not None:
             lib.hddm_lb_plan_destroy(self._h)
-            self._h = None
+        self._h = None

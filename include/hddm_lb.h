@@ -133,6 +133,11 @@ int hddm_lb_plan_destroy(hddm_lb_plan* plan);
  * on this context, when HDDM_LB_OPT_TIMING is set; -1 otherwise. Waits for
  * the second event. */
 double hddm_lb_last_sweep_ms(hddm_lb_ctx* ctx);
+/* Sweep-kernel times of the timed forwards enqueued since the previous call
+ * (at most the last 256): count, total, min and max in ms (min/max may be
+ * NULL). Waits for their events. */
+int hddm_lb_sweep_time_summary(hddm_lb_ctx* ctx, int64_t* ncalls, double* total_ms,
+	double* min_ms, double* max_ms);
 /* With HDDM_LB_OPT_STATS set: counts of the last forward, summed over all
  * work items: stats[0] ops evaluated, [1] subtrees pruned, [2] 16-node chunks
  * contracted, [3] work items. Waits for the forward to finish. */
