@@ -1,13 +1,380 @@
-// Adjoint kernels (placeholder until the adjoint sweep lands).
+// Adjoint of the LinearBasis forward (gfx950). No counterpart in the reference.
+//
+// For y_p = sum_k B_k(x_p) S_k and an upstream gradient g_p = dL/dy_p:
+//   dS_k   = sum_p B_k(x_p) g_p                                     (dsurplus)
+//   dx_p,d = sum_k w_pk dB_k/dx_d,  w_pk = g_p . S_k                 (dx)
+// with the piecewise-constant hat derivative d/dx max(0, 1 - |x i - j|) =
+// -i sign(x i - j) inside the support and 0 outside (and 0 at the peak).
+// The basis is recomputed by the same prefix-trie walk as the forward sweep
+// (workgroups of G waves, one 64-point block per wave, op stream through
+// LDS tiles). Along the walk each trie edge e at depth t keeps
+//   R_e = sum over the surviving nodes k below e of w_k B_k,
+// and when the walk leaves e (a sibling or an ancestor's sibling comes next)
+// it adds R_e * phi'_e / phi_e to dx[dim(e)] and R_e to its parent edge:
+// dB_k/dx_d = B_k phi'_d / phi_d for the one factor of k in dim d, so every
+// node's contribution reaches each of its factors exactly once. w_pk is an
+// FP64 dot product against the node's surplus row at emission; dS is
+// contracted per 16-node chunk with v_mfma_f64_16x16x4f64 (dofs x nodes,
+// K = 4 points per step) and added to the output with FP64 atomics.
+#include "sweep_common.h"
+#include "kernels.h"
 #include "context.h"
 #include "errors.h"
 
+#include <algorithm>
+#include <cstring>
+
+namespace hddm_lb {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+struct AdjointArgs
+{
+	const StateDesc* states;
+	const StatePlan* plans;
+	const int32_t* cost_order;
+	PlanHeader* hdr;
+	const uint32_t* perm;
+	const double* x;
+	int64_t ldx;
+	const double* g;
+	int64_t ldg;
+	double* dx;        // may be null
+	int64_t lddx;
+	double* ds;        // may be null; state s starts at states[s].ds_offset
+	int nstates;
+	int dim;
+	int dt_base;       // dof slab of 32 handled by this launch
+};
+
+// Per-wave LDS of the adjoint: P, Q (phi'/phi), R [PM][64]; dx [16 XB][64];
+// chunk basis [16][64] (doubles); dims of the path [PM] (uint32).
+template <int XB, int PM>
+struct AdjLds
+{
+	static constexpr size_t kDoubles = (size_t)(3 * PM + 16 * XB + kChunk) * kWave;
+	static constexpr size_t kBytes = kDoubles * 8 + ((PM * 4 + 15) / 16) * 16;
+};
+
+namespace {
+
+// dS of one finished chunk: D[dof][node] += sum_pt g[pt][dof] B[node][pt],
+// 16 dofs x 16 nodes per v_mfma_f64_16x16x4f64 tile, 16 steps of 4 points.
+// Operand layouts: A[i][k] from lane i + 16k, B[k][j] from lane j + 16k;
+// D[i][j] in lane j + 16 (i / 4), element i % 4.
+__device__ __forceinline__ void contract_ds(const AdjointArgs& a, const StateDesc* sd, const double* Bc,
+	uint32_t mask, int chunk, int lane, const uint32_t (&og)[16], uint32_t ogvalid, int dofs)
+{
+	double* const ds = a.ds + ldc(&sd->ds_offset);
+	const int32_t* order = ldc(&sd->order);
+	const int col = lane & 15, kq = lane >> 4;
+	const int node = chunk * kChunk + col;
+	const bool live = (mask >> col) & 1u;
+	const int orow = live ? order[node] : 0;
+#pragma unroll
+	for (int tile = 0; tile < 2; tile++)
+	{
+		const int dbase = (a.dt_base * 2 + tile) * 16;
+		if (dbase >= dofs) break;
+		const int dofA = dbase + col;
+		double av[16];
+#pragma unroll
+		for (int s = 0; s < 16; s++)
+			av[s] = ((ogvalid >> s) & 1u) && dofA < dofs ? a.g[(int64_t)og[s] * a.ldg + dofA] : 0.0;
+		d4_t acc = { 0.0, 0.0, 0.0, 0.0 };
+#pragma unroll
+		for (int s = 0; s < 16; s++)
+		{
+			const double bv = Bc[col * kWave + 4 * s + kq];
+			acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s], bv, acc, 0, 0, 0);
+		}
+		if (live)
+		{
+#pragma unroll
+			for (int r = 0; r < 4; r++)
+			{
+				const int dof = dbase + 4 * kq + r;
+				if (dof < dofs) unsafeAtomicAdd(ds + (int64_t)orow * dofs + dof, acc[r]);
+			}
+		}
+	}
+}
+
+} // namespace
+
+template <int XB, int PM>
+__global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
+{
+	extern __shared__ __attribute__((aligned(16))) double lds[];
+	const int lane = threadIdx.x & 63;
+	const int wave = uniform((int)(threadIdx.x >> 6));
+	const int G = uniform((int)(blockDim.x >> 6));
+	constexpr int DXR = 16 * XB;
+	char* const lbase = (char*)lds;
+	int64_t* const item_slot = (int64_t*)lbase;
+	Op* const tile_ops = (Op*)(lbase + kLdsHeader);
+	uint32_t* const tile_skip = (uint32_t*)(lbase + kLdsHeader + 2 * (size_t)kTileOps * sizeof(Op));
+	double* const P = (double*)(lbase + kLdsHeader + kLdsTiles + (size_t)wave * AdjLds<XB, PM>::kBytes);
+	double* const Q = P + PM * kWave;
+	double* const R = Q + PM * kWave;
+	double* const DX = R + PM * kWave;
+	double* const Bc = DX + DXR * kWave;
+	uint32_t* const dims = (uint32_t*)(Bc + kChunk * kWave);
+	const int64_t nitems = ldc(&a.hdr->nitems);
+
+	for (;;)
+	{
+		if (threadIdx.x == 0) *item_slot = (int64_t)atomicAdd(&a.hdr->counter, 1ull);
+		__syncthreads();
+		const int64_t item = uniform64((uint64_t)*item_slot);
+		if (item >= nitems) break;
+		ItemInfo it;
+		decode_item(a, item, G, it);
+		const StateDesc* sd = a.states + it.s;
+		const int dofs = ldc(&sd->dofs);
+		const int DP = ldc(&sd->dofs_pad);
+		const int nops_s = ldc(&sd->nops);
+		const Op* const gops = ldc(&sd->ops);
+		const uint32_t* const gskip = ldc(&sd->skip);
+		const int32_t* const seg_op = ldc(&sd->seg_op);
+		const int q0 = ldc(seg_op + it.seg), q1 = ldc(seg_op + it.seg + 1);
+		const int dt0 = a.dt_base;
+		const bool active = 32 * dt0 < dofs;
+
+		const int64_t blk = it.grp * G + wave;
+		const int64_t rem = it.count - blk * kWave;
+		const int np = !active ? 0 : (rem <= 0 ? 0 : (rem < kWave ? (int)rem : kWave));
+		const int myp = lane < np ? lane : (np > 0 ? np - 1 : 0);
+		const uint32_t orig = np > 0 ? a.perm[it.start + blk * kWave + myp] : 0u;
+		const double* xrow = a.x + (int64_t)orig * a.ldx;
+		double xb0[16], xb1[16], xb2[16], xb3[16];
+#pragma unroll
+		for (int d = 0; d < 16; d++)
+		{
+			xb0[d] = np > 0 ? xrow[d < a.dim ? d : 0] : 0.0;
+			xb1[d] = XB > 1 && np > 0 ? xrow[16 + d < a.dim ? 16 + d : 0] : 0.0;
+			xb2[d] = XB > 2 && np > 0 ? xrow[32 + d < a.dim ? 32 + d : 0] : 0.0;
+			xb3[d] = XB > 2 && np > 0 ? xrow[48 + d < a.dim ? 48 + d : 0] : 0.0;
+		}
+		// the lane's upstream gradient on this dof slab (0 for lanes past the block)
+		double g[32];
+		const double* grow = a.g + (int64_t)orig * a.ldg + 32 * dt0;
+#pragma unroll
+		for (int c = 0; c < 32; c++) g[c] = lane < np && 32 * dt0 + c < dofs ? grow[c] : 0.0;
+		// points whose gradient rows feed the dS operands of this lane
+		uint32_t og[16];
+		uint32_t ogvalid = 0;
+#pragma unroll
+		for (int s = 0; s < 16; s++)
+		{
+			const int pt = 4 * s + (lane >> 4);
+			og[s] = (uint32_t)__shfl((int)orig, pt, 64);
+			if (pt < np) ogvalid |= 1u << s;
+		}
+#pragma unroll
+		for (int d = 0; d < DXR; d++) DX[d * kWave + lane] = 0.0;
+#pragma unroll
+		for (int k = 0; k < kChunk; k++) Bc[k * kWave + lane] = 0.0;
+		P[lane] = 1.0;
+
+		const double* const s64 = ldc(&sd->s64) + 32 * dt0;
+		const bool want_ds = a.ds != nullptr;
+		int q = q0;
+		const int qend = np > 0 ? q1 : q0;
+		int top = -1;          // deepest live edge of the path
+		int last_t = -1;
+		double cur = 1.0, base_prev = 1.0;
+		int chunk = -1;
+		uint32_t mask = 0;
+
+		load_tile(gops, gskip, nops_s, q0, tile_ops, tile_skip, wave, G, lane);
+		__syncthreads();
+
+		int kb = 0;
+		for (int tb = q0; tb < q1; tb += kTileOps, kb ^= 1)
+		{
+			if (tb + kTileOps < q1)
+				load_tile(gops, gskip, nops_s, tb + kTileOps, tile_ops + (kb ^ 1) * kTileOps,
+					tile_skip + (kb ^ 1) * kTileOps, wave, G, lane);
+			const Op* const tops = tile_ops + kb * kTileOps;
+			const uint32_t* const tskip = tile_skip + kb * kTileOps;
+			const int tend = min(tb + kTileOps, qend);
+			while (q < tend)
+			{
+				const int sl = q - tb;
+				const u4_t ov = *(const u4_t*)(tops + sl);
+				const uint32_t dim = __builtin_amdgcn_readfirstlane(ov[0]);
+				const uint32_t ctrl = __builtin_amdgcn_readfirstlane(ov[1]);
+				const uint32_t ihi = __builtin_amdgcn_readfirstlane(ov[2]);
+				const uint32_t njhi = __builtin_amdgcn_readfirstlane(ov[3]);
+				const int t = (int)(ctrl & 15u);
+				// leave the edges at depths >= t
+				for (int tt = top; tt >= t; tt--)
+				{
+					const double r = R[tt * kWave + lane];
+					if (wave_any(r != 0.0))
+					{
+						const uint32_t dd = __builtin_amdgcn_readfirstlane(dims[tt]);
+						DX[dd * kWave + lane] += r * Q[tt * kWave + lane];
+						if (tt > 0) R[(tt - 1) * kWave + lane] += r;
+					}
+				}
+				top = t;
+				double bt;
+				if (t == last_t + 1) bt = cur;
+				else if (t == last_t) bt = base_prev;
+				else bt = P[t * kWave + lane];
+				const double iv = f64_from_high(ihi);
+				const double u = fma(xget<XB>(xb0, xb1, xb2, xb3, dim), iv, f64_from_high(njhi));
+				const double phi = 1.0 - fabs(u);
+				const double p = bt * fmax(phi, 0.0);
+				P[(t + 1) * kWave + lane] = p;
+				Q[t * kWave + lane] = phi > 0.0 ? (u > 0.0 ? -iv : (u < 0.0 ? iv : 0.0)) / phi : 0.0;
+				R[t * kWave + lane] = 0.0;
+				if (lane == 0) dims[t] = dim;
+				base_prev = bt;
+				cur = p;
+				last_t = t;
+				if (!wave_any(p != 0.0))
+				{
+					q = (int)__builtin_amdgcn_readfirstlane(tskip[sl]);
+					continue;
+				}
+				q++;
+				if (ctrl & kOpLast)
+				{
+					const int n = (int)(ctrl >> kOpNodeShift);
+					const double* srow = s64 + (size_t)n * DP;
+					double w = 0.0;
+#pragma unroll
+					for (int c = 0; c < 32; c++) w = fma(g[c], ldc(srow + c), w);
+					R[t * kWave + lane] = w * p;
+					if (want_ds)
+					{
+						if ((n >> 4) != chunk)
+						{
+							if (chunk >= 0)
+							{
+								contract_ds(a, sd, Bc, mask, chunk, lane, og, ogvalid, dofs);
+#pragma unroll
+								for (int k = 0; k < kChunk; k++) Bc[k * kWave + lane] = 0.0;
+							}
+							chunk = n >> 4;
+							mask = 0;
+						}
+						Bc[(n & 15) * kWave + lane] = p;
+						mask |= 1u << (n & 15);
+					}
+				}
+			}
+			__syncthreads();
+		}
+		if (want_ds && chunk >= 0) contract_ds(a, sd, Bc, mask, chunk, lane, og, ogvalid, dofs);
+		for (int tt = top; tt >= 0; tt--)
+		{
+			const double r = R[tt * kWave + lane];
+			const uint32_t dd = __builtin_amdgcn_readfirstlane(dims[tt]);
+			DX[dd * kWave + lane] += r * Q[tt * kWave + lane];
+			if (tt > 0) R[(tt - 1) * kWave + lane] += r;
+		}
+		if (a.dx && lane < np)
+		{
+			double* out = a.dx + (int64_t)orig * a.lddx;
+#pragma unroll
+			for (int d = 0; d < DXR; d++)
+				if (d < a.dim)
+				{
+					const double v = DX[d * kWave + lane];
+					if (v != 0.0) unsafeAtomicAdd(out + d, v);
+				}
+		}
+	}
+}
+
+template <int XB, int PM>
+static const void* adjoint_ptr() { return (const void*)&adjoint_kernel<XB, PM>; }
+
+static const void* adjoint_kernel_ptr(int xbanks, int depth, size_t& per_wave)
+{
+	const bool deep = depth >= 8;
+	switch (xbanks)
+	{
+	case 1: per_wave = deep ? AdjLds<1, 16>::kBytes : AdjLds<1, 8>::kBytes; return deep ? adjoint_ptr<1, 16>() : adjoint_ptr<1, 8>();
+	case 2: per_wave = deep ? AdjLds<2, 16>::kBytes : AdjLds<2, 8>::kBytes; return deep ? adjoint_ptr<2, 16>() : adjoint_ptr<2, 8>();
+	case 3: case 4: per_wave = deep ? AdjLds<4, 16>::kBytes : AdjLds<4, 8>::kBytes; return deep ? adjoint_ptr<4, 16>() : adjoint_ptr<4, 8>();
+	}
+	return nullptr;
+}
+
+} // namespace hddm_lb
+
+using namespace hddm_lb;
+
+#define HIP_TRY(expr) do { hipError_t e_ = (expr); if (e_ != hipSuccess) return hip_fail(e_, #expr, __FILE__, __LINE__); } while (0)
+
 extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 	const double* x, int64_t ldx, const int32_t* state_ids,
-	const float* gbar, int64_t ldg, float* dx, int64_t lddx,
-	float* dsurplus, int precision, void* stream)
+	const double* gbar, int64_t ldg, double* dx, int64_t lddx,
+	double* dsurplus, int precision, void* stream)
 {
-	(void)ctx; (void)npoints; (void)x; (void)ldx; (void)state_ids; (void)gbar; (void)ldg;
-	(void)dx; (void)lddx; (void)dsurplus; (void)precision; (void)stream;
-	return hddm_lb::set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: not available yet");
+	if (!ctx) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "adjoint: ctx is NULL");
+	if (precision != HDDM_LB_FP64)
+		return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: precision %d is not available; use HDDM_LB_FP64", precision);
+	if (npoints < 0 || npoints > ((int64_t)1 << 31) - 1)
+		return set_error(HDDM_LB_ERROR_INVALID_VALUE, "adjoint: npoints %lld out of range", (long long)npoints);
+	if (ctx->dim < 1) return set_error(HDDM_LB_ERROR_STATE, "adjoint: no state is loaded");
+	int maxdofs = 0;
+	for (auto& s : ctx->states) if (s && s->loaded) maxdofs = std::max(maxdofs, s->desc.dofs);
+	if (npoints && (!x || !gbar)) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "adjoint: x and gbar must not be NULL");
+	if (ldx < ctx->dim) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "adjoint: ldx %lld < dim %d", (long long)ldx, ctx->dim);
+	if (ldg < maxdofs) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "adjoint: ldg %lld < dofs %d", (long long)ldg, maxdofs);
+	if (dx && lddx < ctx->dim) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "adjoint: lddx %lld < dim %d", (long long)lddx, ctx->dim);
+	DeviceGuard guard(ctx->device);
+	hipStream_t st = (hipStream_t)stream;
+	int rc = prepare_scratch(ctx, ctx->scratch, std::max<int64_t>(npoints, 1));
+	if (rc) return rc;
+	int64_t ds_total = 0;
+	if ((rc = hddm_lb_surplus_offset(ctx, -1, nullptr, &ds_total))) return rc;
+	// outputs are accumulated: clear them with a kernel (never a memset, so
+	// the sequence stays capturable)
+	if (dsurplus && ds_total) HIP_TRY(launch_fill_u32((uint32_t*)dsurplus, 2 * ds_total, 0u, st));
+	if (dx && npoints) HIP_TRY(launch_fill_u32((uint32_t*)dx, 2 * (npoints - 1) * lddx + 2 * ctx->dim, 0u, st));
+	if (npoints == 0 || (!dx && !dsurplus)) return HDDM_LB_SUCCESS;
+
+	size_t per_wave = 0;
+	const void* kern = adjoint_kernel_ptr(xbanks_for(ctx->dim), ctx->fmax_all, per_wave);
+	if (!kern) return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: dim %d is not supported", ctx->dim);
+	const size_t fixed = sweep_lds_bytes(HDDM_LB_F32, 0, 0);
+	int G = (int)std::min<size_t>(4, (ctx->max_lds - fixed) / per_wave);
+	if (ctx->opt_waves) G = std::min(G, ctx->opt_waves);
+	if (G < 1) return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: LDS of one wave (%zu B) exceeds the CU", per_wave);
+	const size_t lds = fixed + (size_t)G * per_wave;
+	HIP_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+	int per_cu = 0;
+	HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, G * kWave, lds));
+	if (per_cu < 1) return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: kernel does not fit a CU");
+	const int grid = per_cu * ctx->num_cus;
+
+	Scratch& sc = ctx->scratch;
+	const int ns = ctx->tables_n;
+	const int bb = bucket_bits(npoints, ns, ctx->opt_sort);
+	const int64_t nb = (int64_t)(ns + 1) << bb;
+	HIP_TRY(launch_fill_u32(sc.hist, nb, 0u, st));
+	HIP_TRY(launch_bucket_key(npoints, x, ldx, state_ids, ctx->d_state_ok, ns, ctx->dim, bb, sc.bkey, sc.hist, st));
+	HIP_TRY(launch_scan_plan(sc.hist, sc.cursor, nb, bb, ns, ctx->d_states, ctx->d_cost_order, sc.plans, sc.hdr, G, st));
+	HIP_TRY(launch_scatter(npoints, sc.bkey, sc.cursor, sc.perm, st));
+
+	AdjointArgs a;
+	std::memset(&a, 0, sizeof(a));
+	a.states = ctx->d_states; a.plans = sc.plans; a.cost_order = ctx->d_cost_order; a.hdr = sc.hdr;
+	a.perm = sc.perm; a.x = x; a.ldx = ldx; a.g = gbar; a.ldg = ldg; a.dx = dx; a.lddx = lddx;
+	a.ds = dsurplus; a.nstates = ns; a.dim = ctx->dim;
+	for (int sl = 0; sl < ctx->dt_needed; sl++)
+	{
+		a.dt_base = sl;
+		if (sl > 0) HIP_TRY(launch_reset_counter(sc.hdr, st));
+		void* params[] = { (void*)&a };
+		HIP_TRY(hipLaunchKernel(kern, dim3(grid), dim3(G * kWave), params, lds, st));
+	}
+	return HDDM_LB_SUCCESS;
 }

@@ -288,6 +288,7 @@ int prepare_tables(hddm_lb_ctx* ctx)
 	ctx->dt_needed = 1;
 	ctx->fmax_all = 1;
 	ctx->max_nseg_dp = 0;
+	int64_t ds_offset = 0;
 	for (int s = 0; s < ns; s++)
 	{
 		std::memset(&descs[s], 0, sizeof(StateDesc));
@@ -312,6 +313,8 @@ int prepare_tables(hddm_lb_ctx* ctx)
 			slot.seg_len = want;
 		}
 		d = slot.desc;
+		d.ds_offset = ds_offset;
+		ds_offset += (int64_t)d.nno * d.dofs;
 		descs[s] = d;
 		ok[s] = 1;
 		ctx->dt_needed = std::max(ctx->dt_needed, d.dofs_pad / 32);
@@ -416,6 +419,7 @@ int enqueue_forward(hddm_lb_ctx* ctx, Scratch& sc, int64_t npoints, const double
 	const void* kern = sweep_kernel_ptr(prec, cfg.slab, xbanks_for(ctx->dim), ctx->fmax_all);
 	if (!kern) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "forward: no sweep kernel for precision %d, dim %d", prec, ctx->dim);
 	const size_t lds = sweep_lds_bytes(prec, cfg.waves, ctx->fmax_all);
+	HIP_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
 	int per_cu = 0;
 	HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, cfg.waves * kWave, lds));
 	if (per_cu < 1)

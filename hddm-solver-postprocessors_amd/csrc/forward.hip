@@ -22,7 +22,7 @@
 //                    contracts the surviving nodes against the surplus
 //                    (FP64 VALU, or MFMA on 16-node chunks with split operands)
 //   6. reduce        sums node-segment partials of split states (fixed order)
-#include "device_common.h"
+#include "sweep_common.h"
 #include <type_traits>
 #include "kernels.h"
 
@@ -33,7 +33,6 @@ typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
 typedef __bf16 b2_t __attribute__((ext_vector_type(2)));
 typedef __bf16 b8_t __attribute__((ext_vector_type(8)));
 typedef float f16_t __attribute__((ext_vector_type(16)));
-typedef uint32_t u4_t __attribute__((ext_vector_type(4)));
 
 __global__ void fill_u32_kernel(uint32_t* p, int64_t n, uint32_t v)
 {
@@ -147,63 +146,6 @@ __device__ __forceinline__ V as_vec8(uint32_t a, uint32_t b, uint32_t c, uint32_
 {
 	u4_t u = { a, b, c, d };
 	return __builtin_bit_cast(V, u);
-}
-
-__device__ __forceinline__ double f64_from_high(uint32_t hi)
-{
-	return __builtin_bit_cast(double, (uint64_t)hi << 32);
-}
-
-// One work item: G neighbouring 64-point blocks of one state (one block per
-// wave of the workgroup) against one node segment of that state.
-struct ItemInfo
-{
-	int s, seg;
-	int64_t grp, start, count;
-};
-
-__device__ __forceinline__ void decode_item(const SweepArgs& a, int64_t item, int G, ItemInfo& it)
-{
-	int s = 0;
-	int64_t base = 0, ngrp = 1;
-	for (int i = 0; i < a.nstates; i++)
-	{
-		const int si = ldc(&a.cost_order[i]);
-		const int64_t ib = ldc(&a.plans[si].item_base);
-		const int64_t nb = ldc(&a.plans[si].nblk);
-		const int64_t ng = (nb + G - 1) / G;
-		const int ns = ldc(&a.states[si].nseg);
-		if (item < ib + ng * ns) { s = si; base = ib; ngrp = ng; break; }
-	}
-	const int64_t local = item - base;
-	it.s = s;
-	it.seg = (int)(local / ngrp);
-	it.grp = local - (int64_t)it.seg * ngrp;
-	it.start = ldc(&a.plans[s].start);
-	it.count = ldc(&a.plans[s].count);
-}
-
-// Coordinates of the lane's point are held in XB register banks of 16
-// doubles (separate arrays); a bank is indexed with a wave-uniform dim
-// (s_set_gpr_idx), never through scratch.
-template <int XB>
-__device__ __forceinline__ double xget(const double (&b0)[16], const double (&b1)[16], const double (&b2)[16],
-	const double (&b3)[16], uint32_t dim)
-{
-	const uint32_t idx = dim & 15u;
-	if constexpr (XB == 1) return b0[idx];
-	else
-	{
-		const uint32_t bank = dim >> 4;
-		if (bank == 0) return b0[idx];
-		if constexpr (XB == 2) return b1[idx];
-		else
-		{
-			if (bank == 1) return b1[idx];
-			if (bank == 2) return b2[idx];
-			return b3[idx];
-		}
-	}
 }
 
 // S^T operands (the MFMA A side) of one 16-node chunk, loaded ahead of use.
@@ -363,29 +305,6 @@ __device__ __forceinline__ void contract_chunk(const SOperands<PREC, DT>& so, in
 }
 
 } // namespace
-
-// Ops per LDS tile of the op stream; the workgroup double-buffers tiles.
-constexpr int kTileOps = 512;
-
-// Workgroup LDS: [item slot 16 B][ops 2 x kTileOps x 16 B][skip 2 x kTileOps x 4 B]
-// then per wave the path stack [PM][64] doubles and the chunk buffer [16][64] BT.
-constexpr size_t kLdsHeader = 16;
-constexpr size_t kLdsTiles = 2 * (size_t)kTileOps * (sizeof(Op) + sizeof(uint32_t));
-
-// Issues the copy of ops [tb, tb + kTileOps) and their skips into one tile
-// buffer, spread over the G waves of the workgroup (one 64-op row per wave
-// and instruction). Rows past the end of the stream read its sentinel.
-__device__ __forceinline__ void load_tile(const Op* gops, const uint32_t* gskip, int nops, int tb,
-	Op* tops, uint32_t* tskip, int wave, int G, int lane)
-{
-	for (int r = wave; r < kTileOps / kWave; r += G)
-	{
-		int q = tb + r * kWave + lane;
-		q = q < nops ? q : nops;
-		__builtin_amdgcn_global_load_lds(gops + q, tops + r * kWave, 16, 0, 0);
-		__builtin_amdgcn_global_load_lds(gskip + q, tskip + r * kWave, 4, 0, 0);
-	}
-}
 
 // Walk state of one wave. P[t] (the product of the first t factors of the
 // current trie path) lives in the wave's LDS stage; the two most recent

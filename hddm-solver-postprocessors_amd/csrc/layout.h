@@ -48,6 +48,7 @@ struct StateDesc
 	const int32_t* order;           // trie position -> uploaded node index
 	const uint64_t* lomask;         // [nno] dims whose support lies in [0, 1/2]
 	const uint64_t* himask;         // [nno] dims whose support lies in [1/2, 1]
+	int64_t ds_offset;              // first element of the state's rows in an adjoint dsurplus
 };
 
 // Per-state plan of one forward call, written on the device by the plan kernel.

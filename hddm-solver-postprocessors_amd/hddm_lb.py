@@ -307,7 +307,9 @@ class Context:
                                         precision))
         return y
 
-    def adjoint(self, x, gbar, dx=None, dsurplus=None, state_ids=None, precision=F16X2, stream=None):
+    def adjoint(self, x, gbar, dx=None, dsurplus=None, state_ids=None, precision=FP64, stream=None):
+        """x: device FP64 [n, >=dim]; gbar: device FP64 [n, >=dofs]; dx: device FP64 [n, >=dim] or None;
+        dsurplus: device FP64 [total] (state s at surplus_offset(s)) or None. Both outputs are overwritten."""
         n = x.shape[0]
         _check(lib.hddm_lb_adjoint(self._h, n, _ptr(x), x.stride(0), _ptr(state_ids), _ptr(gbar), gbar.stride(0),
                                    _ptr(dx), dx.stride(0) if dx is not None else 0, _ptr(dsurplus), precision,

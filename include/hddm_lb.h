@@ -106,16 +106,19 @@ int hddm_lb_forward_host(hddm_lb_ctx* ctx, int64_t npoints,
 	const double* x, int64_t ldx, const int32_t* state_ids,
 	double* y, int64_t ldy, int precision);
 
-/* Adjoint (device pointers): given gbar [npoints x ldg] FP32,
- *   dx        [npoints x lddx] FP32  = sum_k (gbar . S_k) dB_k/dx
- *   dsurplus  FP32, per state s at element offset hddm_lb_surplus_offset(s),
- *             rows in the uploaded node order, row length dofs(s):
- *             dS_k = sum_p B_k(x_p) gbar_p   (overwritten)
- * dx or dsurplus may be NULL to skip that output. */
+/* Adjoint (device pointers; no counterpart in the reference). For the
+ * forward y_p = sum_k B_k(x_p) S_k and gbar [npoints x ldg] FP64 = dL/dy:
+ *   dx        [npoints x lddx] FP64: dx_p = sum_k (gbar_p . S_k) dB_k/dx_p with the
+ *             piecewise-constant hat derivative -i sign(x i - j) on the support
+ *   dsurplus  FP64, state s at element offset hddm_lb_surplus_offset(s), rows in
+ *             the uploaded node order, row length dofs(s): dS_k = sum_p B_k(x_p) gbar_p
+ * Both outputs are overwritten; either may be NULL. The basis is recomputed.
+ * dsurplus is summed with FP64 atomics, so its last bits vary between runs.
+ * precision: HDDM_LB_FP64. Stream-ordered. */
 int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 	const double* x, int64_t ldx, const int32_t* state_ids,
-	const float* gbar, int64_t ldg, float* dx, int64_t lddx,
-	float* dsurplus, int precision, void* stream);
+	const double* gbar, int64_t ldg, double* dx, int64_t lddx,
+	double* dsurplus, int precision, void* stream);
 
 int hddm_lb_surplus_offset(hddm_lb_ctx* ctx, int istate, int64_t* offset, int64_t* total);
 

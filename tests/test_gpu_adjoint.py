@@ -1,0 +1,140 @@
+"""GPU parity of the adjoint (dx, dsurplus) through the C ABI.
+
+Checks, in FP64:
+  * against the oracle's analytic adjoint (gold basis, the product of the
+    other factors formed directly): dx and dS to 1e-10 normwise;
+  * dx against central finite differences of the oracle forward (h = 1e-4,
+    points at least 1e-3 from every kink, where the interpolant is linear in
+    each coordinate, so the difference quotient is exact up to rounding);
+  * dS against B^T g written out from the oracle's basis on a tiny grid;
+  * the torch.autograd.Function harness on the same inputs.
+The dsurplus sums use FP64 atomics; their order varies, hence 1e-12 and not
+bitwise.
+"""
+import numpy as np
+import pytest
+
+import hddm_lb
+import oracle
+from gpu_util import normwise_rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a device"
+    return torch
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def run_adjoint(torch, ctx, X, G, sid=None):
+    x = torch.from_numpy(X).cuda()
+    g = torch.from_numpy(G).cuda()
+    dx = torch.full(X.shape, np.nan, dtype=torch.float64, device="cuda")
+    _, total = ctx.surplus_offset(-1)
+    ds = torch.full((total,), np.nan, dtype=torch.float64, device="cuda")
+    s = None if sid is None else torch.from_numpy(np.asarray(sid, np.int32)).cuda()
+    ctx.adjoint(x, g, dx, ds, s, hddm_lb.FP64)
+    torch.cuda.synchronize()
+    return dx.cpu().numpy(), ds.cpu().numpy()
+
+
+def grid_case(dim, start, maxl, p, seed, target, dofs, n, kink_level):
+    ii, jj, lv, _ = hddm_lb.generate_grid(dim, start, maxl, p, seed, target)
+    S = hddm_lb.synthetic_surplus(lv, dofs, seed + 1)
+    X = hddm_lb.synthetic_points(n, dim, seed + 2, 0.01, 0.99, kink_level=kink_level, kink_gap=1e-3)
+    G = np.random.default_rng(seed + 3).standard_normal((n, dofs))
+    return ii, jj, S, X, G
+
+
+@pytest.mark.parametrize("dim,dofs,n", [(6, 20, 300), (16, 32, 200), (20, 40, 130)])
+def test_adjoint_matches_oracle(torch_mod, dim, dofs, n):
+    ii, jj, S, X, G = grid_case(dim, 2, 5, 0.4, 3 + dim, 2500, dofs, n, 5)
+    ctx = hddm_lb.Context(0)
+    ctx.upload_state(0, ii, jj, S)
+    say("adjoint dim", dim, "nno", ii.shape[0], "dofs", dofs, "n", n)
+    dx, ds = run_adjoint(torch_mod, ctx, X, G)
+    dxo, dso = oracle.adjoint(ii, jj, S, X, G)
+    ds = ds.reshape(ii.shape[0], dofs)
+    say("  dx err", normwise_rel_err(dx, dxo), "ds err", normwise_rel_err(ds, dso))
+    assert np.isfinite(dx).all() and np.isfinite(ds).all()
+    assert normwise_rel_err(dx, dxo) <= 1e-10
+    assert normwise_rel_err(ds, dso) <= 1e-12
+    ctx.close()
+
+
+def test_adjoint_dx_matches_central_differences(torch_mod):
+    ii, jj, S, X, G = grid_case(5, 2, 5, 0.5, 41, 1500, 8, 40, 5)
+    ctx = hddm_lb.Context(0)
+    ctx.upload_state(0, ii, jj, S)
+    dx, _ = run_adjoint(torch_mod, ctx, X, G)
+    h = 1e-4
+    fd = np.zeros_like(X)
+    for d in range(X.shape[1]):
+        Xp, Xm = X.copy(), X.copy()
+        Xp[:, d] += h
+        Xm[:, d] -= h
+        yp = oracle.forward(ii, jj, S, Xp)
+        ym = oracle.forward(ii, jj, S, Xm)
+        fd[:, d] = ((yp - ym) * G).sum(axis=1) / (2 * h)
+    say("fd err", normwise_rel_err(dx, fd))
+    assert normwise_rel_err(dx, fd) <= 1e-8
+    ctx.close()
+
+
+def test_adjoint_ds_is_basis_transpose_times_g(torch_mod):
+    # tiny grid: B from the oracle forward with unit surpluses, dS = B^T g
+    ii, jj, S, X, G = grid_case(3, 2, 4, 0.5, 5, 0, 3, 70, 4)
+    nno = ii.shape[0]
+    ctx = hddm_lb.Context(0)
+    ctx.upload_state(0, ii, jj, S)
+    _, ds = run_adjoint(torch_mod, ctx, X, G)
+    B = oracle.forward(ii, jj, np.eye(nno), X)  # [n, nno]
+    np.testing.assert_allclose(ds.reshape(nno, 3), B.T @ G, rtol=0, atol=1e-13 * np.abs(B.T @ G).max())
+    ctx.close()
+
+
+def test_adjoint_multistate_offsets(torch_mod):
+    cases = [grid_case(6, 2, 4, 0.5, 60 + s, 600 + 300 * s, d, 100, 4) for s, d in enumerate((8, 20, 33))]
+    ctx = hddm_lb.Context(0)
+    for s, (ii, jj, S, _, _) in enumerate(cases):
+        ctx.upload_state(s, ii, jj, S)
+    X = np.concatenate([c[3] for c in cases])
+    sid = np.repeat(np.arange(3), 100).astype(np.int32)
+    rng = np.random.default_rng(7)
+    perm = rng.permutation(X.shape[0])
+    X, sid = X[perm], sid[perm]
+    G = rng.standard_normal((X.shape[0], 33))
+    dx, ds = run_adjoint(torch_mod, ctx, X, G, sid)
+    for s, (ii, jj, S, _, _) in enumerate(cases):
+        sel = sid == s
+        dofs = S.shape[1]
+        dxo, dso = oracle.adjoint(ii, jj, S, X[sel], G[sel, :dofs])
+        off, _ = ctx.surplus_offset(s)
+        dss = ds[off:off + ii.shape[0] * dofs].reshape(ii.shape[0], dofs)
+        say("state", s, "dx err", normwise_rel_err(dx[sel], dxo), "ds err", normwise_rel_err(dss, dso))
+        assert normwise_rel_err(dx[sel], dxo) <= 1e-10
+        assert normwise_rel_err(dss, dso) <= 1e-12
+    ctx.close()
+
+
+def test_autograd_harness(torch_mod):
+    torch = torch_mod
+    import hddm_lb_autograd
+    ii, jj, S, X, G = grid_case(8, 2, 4, 0.5, 77, 1200, 16, 150, 4)
+    ctx = hddm_lb.Context(0)
+    ctx.upload_state(0, ii, jj, S)
+    x = torch.from_numpy(X).cuda().requires_grad_(True)
+    s = torch.from_numpy(S).cuda().requires_grad_(True)
+    y = hddm_lb_autograd.interpolate(ctx, x, s)
+    (y * torch.from_numpy(G).cuda()).sum().backward()
+    dxo, dso = oracle.adjoint(ii, jj, S, X, G)
+    assert normwise_rel_err(y.detach().cpu().numpy(), oracle.forward(ii, jj, S, X)) <= 1e-12
+    assert normwise_rel_err(x.grad.cpu().numpy(), dxo) <= 1e-10
+    assert normwise_rel_err(s.grad.cpu().numpy(), dso) <= 1e-12
+    ctx.close()
