@@ -100,6 +100,8 @@ def main():
     ap.add_argument("--points", type=int, default=1 << 20)
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and error check")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device-to-device 3 times (calibrates profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (a forward shards by points; "
@@ -109,6 +111,14 @@ def main():
     import numpy as np
     import torch
     import hddm_lb
+
+    if args.copy_probe > 0:
+        src = torch.ones(args.copy_probe << 17, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     precs = {"f16x2": hddm_lb.F16X2, "bf16x3": hddm_lb.BF16X3, "f32": hddm_lb.F32, "fp64": hddm_lb.FP64}
     ii, jj, S, X = build_workload(args.points)

@@ -191,19 +191,23 @@ struct SOperands
 
 // MFMA contraction of one 16-node chunk: acc[t][dt] (points 32t.., dofs
 // 32(dt0+dt)..) += S^T[dofs x 16 nodes] * B^T[16 nodes x 32 points].
+constexpr float kBScale = 4096.0f;
+
 template <int PREC, int DT>
 __device__ __forceinline__ void contract_chunk(const SOperands<PREC, DT>& so, int ndt,
-	const float (&b)[kChunk], f16_t (&acc)[2][DT])
+	const double (&b)[kChunk], f16_t (&acc)[2][DT])
 {
 	if constexpr (PREC == HDDM_LB_F16X2)
 	{
+		// B is scaled by 2^12 before the split so that the low halves of
+		// basis values down to 2^-12 stay normal f16 numbers
 		uint32_t hi[8], lo[8];
 #pragma unroll
 		for (int m = 0; m < 8; m++)
 		{
-			const float f0 = b[2 * m], f1 = b[2 * m + 1];
-			hi[m] = pack_h2(f0, f1);
-			lo[m] = pack_h2(f0 - lo_half_f16(hi[m]), f1 - hi_half_f16(hi[m]));
+			const double d0 = b[2 * m] * (double)kBScale, d1 = b[2 * m + 1] * (double)kBScale;
+			hi[m] = pack_h2((float)d0, (float)d1);
+			lo[m] = pack_h2((float)(d0 - (double)lo_half_f16(hi[m])), (float)(d1 - (double)hi_half_f16(hi[m])));
 		}
 		// Lanes 0-31 own points 0-31, lanes 32-63 points 32-63; each holds
 		// nodes 0-15 of its point. The B operand of points 0-31 needs nodes
@@ -239,11 +243,11 @@ __device__ __forceinline__ void contract_chunk(const SOperands<PREC, DT>& so, in
 #pragma unroll
 		for (int m = 0; m < 8; m++)
 		{
-			const float f0 = b[2 * m], f1 = b[2 * m + 1];
-			t1[m] = pack_b2(f0, f1);
-			const float r0 = f0 - lo_half_bf16(t1[m]), r1 = f1 - hi_half_bf16(t1[m]);
-			t2[m] = pack_b2(r0, r1);
-			t3[m] = pack_b2(r0 - lo_half_bf16(t2[m]), r1 - hi_half_bf16(t2[m]));
+			const double d0 = b[2 * m], d1 = b[2 * m + 1];
+			t1[m] = pack_b2((float)d0, (float)d1);
+			const double r0 = d0 - (double)lo_half_bf16(t1[m]), r1 = d1 - (double)hi_half_bf16(t1[m]);
+			t2[m] = pack_b2((float)r0, (float)r1);
+			t3[m] = pack_b2((float)(r0 - (double)lo_half_bf16(t2[m])), (float)(r1 - (double)hi_half_bf16(t2[m])));
 		}
 #pragma unroll
 		for (int m = 0; m < 4; m++)
@@ -285,8 +289,8 @@ __device__ __forceinline__ void contract_chunk(const SOperands<PREC, DT>& so, in
 #pragma unroll
 		for (int q = 0; q < 8; q++)
 		{
-			auto r = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, b[2 * q]),
-				__builtin_bit_cast(uint32_t, b[2 * q + 1]), false, false);
+			auto r = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, (float)b[2 * q]),
+				__builtin_bit_cast(uint32_t, (float)b[2 * q + 1]), false, false);
 			B0[q] = __builtin_bit_cast(float, (uint32_t)r[0]);
 			B1[q] = __builtin_bit_cast(float, (uint32_t)r[1]);
 		}
@@ -401,7 +405,7 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 	// wave-uniform values the compiler cannot see as uniform by itself
 	const int wave = uniform((int)(threadIdx.x >> 6));
 	const int G = uniform((int)(blockDim.x >> 6));
-	typedef typename std::conditional<PREC == HDDM_LB_FP64, double, float>::type BT;
+	typedef double BT;  // basis values of the chunk stay FP64 until the operand split
 	char* const lbase = (char*)lds;
 	int64_t* const item_slot = (int64_t*)lbase;
 	Op* const tile_ops = (Op*)(lbase + kLdsHeader);
@@ -544,7 +548,7 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 							for (int r = 0; r < 16; r++) { acc64[t][dt][r] += (double)acc[t][dt][r]; acc[t][dt][r] = 0.0f; }
 					cur_win = win;
 				}
-				float b[kChunk];
+				double b[kChunk];
 #pragma unroll
 				for (int kk = 0; kk < kChunk; kk++) { b[kk] = Bl[kk * kWave + lane]; Bl[kk * kWave + lane] = 0.0f; }
 				contract_chunk<PREC, DT>(so, ndt, b, acc);
@@ -602,7 +606,7 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 								if (dof < dofs)
 								{
 									double v = acc64[t][dt][r];
-									if constexpr (PREC == HDDM_LB_F16X2) v *= ldc(iscale + dof);
+									if constexpr (PREC == HDDM_LB_F16X2) v *= ldc(iscale + dof) * (1.0 / kBScale);
 									out[dof] = v;
 								}
 							}
@@ -623,8 +627,8 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 size_t sweep_lds_bytes(int prec, int waves, int depth)
 {
 	const size_t pm = depth < 8 ? 8 : 16;
-	const size_t bt = prec == HDDM_LB_FP64 ? 8 : 4;
-	return kLdsHeader + kLdsTiles + (size_t)waves * (pm * kWave * 8 + kChunk * kWave * bt);
+	(void)prec;
+	return kLdsHeader + kLdsTiles + (size_t)waves * (pm * kWave * 8 + kChunk * kWave * 8);
 }
 
 __global__ void reset_counter_kernel(PlanHeader* hdr)

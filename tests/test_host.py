@@ -206,6 +206,68 @@ def test_layout_trie_walk_matches_oracle(dim, start, level, target, seed):
         np.testing.assert_allclose(yw, yo, rtol=1e-12, atol=1e-12 * np.abs(yo).max())
 
 
+def walk_adjoint(L, X, G, S_sorted):
+    """CPU emulation of one wave of the adjoint sweep: the trie walk with the
+    per-edge sums R_e = sum_{k below e} w_k B_k, flushed into dx[dim(e)] with
+    phi'/phi when the walk leaves e. Returns dx [64, dim], dS [nno, dofs] in
+    trie order."""
+    n, dim = X.shape
+    P = [np.ones(n) for _ in range(17)]
+    Q = [np.zeros(n) for _ in range(16)]
+    R = [np.zeros(n) for _ in range(16)]
+    dims = [0] * 16
+    dx = np.zeros((n, dim))
+    ds = np.zeros((L.nno, L.dofs))
+    top, q = -1, 0
+
+    def leave(t):
+        nonlocal top
+        for tt in range(top, t - 1, -1):
+            dx[:, dims[tt]] += R[tt] * Q[tt]
+            if tt > 0:
+                R[tt - 1] += R[tt]
+        top = t
+
+    while q < L.nops:
+        t = L.op_depth[q]
+        leave(t)
+        iv = L.op_i[q]
+        u = X[:, L.op_dim[q]] * iv + L.op_negj[q]
+        phi = 1.0 - np.abs(u)
+        p = P[t] * np.maximum(phi, 0.0)
+        P[t + 1] = p
+        with np.errstate(divide="ignore", invalid="ignore"):
+            Q[t] = np.where(phi > 0, np.where(u > 0, -iv, np.where(u < 0, iv, 0.0)) / phi, 0.0)
+        R[t] = np.zeros(n)
+        dims[t] = L.op_dim[q]
+        if not p.any():
+            q = int(L.skip[q])
+            continue
+        if L.op_last[q]:
+            k = L.op_node[q]
+            w = G @ S_sorted[k]
+            R[t] = w * p
+            ds[k] += p @ G
+        q += 1
+    leave(0)
+    return dx, ds
+
+
+@pytest.mark.parametrize("dim,start,level,target,seed", [(3, 3, 4, 0, 4), (6, 2, 5, 1500, 5), (12, 2, 6, 3000, 6)])
+def test_adjoint_edge_sums_match_oracle(dim, start, level, target, seed):
+    ii, jj, lv, ix = hddm_lb.generate_grid(dim, start, level, 0.4, seed, target)
+    S = hddm_lb.synthetic_surplus(lv, 4, seed)
+    L = hddm_lb.Layout(ii, jj, S)
+    X = hddm_lb.synthetic_points(64, dim, seed, 0.01, 0.99, kink_level=level, kink_gap=1e-3)
+    G = np.random.default_rng(seed).standard_normal((64, 4))
+    dx, ds_t = walk_adjoint(L, X, G, S[L.order])
+    dxo, dso = oracle.adjoint(ii, jj, S, X, G)
+    ds = np.zeros_like(ds_t)
+    ds[L.order] = ds_t
+    np.testing.assert_allclose(dx, dxo, rtol=0, atol=1e-11 * np.abs(dxo).max())
+    np.testing.assert_allclose(ds, dso, rtol=0, atol=1e-13 * np.abs(dso).max())
+
+
 def test_layout_rejects_bad_input():
     with pytest.raises(hddm_lb.HddmLbError) as e:
         hddm_lb.Layout(np.zeros((0, 3), np.int32), np.zeros((0, 3), np.int32), np.zeros((0, 2)))
