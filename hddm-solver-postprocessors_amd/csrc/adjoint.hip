@@ -61,7 +61,7 @@ namespace {
 // dS of one finished chunk: D[dof][node] += sum_pt g[pt][dof] B[node][pt],
 // 16 dofs x 16 nodes per v_mfma_f64_16x16x4f64 tile, 16 steps of 4 points.
 // Operand layouts: A[i][k] from lane i + 16k, B[k][j] from lane j + 16k;
-// D[i][j] in lane j + 16 (i / 4), element i % 4.
+// D[i][j] in lane j + 16 (i % 4), element i / 4.
 __device__ __forceinline__ void contract_ds(const AdjointArgs& a, const StateDesc* sd, const double* Bc,
 	uint32_t mask, int chunk, int lane, const uint32_t (&og)[16], uint32_t ogvalid, int dofs)
 {
@@ -93,7 +93,7 @@ __device__ __forceinline__ void contract_ds(const AdjointArgs& a, const StateDes
 #pragma unroll
 			for (int r = 0; r < 4; r++)
 			{
-				const int dof = dbase + 4 * kq + r;
+				const int dof = dbase + kq + 4 * r;
 				if (dof < dofs) unsafeAtomicAdd(ds + (int64_t)orow * dofs + dof, acc[r]);
 			}
 		}
@@ -344,7 +344,7 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 	size_t per_wave = 0;
 	const void* kern = adjoint_kernel_ptr(xbanks_for(ctx->dim), ctx->fmax_all, per_wave);
 	if (!kern) return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: dim %d is not supported", ctx->dim);
-	const size_t fixed = sweep_lds_bytes(HDDM_LB_F32, 0, 0);
+	const size_t fixed = sweep_lds_bytes(HDDM_LB_F32, 0, 0, kWalkLds);
 	int G = (int)std::min<size_t>(4, (ctx->max_lds - fixed) / per_wave);
 	if (ctx->opt_waves) G = std::min(G, ctx->opt_waves);
 	if (G < 1) return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: LDS of one wave (%zu B) exceeds the CU", per_wave);

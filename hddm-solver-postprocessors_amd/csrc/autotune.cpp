@@ -1,9 +1,10 @@
 // Autotuner of the sweep's launch shape, keyed by (precision, D, total nodes,
 // max dofs, log2 of the point count). On the first forward of a key it times
-// every candidate (waves per workgroup x dof tiles per pass) on the caller's
+// every candidate (op source x waves per workgroup x dof tiles per pass) on the caller's
 // data with hipEvents and keeps the fastest. Tuning runs write y like the
 // forward itself, so the caller's output is valid afterwards.
 #include "context.h"
+#include "kernels.h"
 #include "errors.h"
 
 #include <algorithm>
@@ -25,17 +26,22 @@ int autotune(hddm_lb_ctx* ctx, int64_t npoints, const double* x, int64_t ldx, co
 	if (hit != ctx->tuned.end()) { cfg = hit->second; return HDDM_LB_SUCCESS; }
 
 	std::vector<TuneConfig> cands;
-	for (int waves : { 1, 2, 4 })
+	for (int walk : { kWalkSmem, kWalkLds })
 	{
-		if (ctx->opt_waves && waves != ctx->opt_waves) continue;
-		for (int slab : { 1, 2 })
+		if (ctx->opt_walk && walk != (ctx->opt_walk == 2 ? kWalkLds : kWalkSmem)) continue;
+		for (int waves : { 1, 2, 4 })
 		{
-			if (slab > ctx->dt_needed) continue;
-			if (ctx->opt_slab && slab != ctx->opt_slab) continue;
-			TuneConfig c = cfg;
-			c.waves = waves;
-			c.slab = slab;
-			cands.push_back(c);
+			if (ctx->opt_waves && waves != ctx->opt_waves) continue;
+			for (int slab : { 1, 2 })
+			{
+				if (slab > ctx->dt_needed || (prec == HDDM_LB_FP64 && slab > 1)) continue;
+				if (ctx->opt_slab && slab != ctx->opt_slab) continue;
+				TuneConfig c = cfg;
+				c.walk = walk;
+				c.waves = waves;
+				c.slab = slab;
+				cands.push_back(c);
+			}
 		}
 	}
 	if (cands.empty()) { ctx->tuned[key] = cfg; return HDDM_LB_SUCCESS; }

@@ -166,6 +166,9 @@ extern "C" int hddm_lb_set_option(hddm_lb_ctx* ctx, int option, int64_t value)
 			return set_error(HDDM_LB_ERROR_INVALID_VALUE, "set_option: slab tiles must be 0, 1 or 2");
 		ctx->opt_slab = (int)value; break;
 	case HDDM_LB_OPT_STATS: ctx->opt_stats = value != 0; break;
+	case HDDM_LB_OPT_WALK:
+		if (value < 0 || value > 2) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "set_option: walk must be 0, 1 or 2");
+		ctx->opt_walk = (int)value; break;
 	case HDDM_LB_OPT_FLUSH_LOG2:
 		if (value < 0 || value > 20) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "set_option: flush log2 in [0, 20]");
 		ctx->opt_wlog = (int)value; break;
@@ -393,6 +396,7 @@ TuneConfig default_config(hddm_lb_ctx* ctx, int prec)
 {
 	TuneConfig c;
 	c.waves = ctx->opt_waves ? ctx->opt_waves : 4;
+	c.walk = ctx->opt_walk == 2 ? kWalkLds : kWalkSmem;
 	int slab = ctx->opt_slab;
 	if (!slab || prec == HDDM_LB_FP64) slab = 1;
 	c.slab = slab;
@@ -413,12 +417,12 @@ int enqueue_forward(hddm_lb_ctx* ctx, Scratch& sc, int64_t npoints, const double
 	HIP_TRY(launch_fill_u32(sc.hist, nb, 0u, st));
 	HIP_TRY(launch_bucket_key(npoints, x, ldx, sid, ctx->d_state_ok, ns, ctx->dim, bb, sc.bkey, sc.hist, st));
 	HIP_TRY(launch_scan_plan(sc.hist, sc.cursor, nb, bb, ns, ctx->d_states, ctx->d_cost_order,
-		sc.plans, sc.hdr, cfg.waves, st));
+		sc.plans, sc.hdr, cfg.walk == kWalkLds ? cfg.waves : 1, st));
 	HIP_TRY(launch_scatter(npoints, sc.bkey, sc.cursor, sc.perm, st));
 
-	const void* kern = sweep_kernel_ptr(prec, cfg.slab, xbanks_for(ctx->dim), ctx->fmax_all);
+	const void* kern = sweep_kernel_ptr(prec, cfg.slab, xbanks_for(ctx->dim), ctx->fmax_all, cfg.walk);
 	if (!kern) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "forward: no sweep kernel for precision %d, dim %d", prec, ctx->dim);
-	const size_t lds = sweep_lds_bytes(prec, cfg.waves, ctx->fmax_all);
+	const size_t lds = sweep_lds_bytes(prec, cfg.waves, ctx->fmax_all, cfg.walk);
 	HIP_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
 	int per_cu = 0;
 	HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, cfg.waves * kWave, lds));

@@ -312,9 +312,12 @@ __device__ __forceinline__ void contract_chunk(const SOperands<PREC, DT>& so, in
 
 // Walk state of one wave. P[t] (the product of the first t factors of the
 // current trie path) lives in the wave's LDS stage; the two most recent
-// products stay in registers, so only a move up the trie reads LDS. Ops are
-// read from the workgroup's LDS tile with a wave-uniform address.
-template <int XB, typename BT>
+// products stay in registers, so only a move up the trie reads LDS.
+// WALK selects where ops come from:
+//   kWalkSmem  scalar loads from the state's op stream, the sequential
+//              successor read one op ahead; waves are independent
+//   kWalkLds   the workgroup's LDS tile (wave-uniform vector reads)
+template <int XB, typename BT, int WALK>
 struct Walker
 {
 	double* P;          // [PM][64] in LDS
@@ -326,8 +329,13 @@ struct Walker
 	int chunk;          // chunk whose positions B holds (-1: none)
 	uint32_t mask;      // positions of B written in this chunk
 	uint32_t nops, nprunes;  // counts for measurement
+	const Op* gops;     // kWalkSmem: the op stream and skips
+	const uint32_t* gskip;
+	Op op;              // kWalkSmem: op q and its skip, in SGPRs
+	uint32_t sk;
 
-	__device__ __forceinline__ void init(int q0, int qend, double* Pl, BT* Bl, int lane)
+	__device__ __forceinline__ void init(int q0, int qend, double* Pl, BT* Bl, int lane,
+		const Op* ops, const uint32_t* skip)
 	{
 		q = q0;
 		q1 = qend;
@@ -341,6 +349,31 @@ struct Walker
 		mask = 0;
 		nops = 0;
 		nprunes = 0;
+		gops = ops;
+		gskip = skip;
+		if constexpr (WALK == kWalkSmem)
+		{
+			op = ldc_op(ops + q);
+			sk = ldc(skip + q);
+		}
+	}
+
+	// One factor at depth t: the new prefix product.
+	__device__ __forceinline__ double step(int t, uint32_t dim, uint32_t ihi, uint32_t njhi, int lane,
+		const double (&xb0)[16], const double (&xb1)[16], const double (&xb2)[16], const double (&xb3)[16])
+	{
+		double bt;
+		if (t == last_t + 1) bt = cur;
+		else if (t == last_t) bt = base_prev;
+		else bt = P[t * kWave + lane];
+		const double tt = fma(xget<XB>(xb0, xb1, xb2, xb3, dim), f64_from_high(ihi), f64_from_high(njhi));
+		const double p = bt * fmax(1.0 - fabs(tt), 0.0);
+		P[(t + 1) * kWave + lane] = p;
+		base_prev = bt;
+		cur = p;
+		last_t = t;
+		nops++;
+		return p;
 	}
 
 	// Walks ops below tend (the end of the resident tile or of the segment),
@@ -353,31 +386,45 @@ struct Walker
 	{
 		while (q < tend)
 		{
-			const int sl = q - tb;
-			const u4_t ov = *(const u4_t*)(tops + sl);
-			const uint32_t dim = __builtin_amdgcn_readfirstlane(ov[0]);
-			const uint32_t ctrl = __builtin_amdgcn_readfirstlane(ov[1]);
-			const uint32_t ihi = __builtin_amdgcn_readfirstlane(ov[2]);
-			const uint32_t njhi = __builtin_amdgcn_readfirstlane(ov[3]);
-			const int t = (int)(ctrl & 15u);
-			double bt;
-			if (t == last_t + 1) bt = cur;
-			else if (t == last_t) bt = base_prev;
-			else bt = P[t * kWave + lane];
-			const double tt = fma(xget<XB>(xb0, xb1, xb2, xb3, dim), f64_from_high(ihi), f64_from_high(njhi));
-			const double p = bt * fmax(1.0 - fabs(tt), 0.0);
-			P[(t + 1) * kWave + lane] = p;
-			base_prev = bt;
-			cur = p;
-			last_t = t;
-			nops++;
+			uint32_t dim, ctrl, ihi, njhi, nsk = 0;
+			Op nop;
+			int sl = 0;
+			if constexpr (WALK == kWalkLds)
+			{
+				sl = q - tb;
+				const u4_t ov = *(const u4_t*)(tops + sl);
+				dim = __builtin_amdgcn_readfirstlane(ov[0]);
+				ctrl = __builtin_amdgcn_readfirstlane(ov[1]);
+				ihi = __builtin_amdgcn_readfirstlane(ov[2]);
+				njhi = __builtin_amdgcn_readfirstlane(ov[3]);
+			}
+			else
+			{
+				// the stream and the skips end with a sentinel, so q + 1 is in bounds
+				nop = ldc_op(gops + q + 1);
+				nsk = ldc(gskip + q + 1);
+				dim = op.dim; ctrl = op.ctrl; ihi = op.ihi; njhi = op.njhi;
+			}
+			const double p = step((int)(ctrl & 15u), dim, ihi, njhi, lane, xb0, xb1, xb2, xb3);
 			if (!wave_any(p != 0.0))
 			{
 				nprunes++;
-				q = (int)__builtin_amdgcn_readfirstlane(tskip[sl]);
+				if constexpr (WALK == kWalkLds)
+					q = (int)__builtin_amdgcn_readfirstlane(tskip[sl]);
+				else
+				{
+					q = (int)sk;
+					op = ldc_op(gops + q);
+					sk = ldc(gskip + q);
+				}
 				continue;
 			}
 			q++;
+			if constexpr (WALK == kWalkSmem)
+			{
+				op = nop;
+				sk = nsk;
+			}
 			if (ctrl & kOpLast)
 			{
 				const int n = (int)(ctrl >> kOpNodeShift);
@@ -397,28 +444,44 @@ struct Walker
 	}
 };
 
-template <int PREC, int DT, int XB, int PM>
+// Per-wave LDS stage: the path stack [PM][64] and the chunk buffer [16][64], doubles.
+template <int PM>
+constexpr size_t wave_stage_bytes() { return (size_t)(PM + kChunk) * kWave * 8; }
+
+template <int PREC, int DT, int XB, int PM, int WALK>
 __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 {
 	extern __shared__ __attribute__((aligned(16))) double lds[];
 	const int lane = threadIdx.x & 63;
 	// wave-uniform values the compiler cannot see as uniform by itself
 	const int wave = uniform((int)(threadIdx.x >> 6));
-	const int G = uniform((int)(blockDim.x >> 6));
+	// blocks per item: the workgroup's waves with LDS tiles, one otherwise
+	const int G = WALK == kWalkLds ? uniform((int)(blockDim.x >> 6)) : 1;
 	typedef double BT;  // basis values of the chunk stay FP64 until the operand split
 	char* const lbase = (char*)lds;
 	int64_t* const item_slot = (int64_t*)lbase;
 	Op* const tile_ops = (Op*)(lbase + kLdsHeader);
 	uint32_t* const tile_skip = (uint32_t*)(lbase + kLdsHeader + 2 * (size_t)kTileOps * sizeof(Op));
-	double* const Pl = (double*)(lbase + kLdsHeader + kLdsTiles + (size_t)wave * (PM * kWave * 8 + kChunk * kWave * sizeof(BT)));
+	const size_t stage0 = WALK == kWalkLds ? kLdsHeader + kLdsTiles : 0;
+	double* const Pl = (double*)(lbase + stage0 + (size_t)wave * wave_stage_bytes<PM>());
 	BT* const Bl = (BT*)(Pl + PM * kWave);
 	const int64_t nitems = ldc(&a.hdr->nitems);
 
 	for (;;)
 	{
-		if (threadIdx.x == 0) *item_slot = (int64_t)atomicAdd(&a.hdr->counter, 1ull);
-		__syncthreads();
-		const int64_t item = uniform64((uint64_t)*item_slot);
+		int64_t item;
+		if constexpr (WALK == kWalkLds)
+		{
+			if (threadIdx.x == 0) *item_slot = (int64_t)atomicAdd(&a.hdr->counter, 1ull);
+			__syncthreads();
+			item = uniform64((uint64_t)*item_slot);
+		}
+		else
+		{
+			uint64_t v = 0;
+			if (lane == 0) v = atomicAdd(&a.hdr->counter, 1ull);
+			item = (int64_t)uniform64(v);
+		}
 		if (item >= nitems) break;
 		ItemInfo it;
 		decode_item(a, item, G, it);
@@ -434,10 +497,11 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 		const int dt0 = a.dt_base;
 		const int ndt = min(DT, (DP >> 5) - dt0);
 
-		// this wave's block; waves past the state's last block only help load tiles
-		const int64_t blk = it.grp * G + wave;
+		// this wave's block; with LDS tiles, waves past the state's last block only help load tiles
+		const int64_t blk = it.grp * G + (WALK == kWalkLds ? wave : 0);
 		const int64_t rem = it.count - blk * kWave;
 		const int np = ndt <= 0 ? 0 : (rem <= 0 ? 0 : (rem < kWave ? (int)rem : kWave));
+		if (WALK == kWalkSmem && np == 0) continue;
 		const int myp = lane < np ? lane : (np > 0 ? np - 1 : 0);
 		const uint32_t orig = np > 0 ? a.perm[it.start + blk * kWave + myp] : 0u;
 		const double* xrow = a.x + (int64_t)orig * a.ldx;
@@ -453,14 +517,19 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 #pragma unroll
 		for (int k = 0; k < kChunk; k++) Bl[k * kWave + lane] = (BT)0;
 
-		Walker<XB, BT> w;
-		w.init(q0, np > 0 ? q1 : q0, Pl, Bl, lane);
+		Walker<XB, BT, WALK> w;
+		w.init(q0, np > 0 ? q1 : q0, Pl, Bl, lane, gops, gskip);
 		uint32_t stat_chunks = 0;
 		const bool split = nseg > 1;
 		double* const part = a.partial + ldc(&a.plans[it.s].partial_base);
+		// one pass over [q0, q1) for scalar-load walks; tiles of kTileOps otherwise
+		const int tstep = WALK == kWalkLds ? kTileOps : q1 - q0;
 
-		load_tile(gops, gskip, nops_s, q0, tile_ops, tile_skip, wave, G, lane);
-		__syncthreads();
+		if constexpr (WALK == kWalkLds)
+		{
+			load_tile(gops, gskip, nops_s, q0, tile_ops, tile_skip, wave, G, lane);
+			__syncthreads();
+		}
 
 		if constexpr (PREC == HDDM_LB_FP64)
 		{
@@ -468,38 +537,8 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 			double acc[32];
 #pragma unroll
 			for (int c = 0; c < 32; c++) acc[c] = 0.0;
-			int k = 0;
-			for (int tb = q0; tb < q1; tb += kTileOps, k ^= 1)
-			{
-				if (tb + kTileOps < q1)
-					load_tile(gops, gskip, nops_s, tb + kTileOps, tile_ops + (k ^ 1) * kTileOps,
-						tile_skip + (k ^ 1) * kTileOps, wave, G, lane);
-				const int tend = min(tb + kTileOps, w.q1);
-				for (;;)
-				{
-					int node = 0;
-					double val = 0.0;
-					if (!w.run(tile_ops + k * kTileOps, tile_skip + k * kTileOps, tb, tend, lane,
-							xb0, xb1, xb2, xb3, node, val)) break;
-					// FP64 contraction of the finished chunk, positions in ascending order
-					if (w.chunk >= 0)
-					{
-						if (a.stats) stat_chunks++;
-						for (uint32_t m = w.mask; m; m &= m - 1)
-						{
-							const int kk = __builtin_ctz(m);
-							const double b = Bl[kk * kWave + lane];
-							const double* srow = s64 + (size_t)(w.chunk * kChunk + kk) * DP;
-#pragma unroll
-							for (int c = 0; c < 32; c++) acc[c] = fma(b, ldc(srow + c), acc[c]);
-						}
-					}
-					w.start_chunk(node, val, lane);
-				}
-				__syncthreads();
-			}
-			if (w.chunk >= 0)
-			{
+			// FP64 contraction of the finished chunk, positions in ascending order
+			auto contract = [&]() {
 				if (a.stats) stat_chunks++;
 				for (uint32_t m = w.mask; m; m &= m - 1)
 				{
@@ -509,7 +548,26 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 #pragma unroll
 					for (int c = 0; c < 32; c++) acc[c] = fma(b, ldc(srow + c), acc[c]);
 				}
+			};
+			int k = 0;
+			for (int tb = q0; tb < q1; tb += tstep, k ^= 1)
+			{
+				if (WALK == kWalkLds && tb + kTileOps < q1)
+					load_tile(gops, gskip, nops_s, tb + kTileOps, tile_ops + (k ^ 1) * kTileOps,
+						tile_skip + (k ^ 1) * kTileOps, wave, G, lane);
+				const int tend = min(tb + tstep, w.q1);
+				for (;;)
+				{
+					int node = 0;
+					double val = 0.0;
+					if (!w.run(tile_ops + k * kTileOps, tile_skip + k * kTileOps, tb, tend, lane,
+							xb0, xb1, xb2, xb3, node, val)) break;
+					if (w.chunk >= 0) contract();
+					w.start_chunk(node, val, lane);
+				}
+				if constexpr (WALK == kWalkLds) __syncthreads();
 			}
+			if (w.chunk >= 0) contract();
 			if (lane < np)
 			{
 				double* out = split ? part + ((int64_t)it.seg * it.count + blk * kWave + lane) * DP
@@ -550,16 +608,16 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 				}
 				double b[kChunk];
 #pragma unroll
-				for (int kk = 0; kk < kChunk; kk++) { b[kk] = Bl[kk * kWave + lane]; Bl[kk * kWave + lane] = 0.0f; }
+				for (int kk = 0; kk < kChunk; kk++) { b[kk] = Bl[kk * kWave + lane]; Bl[kk * kWave + lane] = 0.0; }
 				contract_chunk<PREC, DT>(so, ndt, b, acc);
 			};
 			int k = 0;
-			for (int tb = q0; tb < q1; tb += kTileOps, k ^= 1)
+			for (int tb = q0; tb < q1; tb += tstep, k ^= 1)
 			{
-				if (tb + kTileOps < q1)
+				if (WALK == kWalkLds && tb + kTileOps < q1)
 					load_tile(gops, gskip, nops_s, tb + kTileOps, tile_ops + (k ^ 1) * kTileOps,
 						tile_skip + (k ^ 1) * kTileOps, wave, G, lane);
-				const int tend = min(tb + kTileOps, w.q1);
+				const int tend = min(tb + tstep, w.q1);
 				for (;;)
 				{
 					int node = 0;
@@ -572,7 +630,7 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 					// consumed after the walk has finished the chunk
 					so.load(sd, w.chunk, DP, dt0, ndt, lane);
 				}
-				__syncthreads();
+				if constexpr (WALK == kWalkLds) __syncthreads();
 			}
 			if (w.chunk >= 0) contract();
 #pragma unroll
@@ -624,11 +682,11 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 	}
 }
 
-size_t sweep_lds_bytes(int prec, int waves, int depth)
+size_t sweep_lds_bytes(int prec, int waves, int depth, int walk)
 {
-	const size_t pm = depth < 8 ? 8 : 16;
 	(void)prec;
-	return kLdsHeader + kLdsTiles + (size_t)waves * (pm * kWave * 8 + kChunk * kWave * 8);
+	const size_t stage = depth < 8 ? wave_stage_bytes<8>() : wave_stage_bytes<16>();
+	return (walk == kWalkLds ? kLdsHeader + kLdsTiles : 0) + (size_t)waves * stage;
 }
 
 __global__ void reset_counter_kernel(PlanHeader* hdr)
@@ -659,34 +717,38 @@ __global__ void reduce_partials_kernel(const StateDesc* __restrict__ states, con
 // ---- launchers ----
 
 template <int PREC, int DT, int XB, int PM>
-static const void* sweep_ptr() { return (const void*)&sweep_kernel<PREC, DT, XB, PM>; }
+static const void* sweep_ptr(int walk)
+{
+	return walk == kWalkLds ? (const void*)&sweep_kernel<PREC, DT, XB, PM, kWalkLds>
+		: (const void*)&sweep_kernel<PREC, DT, XB, PM, kWalkSmem>;
+}
 
 template <int PREC, int DT, int XB>
-static const void* sweep_pm(int depth)
+static const void* sweep_pm(int depth, int walk)
 {
-	return depth < 8 ? sweep_ptr<PREC, DT, XB, 8>() : sweep_ptr<PREC, DT, XB, 16>();
+	return depth < 8 ? sweep_ptr<PREC, DT, XB, 8>(walk) : sweep_ptr<PREC, DT, XB, 16>(walk);
 }
 
 template <int PREC, int DT>
-static const void* sweep_xb(int xbanks, int depth)
+static const void* sweep_xb(int xbanks, int depth, int walk)
 {
 	switch (xbanks)
 	{
-	case 1: return sweep_pm<PREC, DT, 1>(depth);
-	case 2: return sweep_pm<PREC, DT, 2>(depth);
-	case 3: case 4: return sweep_pm<PREC, DT, 4>(depth);
+	case 1: return sweep_pm<PREC, DT, 1>(depth, walk);
+	case 2: return sweep_pm<PREC, DT, 2>(depth, walk);
+	case 3: case 4: return sweep_pm<PREC, DT, 4>(depth, walk);
 	default: return nullptr;
 	}
 }
 
-const void* sweep_kernel_ptr(int prec, int dt, int xbanks, int depth)
+const void* sweep_kernel_ptr(int prec, int dt, int xbanks, int depth, int walk)
 {
 	switch (prec)
 	{
-	case HDDM_LB_FP64: return sweep_xb<HDDM_LB_FP64, 1>(xbanks, depth);
-	case HDDM_LB_F16X2: return dt == 1 ? sweep_xb<HDDM_LB_F16X2, 1>(xbanks, depth) : sweep_xb<HDDM_LB_F16X2, 2>(xbanks, depth);
-	case HDDM_LB_BF16X3: return dt == 1 ? sweep_xb<HDDM_LB_BF16X3, 1>(xbanks, depth) : sweep_xb<HDDM_LB_BF16X3, 2>(xbanks, depth);
-	case HDDM_LB_F32: return dt == 1 ? sweep_xb<HDDM_LB_F32, 1>(xbanks, depth) : sweep_xb<HDDM_LB_F32, 2>(xbanks, depth);
+	case HDDM_LB_FP64: return sweep_xb<HDDM_LB_FP64, 1>(xbanks, depth, walk);
+	case HDDM_LB_F16X2: return dt == 1 ? sweep_xb<HDDM_LB_F16X2, 1>(xbanks, depth, walk) : sweep_xb<HDDM_LB_F16X2, 2>(xbanks, depth, walk);
+	case HDDM_LB_BF16X3: return dt == 1 ? sweep_xb<HDDM_LB_BF16X3, 1>(xbanks, depth, walk) : sweep_xb<HDDM_LB_BF16X3, 2>(xbanks, depth, walk);
+	case HDDM_LB_F32: return dt == 1 ? sweep_xb<HDDM_LB_F32, 1>(xbanks, depth, walk) : sweep_xb<HDDM_LB_F32, 2>(xbanks, depth, walk);
 	}
 	return nullptr;
 }

@@ -28,10 +28,16 @@ struct SweepArgs
 	unsigned long long* stats; // optional: ops evaluated, prunes, chunks contracted, items
 };
 
+// How a sweep reads the op stream: each wave through scalar loads, waves
+// independent (kWalkSmem), or the workgroup through shared LDS tiles, its
+// waves walking neighbouring point blocks in step (kWalkLds).
+constexpr int kWalkLds = 0;
+constexpr int kWalkSmem = 1;
+
 // Kernel symbol of the sweep instantiation for (precision, dof tiles of 32
-// per pass, register banks of 16 coordinates, max node depth); NULL when
-// unsupported. FP64 always runs one dof tile per pass.
-const void* sweep_kernel_ptr(int prec, int dt, int xbanks, int depth);
+// per pass, register banks of 16 coordinates, max node depth, walk); NULL
+// when unsupported. FP64 always runs one dof tile per pass.
+const void* sweep_kernel_ptr(int prec, int dt, int xbanks, int depth, int walk);
 // Register banks needed for dim (0: dim above 64 is unsupported).
 int xbanks_for(int dim);
 
@@ -43,9 +49,9 @@ hipError_t launch_scan_plan(const uint32_t* hist, uint32_t* cursor, int64_t nbuc
 hipError_t launch_scatter(int64_t n, const uint32_t* bkey, uint32_t* cursor, uint32_t* perm, hipStream_t st);
 hipError_t launch_reset_counter(PlanHeader* hdr, hipStream_t st);
 hipError_t launch_sweep(const void* kern, int grid, int waves, size_t lds, const SweepArgs& args, hipStream_t st);
-// Dynamic LDS of the sweep: the workgroup's op tiles, then per wave the path
-// stack and the chunk buffer.
-size_t sweep_lds_bytes(int prec, int waves, int depth);
+// Dynamic LDS of the sweep: the workgroup's op tiles (kWalkLds), then per
+// wave the path stack and the chunk buffer.
+size_t sweep_lds_bytes(int prec, int waves, int depth, int walk);
 hipError_t launch_reduce(const StateDesc* states, const StatePlan* plans, const uint32_t* perm,
 	const double* partial, double* y, int64_t ldy, int nstates, hipStream_t st);
 

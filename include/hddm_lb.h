@@ -62,6 +62,7 @@ typedef struct hddm_lb_grid hddm_lb_grid;
 #define HDDM_LB_OPT_SLAB_TILES    6  /* dof tiles of 32 per sweep pass (1, 2; 0: automatic) */
 #define HDDM_LB_OPT_FLUSH_LOG2    7  /* log2 of the FP32->FP64 flush window in 16-node chunks (-1: default 3) */
 #define HDDM_LB_OPT_STATS         8  /* 1: count walk ops, prunes, contracted chunks and items (hddm_lb_last_stats) */
+#define HDDM_LB_OPT_WALK          9  /* op stream source: 0 automatic, 1 scalar loads per wave, 2 LDS tiles per workgroup */
 
 /* ---- devices and contexts ---- */
 

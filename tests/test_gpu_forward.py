@@ -218,3 +218,28 @@ def test_forward_host_path(ctx, kat):
     c.forward_host(np.array([kat["x"]]), y, precision=hddm_lb.FP64)
     np.testing.assert_allclose(y[0], kat["expected"], rtol=0, atol=kat["epsilon"])
     c.close()
+
+
+@pytest.mark.parametrize("walk,waves", [(1, 1), (1, 4), (2, 1), (2, 2), (2, 4)])
+def test_walk_modes_and_group_sizes(ctx, walk, waves):
+    """Both op-stream sources (scalar loads per wave, LDS tiles per workgroup)
+    and every workgroup size give the oracle's results, with and without node
+    segments (several LDS tiles per segment, ragged last group)."""
+    ii, jj, lv, ix = hddm_lb.generate_grid(10, 2, 6, 0.4, 21, 9000)
+    S = hddm_lb.synthetic_surplus(lv, 40, 22)
+    ctx.upload_state(0, ii, jj, S)
+    X = hddm_lb.synthetic_points(1000, 10, 23)
+    yo = oracle.forward(ii, jj, S, X)
+    ctx.set_option(hddm_lb.OPT_WALK, walk)
+    ctx.set_option(hddm_lb.OPT_WAVES_PER_WG, waves)
+    try:
+        for seg in (0, 2500):
+            ctx.set_option(hddm_lb.OPT_SEGMENT_NODES, seg)
+            for prec in (hddm_lb.FP64, hddm_lb.F16X2):
+                say("walk", walk, "waves", waves, "segment", seg, "precision", prec)
+                y = run_forward(ctx, X, precision=prec)
+                assert normwise_rel_err(y, yo) <= TOL[prec]
+    finally:
+        ctx.set_option(hddm_lb.OPT_WALK, 0)
+        ctx.set_option(hddm_lb.OPT_WAVES_PER_WG, 0)
+        ctx.set_option(hddm_lb.OPT_SEGMENT_NODES, 0)
