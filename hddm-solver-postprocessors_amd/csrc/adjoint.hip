@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 			{
 				const int sl = q - tb;
 				const u4_t ov = *(const u4_t*)(tops + sl);
-				const uint32_t dim = __builtin_amdgcn_readfirstlane(ov[0]);
+				const uint32_t dim = __builtin_amdgcn_readfirstlane(ov[0]) & kOpDimMask;
 				const uint32_t ctrl = __builtin_amdgcn_readfirstlane(ov[1]);
 				const uint32_t ihi = __builtin_amdgcn_readfirstlane(ov[2]);
 				const uint32_t njhi = __builtin_amdgcn_readfirstlane(ov[3]);

@@ -217,8 +217,13 @@ extern "C" int hddm_lb_upload_state(hddm_lb_ctx* ctx, int istate, int dim, int n
 	d.dim = dim; d.nno = nno; d.dofs = dofs; d.dofs_pad = L.dofs_pad;
 	d.nops = (int)L.skip.size() - 1; d.nchunks = L.nchunks; d.max_depth = L.fmax;
 	d.nseg = 1;
-	d.ops = (const Op*)up(L.ops.data(), L.ops.size() * sizeof(Op));
-	d.skip = (const uint32_t*)up(L.skip.data(), L.skip.size() * sizeof(uint32_t));
+	// device copies carry kOpPad sentinels so that read-ahead stays in bounds
+	std::vector<Op> ops_pad(L.ops);
+	std::vector<uint32_t> skip_pad(L.skip);
+	ops_pad.resize((size_t)d.nops + kOpPad, L.ops[d.nops]);
+	skip_pad.resize((size_t)d.nops + kOpPad, (uint32_t)d.nops);
+	d.ops = (const Op*)up(ops_pad.data(), ops_pad.size() * sizeof(Op));
+	d.skip = (const uint32_t*)up(skip_pad.data(), skip_pad.size() * sizeof(uint32_t));
 	d.s64 = (const double*)up(L.s64.data(), L.s64.size() * sizeof(double));
 	d.s16 = (const uint16_t*)up(L.s16.data(), L.s16.size() * sizeof(uint16_t));
 	d.sbf = (const uint16_t*)up(L.sbf.data(), L.sbf.size() * sizeof(uint16_t));
@@ -400,7 +405,7 @@ TuneConfig default_config(hddm_lb_ctx* ctx, int prec)
 	int slab = ctx->opt_slab;
 	if (!slab || prec == HDDM_LB_FP64) slab = 1;
 	c.slab = slab;
-	c.wlog = ctx->opt_wlog >= 0 ? ctx->opt_wlog : 3;
+	c.wlog = ctx->opt_wlog >= 0 ? ctx->opt_wlog : 0;
 	return c;
 }
 

@@ -58,7 +58,7 @@ struct TuneConfig
 	int waves = 4;     // waves per workgroup
 	int walk = 1;      // kWalkSmem (1) or kWalkLds (0)
 	int slab = 1;      // dof tiles of 32 per sweep pass
-	int wlog = 3;      // log2 flush window (chunks)
+	int wlog = 0;      // log2 flush window (chunks)
 	int64_t seg = -1;  // node segment length used while tuning (-1: context option)
 };
 

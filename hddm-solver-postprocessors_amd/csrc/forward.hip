@@ -205,9 +205,10 @@ __device__ __forceinline__ void contract_chunk(const SOperands<PREC, DT>& so, in
 #pragma unroll
 		for (int m = 0; m < 8; m++)
 		{
-			const double d0 = b[2 * m] * (double)kBScale, d1 = b[2 * m + 1] * (double)kBScale;
-			hi[m] = pack_h2((float)d0, (float)d1);
-			lo[m] = pack_h2((float)(d0 - (double)lo_half_f16(hi[m])), (float)(d1 - (double)hi_half_f16(hi[m])));
+			// split in FP32 (one rounding from FP64; the f16 pair keeps 22 bits)
+			const float f0 = (float)b[2 * m] * kBScale, f1 = (float)b[2 * m + 1] * kBScale;
+			hi[m] = pack_h2(f0, f1);
+			lo[m] = pack_h2(f0 - lo_half_f16(hi[m]), f1 - hi_half_f16(hi[m]));
 		}
 		// Lanes 0-31 own points 0-31, lanes 32-63 points 32-63; each holds
 		// nodes 0-15 of its point. The B operand of points 0-31 needs nodes
@@ -243,11 +244,12 @@ __device__ __forceinline__ void contract_chunk(const SOperands<PREC, DT>& so, in
 #pragma unroll
 		for (int m = 0; m < 8; m++)
 		{
-			const double d0 = b[2 * m], d1 = b[2 * m + 1];
-			t1[m] = pack_b2((float)d0, (float)d1);
-			const double r0 = d0 - (double)lo_half_bf16(t1[m]), r1 = d1 - (double)hi_half_bf16(t1[m]);
-			t2[m] = pack_b2((float)r0, (float)r1);
-			t3[m] = pack_b2((float)(r0 - (double)lo_half_bf16(t2[m])), (float)(r1 - (double)hi_half_bf16(t2[m])));
+			// split in FP32: three bf16 terms carry the 24 bits of the FP32 value
+			const float f0 = (float)b[2 * m], f1 = (float)b[2 * m + 1];
+			t1[m] = pack_b2(f0, f1);
+			const float r0 = f0 - lo_half_bf16(t1[m]), r1 = f1 - hi_half_bf16(t1[m]);
+			t2[m] = pack_b2(r0, r1);
+			t3[m] = pack_b2(r0 - lo_half_bf16(t2[m]), r1 - hi_half_bf16(t2[m]));
 		}
 #pragma unroll
 		for (int m = 0; m < 4; m++)
@@ -317,7 +319,7 @@ __device__ __forceinline__ void contract_chunk(const SOperands<PREC, DT>& so, in
 //   kWalkSmem  scalar loads from the state's op stream, the sequential
 //              successor read one op ahead; waves are independent
 //   kWalkLds   the workgroup's LDS tile (wave-uniform vector reads)
-template <int XB, typename BT, int WALK>
+template <int XB, typename BT, int WALK, int PM>
 struct Walker
 {
 	double* P;          // [PM][64] in LDS
@@ -358,6 +360,10 @@ struct Walker
 		}
 	}
 
+	__device__ __forceinline__ double pget(int t, int lane) { return P[t * kWave + lane]; }
+
+	__device__ __forceinline__ void pset(int t, double v, int lane) { P[t * kWave + lane] = v; }
+
 	// One factor at depth t: the new prefix product.
 	__device__ __forceinline__ double step(int t, uint32_t dim, uint32_t ihi, uint32_t njhi, int lane,
 		const double (&xb0)[16], const double (&xb1)[16], const double (&xb2)[16], const double (&xb3)[16])
@@ -365,15 +371,62 @@ struct Walker
 		double bt;
 		if (t == last_t + 1) bt = cur;
 		else if (t == last_t) bt = base_prev;
-		else bt = P[t * kWave + lane];
+		else bt = pget(t, lane);
 		const double tt = fma(xget<XB>(xb0, xb1, xb2, xb3, dim), f64_from_high(ihi), f64_from_high(njhi));
 		const double p = bt * fmax(1.0 - fabs(tt), 0.0);
-		P[(t + 1) * kWave + lane] = p;
+		pset(t + 1, p, lane);
 		base_prev = bt;
 		cur = p;
 		last_t = t;
 		nops++;
 		return p;
+	}
+
+	// Evaluates the leaf run of `run` (2..kMaxLeafRun) sibling ops starting at
+	// q, whose nodes lie in the current chunk: the factors are independent
+	// given the parent product, so their FP64 chains overlap. Leaves have no
+	// subtree, so an all-zero leaf needs no skip; it only writes no basis.
+	__device__ __forceinline__ void run_leaves(int run, int lane,
+		const double (&xb0)[16], const double (&xb1)[16], const double (&xb2)[16], const double (&xb3)[16])
+	{
+		const int t = (int)(op.ctrl & 15u);
+		double bt;
+		if (t == last_t + 1) bt = cur;
+		else if (t == last_t) bt = base_prev;
+		else bt = P[t * kWave + lane];
+		Op o[kMaxLeafRun];
+		o[0] = op;
+#pragma unroll
+		for (int r = 1; r < kMaxLeafRun; r++) o[r] = ldc_op(gops + q + r);  // kOpPad keeps q + 3 in bounds
+		double p[kMaxLeafRun];
+#pragma unroll
+		for (int r = 0; r < kMaxLeafRun; r++)
+		{
+			const double tt = fma(xget<XB>(xb0, xb1, xb2, xb3, o[r].dim & kOpDimMask),
+				f64_from_high(o[r].ihi), f64_from_high(o[r].njhi));
+			p[r] = bt * fmax(1.0 - fabs(tt), 0.0);
+		}
+#pragma unroll
+		for (int r = 0; r < kMaxLeafRun; r++)
+			if (r < run)
+			{
+				if (wave_any(p[r] != 0.0))
+				{
+					const int n = (int)(o[r].ctrl >> kOpNodeShift);
+					B[(n & 15) * kWave + lane] = (BT)p[r];
+					mask |= 1u << (n & 15);
+				}
+				else nprunes++;
+			}
+		const double plast = run == 2 ? p[1] : (run == 3 ? p[2] : p[3]);
+		P[(t + 1) * kWave + lane] = plast;
+		base_prev = bt;
+		cur = plast;
+		last_t = t;
+		nops += run;
+		q += run;
+		op = ldc_op(gops + q);
+		sk = ldc(gskip + q);
 	}
 
 	// Walks ops below tend (the end of the resident tile or of the segment),
@@ -393,17 +446,24 @@ struct Walker
 			{
 				sl = q - tb;
 				const u4_t ov = *(const u4_t*)(tops + sl);
-				dim = __builtin_amdgcn_readfirstlane(ov[0]);
+				dim = __builtin_amdgcn_readfirstlane(ov[0]) & kOpDimMask;
 				ctrl = __builtin_amdgcn_readfirstlane(ov[1]);
 				ihi = __builtin_amdgcn_readfirstlane(ov[2]);
 				njhi = __builtin_amdgcn_readfirstlane(ov[3]);
 			}
 			else
 			{
-				// the stream and the skips end with a sentinel, so q + 1 is in bounds
+				// sibling leaves in one chunk: up to kMaxLeafRun factors on one parent product
+				const int run = min((int)(op.dim >> kOpRunShift), tend - q);
+				if (run >= 2 && (int)(op.ctrl >> (kOpNodeShift + 4)) == chunk)
+				{
+					run_leaves(run, lane, xb0, xb1, xb2, xb3);
+					continue;
+				}
+				// the stream and the skips end with kOpPad sentinels, so q + 1 is in bounds
 				nop = ldc_op(gops + q + 1);
 				nsk = ldc(gskip + q + 1);
-				dim = op.dim; ctrl = op.ctrl; ihi = op.ihi; njhi = op.njhi;
+				dim = op.dim & kOpDimMask; ctrl = op.ctrl; ihi = op.ihi; njhi = op.njhi;
 			}
 			const double p = step((int)(ctrl & 15u), dim, ihi, njhi, lane, xb0, xb1, xb2, xb3);
 			if (!wave_any(p != 0.0))
@@ -517,7 +577,7 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 #pragma unroll
 		for (int k = 0; k < kChunk; k++) Bl[k * kWave + lane] = (BT)0;
 
-		Walker<XB, BT, WALK> w;
+		Walker<XB, BT, WALK, PM> w;
 		w.init(q0, np > 0 ? q1 : q0, Pl, Bl, lane, gops, gskip);
 		uint32_t stat_chunks = 0;
 		const bool split = nseg > 1;

@@ -18,27 +18,36 @@ constexpr int kMortonBits = 40;
 // One factor evaluation of the prefix-trie walk ("op"), 16 bytes, read by
 // one scalar load. The ops of all nodes are stored in trie order; node n
 // contributes one op per factor it does not share with node n-1.
-//   dim    dimension of x the factor reads
+//   dim    dimension of x the factor reads (bits 0-7) | leaf run << 8
 //   ctrl   depth t (bits 0-3) | last factor of its node (bit 4) | node index << 5
 //   ihi    high word of the FP64 value of i (the low word is 0)
 //   njhi   high word of the FP64 value of -j (the low word is 0)
 // The walk computes P[t+1] = P[t] * max(0, 1 - |x_dim * i - j|). When P[t+1]
 // is zero on every lane, the run of nodes sharing the node's first t+1
 // factors is skipped: skip[op] is the first op after that run.
+// Leaf run: for the last op of a node, the number (1..kMaxLeafRun) of
+// consecutive ops from this one that are last ops at the same depth with
+// nodes in the same 16-node chunk: sibling leaves under one parent product,
+// which the walk may evaluate together; 0 for other ops.
+// The device copy of ops and skip carries kOpPad sentinels after the stream.
 struct Op
 {
 	uint32_t dim, ctrl, ihi, njhi;
 };
 constexpr uint32_t kOpLast = 16u;
 constexpr int kOpNodeShift = 5;
+constexpr uint32_t kOpDimMask = 0xffu;
+constexpr int kOpRunShift = 8;
+constexpr int kMaxLeafRun = 4;
+constexpr int kOpPad = 4;
 
 struct StateDesc
 {
 	int dim, nno, dofs, dofs_pad;   // dofs_pad: dofs rounded up to 32
 	int nops, nchunks;
 	int nseg, max_depth;            // node segments (work-item split), max factors per node
-	const Op* ops;                  // [nops + 1]
-	const uint32_t* skip;           // [nops]
+	const Op* ops;                  // [nops + kOpPad]
+	const uint32_t* skip;           // [nops + kOpPad]
 	const int32_t* seg_op;          // [nseg + 1] first op of each segment
 	const double* s64;              // [nno * dofs_pad] FP64 rows, trie order
 	const uint16_t* s16;            // [nchunks][2][dofs_pad][16] f16 hi, lo (scaled)

@@ -208,6 +208,24 @@ int build_layout(int dim, int nno, int dofs,
 	L.ops[nops] = Op{ 0u, 0u, 0u, 0u };
 	L.skip[nops] = (uint32_t)nops;
 
+	// Leaf runs: consecutive last ops at one depth (siblings under one parent
+	// product) whose nodes share a 16-node chunk, counted from each op.
+	for (int64_t q = nops - 1; q >= 0; q--)
+	{
+		const Op& o = L.ops[q];
+		if (!(o.ctrl & kOpLast)) continue;
+		uint32_t run = 1;
+		if (q + 1 < nops)
+		{
+			const Op& nx = L.ops[q + 1];
+			const uint32_t nrun = nx.dim >> kOpRunShift;
+			if ((nx.ctrl & kOpLast) && (nx.ctrl & 15u) == (o.ctrl & 15u) &&
+				((nx.ctrl >> kOpNodeShift) >> 4) == ((o.ctrl >> kOpNodeShift) >> 4) && nrun > 0)
+				run = std::min<uint32_t>(nrun + 1, kMaxLeafRun);
+		}
+		L.ops[q].dim |= run << kOpRunShift;
+	}
+
 	// Support masks: bit d of lomask (himask) when the node's support in dim d
 	// lies in [0, 1/2] ([1/2, 1]); dims >= 64 are left out (conservative).
 	L.lomask.assign(nno, 0);

@@ -60,7 +60,7 @@ typedef struct hddm_lb_grid hddm_lb_grid;
 #define HDDM_LB_OPT_SORT_POINTS   4  /* 1 (default): spatially sort points before the sweep */
 #define HDDM_LB_OPT_TIMING        5  /* 1: time the sweep kernels with hipEvents (hddm_lb_last_sweep_ms) */
 #define HDDM_LB_OPT_SLAB_TILES    6  /* dof tiles of 32 per sweep pass (1, 2; 0: automatic) */
-#define HDDM_LB_OPT_FLUSH_LOG2    7  /* log2 of the FP32->FP64 flush window in 16-node chunks (-1: default 3) */
+#define HDDM_LB_OPT_FLUSH_LOG2    7  /* log2 of the FP32->FP64 flush window in 16-node chunks (default 0: every chunk) */
 #define HDDM_LB_OPT_STATS         8  /* 1: count walk ops, prunes, contracted chunks and items (hddm_lb_last_stats) */
 #define HDDM_LB_OPT_WALK          9  /* op stream source: 0 automatic, 1 scalar loads per wave, 2 LDS tiles per workgroup */
 
@@ -161,7 +161,8 @@ int hddm_lb_layout_build(int dim, int nno, int dofs,
 	const double* surplus, int64_t ld_surplus, hddm_lb_layout** layout);
 int hddm_lb_layout_info(const hddm_lb_layout* layout, int* max_factors, int* nops,
 	int* nchunks, int* dofs_pad);
-/* ops: 4 * nops words; skip: nops; opstart: nno + 1 */
+/* ops: 4 * nops words per op {dim | leaf run << 8, depth | last << 4 | node << 5,
+ * high word of FP64 i, high word of FP64 -j}; skip: nops; opstart: nno + 1 */
 int hddm_lb_layout_ops(const hddm_lb_layout* layout, uint32_t* ops, uint32_t* skip, int32_t* opstart);
 /* order[n]: uploaded index of the node at trie position n */
 int hddm_lb_layout_order(const hddm_lb_layout* layout, int32_t* order);

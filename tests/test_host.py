@@ -193,6 +193,13 @@ def test_layout_trie_walk_matches_oracle(dim, start, level, target, seed):
             while e < nno and seqs[e][:t + 1] == seqs[n][:t + 1]:
                 e += 1
             assert L.skip[q] == L.opstart[e]
+    # leaf runs: each op of a run is the last op of its node, at the run's depth, in its chunk
+    for q in range(L.nops):
+        r = L.op_run[q]
+        assert (r > 0) == bool(L.op_last[q]) and r <= 4
+        for k in range(1, r):
+            assert L.op_last[q + k] and L.op_depth[q + k] == L.op_depth[q]
+            assert L.op_node[q + k] >> 4 == L.op_node[q] >> 4 and L.op_node[q + k] == L.op_node[q] + k
     # walks with pruning reproduce the oracle on a clustered and a random block
     rng = np.random.default_rng(seed)
     X = rng.random((4096, dim))
