@@ -6,11 +6,12 @@ p=0.3 from level sum <= 2, max level 7, stopped at 60,018 nodes), dofs=32,
 1,048,576 points x ~ U[0,1)^16, all resident in HBM before timing.
 
 A step is one forward over the whole batch. The headline path is the
-mixed-precision MFMA sweep (f16 hi/lo split operands, FP32 accumulation
-folded into FP64); the plain FP64 GPU sweep (same walk, FP64 contraction) is
+mixed-precision MFMA sweep (basis formed in FP64, contraction on
+v_mfma_f32_32x32x2f32 with FP32 accumulation folded into FP64 after every
+16-node chunk); the plain FP64 GPU sweep (same walk, FP64 contraction) is
 timed beside it. Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--precision f16x2]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--precision f32]
 """
 import argparse
 import json
@@ -23,6 +24,8 @@ sys.path.insert(0, os.path.join(ROOT, "hddm-solver-postprocessors_amd"))
 
 METRIC = "point·dof evals/sec on 1 MI355X, mixed/MFMA vs FP64 plain at ≤1e-6 rel err"
 PEAK_TFLOPS = {"f16x2": 2500.0, "bf16x3": 2500.0, "f32": 157.3, "fp64": 78.6}
+PRODUCTS = {"f16x2": 3, "bf16x3": 6, "f32": 1}   # MFMA products per chunk and dof tile
+kChunkNodes = 16
 DTYPE = {"f16x2": "f16", "bf16x3": "bf16", "f32": "f32", "fp64": "f64"}
 
 
@@ -96,7 +99,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--precision", default="f16x2", choices=["f16x2", "bf16x3", "f32"])
+    ap.add_argument("--precision", default="f32", choices=["f16x2", "bf16x3", "f32"])
     ap.add_argument("--points", type=int, default=1 << 20)
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and error check")
@@ -135,6 +138,12 @@ def main():
         res[name] = (ms, sweep_ms, mn, mx)
         ygpu[name] = y.cpu().numpy()
     ms, sweep_ms, _, _ = res[args.precision]
+    # executed work of one sweep (outside the timed steps): trie ops evaluated
+    # per 64-point block and 16-node chunks contracted
+    ctx.set_option(hddm_lb.OPT_STATS, 1)
+    ctx.forward(x, y, None, precs[args.precision])
+    nops_ex, nprune, nchunk, nitem = ctx.last_stats()
+    ctx.set_option(hddm_lb.OPT_STATS, 0)
 
     # algorithmic work of one launch: dense contraction 2*nno*dofs plus
     # 4 flops per active factor, per point
@@ -171,6 +180,11 @@ def main():
                        "kernel_ms": res["fp64"][1],
                        "roofline_frac": flops / (res["fp64"][1] * 1e-3) / 1e12 / PEAK_TFLOPS["fp64"]},
         "mixed_over_fp64": res["fp64"][0] / ms,
+        "executed": {"walk_ops_per_block": nops_ex / max(nitem, 1),
+                     "pruned_runs_per_block": nprune / max(nitem, 1),
+                     "chunks_per_block": nchunk / max(nitem, 1), "blocks": nitem,
+                     "mfma_tflops": nchunk * 2.0 * kChunkNodes * 64 * 32 * PRODUCTS[args.precision]
+                     / (sweep_ms * 1e-3) / 1e12},
     }
     if not args.no_cpu:
         cb, errs, port_err = cpu_baseline(ii, jj, S, X, ygpu, args.cpu_seconds)

@@ -382,53 +382,6 @@ struct Walker
 		return p;
 	}
 
-	// Evaluates the leaf run of `run` (2..kMaxLeafRun) sibling ops starting at
-	// q, whose nodes lie in the current chunk: the factors are independent
-	// given the parent product, so their FP64 chains overlap. Leaves have no
-	// subtree, so an all-zero leaf needs no skip; it only writes no basis.
-	__device__ __forceinline__ void run_leaves(int run, int lane,
-		const double (&xb0)[16], const double (&xb1)[16], const double (&xb2)[16], const double (&xb3)[16])
-	{
-		const int t = (int)(op.ctrl & 15u);
-		double bt;
-		if (t == last_t + 1) bt = cur;
-		else if (t == last_t) bt = base_prev;
-		else bt = P[t * kWave + lane];
-		Op o[kMaxLeafRun];
-		o[0] = op;
-#pragma unroll
-		for (int r = 1; r < kMaxLeafRun; r++) o[r] = ldc_op(gops + q + r);  // kOpPad keeps q + 3 in bounds
-		double p[kMaxLeafRun];
-#pragma unroll
-		for (int r = 0; r < kMaxLeafRun; r++)
-		{
-			const double tt = fma(xget<XB>(xb0, xb1, xb2, xb3, o[r].dim & kOpDimMask),
-				f64_from_high(o[r].ihi), f64_from_high(o[r].njhi));
-			p[r] = bt * fmax(1.0 - fabs(tt), 0.0);
-		}
-#pragma unroll
-		for (int r = 0; r < kMaxLeafRun; r++)
-			if (r < run)
-			{
-				if (wave_any(p[r] != 0.0))
-				{
-					const int n = (int)(o[r].ctrl >> kOpNodeShift);
-					B[(n & 15) * kWave + lane] = (BT)p[r];
-					mask |= 1u << (n & 15);
-				}
-				else nprunes++;
-			}
-		const double plast = run == 2 ? p[1] : (run == 3 ? p[2] : p[3]);
-		P[(t + 1) * kWave + lane] = plast;
-		base_prev = bt;
-		cur = plast;
-		last_t = t;
-		nops += run;
-		q += run;
-		op = ldc_op(gops + q);
-		sk = ldc(gskip + q);
-	}
-
 	// Walks ops below tend (the end of the resident tile or of the segment),
 	// writing basis values of nodes of `chunk` into B. Stops at the first
 	// surviving node of another chunk, returned in (node, val) and not yet
@@ -453,13 +406,6 @@ struct Walker
 			}
 			else
 			{
-				// sibling leaves in one chunk: up to kMaxLeafRun factors on one parent product
-				const int run = min((int)(op.dim >> kOpRunShift), tend - q);
-				if (run >= 2 && (int)(op.ctrl >> (kOpNodeShift + 4)) == chunk)
-				{
-					run_leaves(run, lane, xb0, xb1, xb2, xb3);
-					continue;
-				}
 				// the stream and the skips end with kOpPad sentinels, so q + 1 is in bounds
 				nop = ldc_op(gops + q + 1);
 				nsk = ldc(gskip + q + 1);
