@@ -333,8 +333,7 @@ struct Walker
 	uint32_t nops, nprunes;  // counts for measurement
 	const Op* gops;     // kWalkSmem: the op stream and skips
 	const uint32_t* gskip;
-	Op op;              // kWalkSmem: op q and its skip, in SGPRs
-	uint32_t sk;
+	Op op;              // kWalkSmem: op q, in SGPRs
 
 	__device__ __forceinline__ void init(int q0, int qend, double* Pl, BT* Bl, int lane,
 		const Op* ops, const uint32_t* skip)
@@ -353,11 +352,7 @@ struct Walker
 		nprunes = 0;
 		gops = ops;
 		gskip = skip;
-		if constexpr (WALK == kWalkSmem)
-		{
-			op = ldc_op(ops + q);
-			sk = ldc(skip + q);
-		}
+		if constexpr (WALK == kWalkSmem) op = ldc_op(ops + q);
 	}
 
 	__device__ __forceinline__ double pget(int t, int lane) { return P[t * kWave + lane]; }
@@ -390,56 +385,76 @@ struct Walker
 		const double (&xb0)[16], const double (&xb1)[16], const double (&xb2)[16], const double (&xb3)[16],
 		int& node, double& val)
 	{
-		while (q < tend)
+		if constexpr (WALK == kWalkSmem)
 		{
-			uint32_t dim, ctrl, ihi, njhi, nsk = 0;
-			Op nop;
-			int sl = 0;
-			if constexpr (WALK == kWalkLds)
+			// Scalar-load walk on the precomputed relation and children bits:
+			// no depth compares, the stack is read only on a move up the trie
+			// and written only by ops with children.
+			while (q < tend)
 			{
-				sl = q - tb;
-				const u4_t ov = *(const u4_t*)(tops + sl);
-				dim = __builtin_amdgcn_readfirstlane(ov[0]) & kOpDimMask;
-				ctrl = __builtin_amdgcn_readfirstlane(ov[1]);
-				ihi = __builtin_amdgcn_readfirstlane(ov[2]);
-				njhi = __builtin_amdgcn_readfirstlane(ov[3]);
-			}
-			else
-			{
-				// the stream and the skips end with kOpPad sentinels, so q + 1 is in bounds
-				nop = ldc_op(gops + q + 1);
-				nsk = ldc(gskip + q + 1);
-				dim = op.dim & kOpDimMask; ctrl = op.ctrl; ihi = op.ihi; njhi = op.njhi;
-			}
-			const double p = step((int)(ctrl & 15u), dim, ihi, njhi, lane, xb0, xb1, xb2, xb3);
-			if (!wave_any(p != 0.0))
-			{
-				nprunes++;
-				if constexpr (WALK == kWalkLds)
-					q = (int)__builtin_amdgcn_readfirstlane(tskip[sl]);
-				else
+				const uint32_t dimw = op.dim, ctrl = op.ctrl;
+				// the stream ends with kOpPad sentinels, so q + 1 is in bounds
+				const Op nop = ldc_op(gops + q + 1);
+				const int t = (int)(ctrl & 15u);
+				const uint32_t rel = (ctrl >> kOpRelShift) & 3u;
+				double bt;
+				if (rel == 0u) bt = cur;
+				else if (rel == 1u) bt = base_prev;
+				else bt = P[t * kWave + lane];
+				const double tt = fma(xget<XB>(xb0, xb1, xb2, xb3, dimw & kOpDimMask),
+					f64_from_high(op.ihi), f64_from_high(op.njhi));
+				const double p = bt * fmax(1.0 - fabs(tt), 0.0);
+				base_prev = bt;
+				cur = p;
+				if (ctrl & kOpKids) P[(t + 1) * kWave + lane] = p;
+				nops++;
+				if (!wave_any(p != 0.0))
 				{
-					q = (int)sk;
+					nprunes++;
+					q += (int)(dimw >> kOpSkipShift);
 					op = ldc_op(gops + q);
-					sk = ldc(gskip + q);
+					continue;
 				}
-				continue;
-			}
-			q++;
-			if constexpr (WALK == kWalkSmem)
-			{
+				q++;
 				op = nop;
-				sk = nsk;
+				if (ctrl & kOpLast)
+				{
+					const int n = (int)(ctrl >> kOpNodeShift);
+					if ((n >> 4) != chunk) { node = n; val = p; return true; }
+					B[(n & 15) * kWave + lane] = (BT)p;
+					mask |= 1u << (n & 15);
+				}
 			}
-			if (ctrl & kOpLast)
-			{
-				const int n = (int)(ctrl >> kOpNodeShift);
-				if ((n >> 4) != chunk) { node = n; val = p; return true; }
-				B[(n & 15) * kWave + lane] = (BT)p;
-				mask |= 1u << (n & 15);
-			}
+			return false;
 		}
-		return false;
+		else
+		{
+			while (q < tend)
+			{
+				const int sl = q - tb;
+				const u4_t ov = *(const u4_t*)(tops + sl);
+				const uint32_t dim = __builtin_amdgcn_readfirstlane(ov[0]) & kOpDimMask;
+				const uint32_t ctrl = __builtin_amdgcn_readfirstlane(ov[1]);
+				const uint32_t ihi = __builtin_amdgcn_readfirstlane(ov[2]);
+				const uint32_t njhi = __builtin_amdgcn_readfirstlane(ov[3]);
+				const double p = step((int)(ctrl & 15u), dim, ihi, njhi, lane, xb0, xb1, xb2, xb3);
+				if (!wave_any(p != 0.0))
+				{
+					nprunes++;
+					q = (int)__builtin_amdgcn_readfirstlane(tskip[sl]);
+					continue;
+				}
+				q++;
+				if (ctrl & kOpLast)
+				{
+					const int n = (int)(ctrl >> kOpNodeShift);
+					if ((n >> 4) != chunk) { node = n; val = p; return true; }
+					B[(n & 15) * kWave + lane] = (BT)p;
+					mask |= 1u << (n & 15);
+				}
+			}
+			return false;
+		}
 	}
 
 	__device__ __forceinline__ void start_chunk(int node, double val, int lane)

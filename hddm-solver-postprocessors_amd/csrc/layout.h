@@ -18,27 +18,33 @@ constexpr int kMortonBits = 40;
 // One factor evaluation of the prefix-trie walk ("op"), 16 bytes, read by
 // one scalar load. The ops of all nodes are stored in trie order; node n
 // contributes one op per factor it does not share with node n-1.
-//   dim    dimension of x the factor reads (bits 0-7) | leaf run << 8
-//   ctrl   depth t (bits 0-3) | last factor of its node (bit 4) | node index << 5
+//   dim    dimension of x the factor reads (bits 0-7) | (skip[op] - op) << 8
+//   ctrl   depth t (bits 0-3) | last factor of its node (bit 4)
+//          | relation to the previous op in the stream (bits 5-6)
+//          | the op has children (bit 7) | node index << 8
 //   ihi    high word of the FP64 value of i (the low word is 0)
 //   njhi   high word of the FP64 value of -j (the low word is 0)
 // The walk computes P[t+1] = P[t] * max(0, 1 - |x_dim * i - j|). When P[t+1]
 // is zero on every lane, the run of nodes sharing the node's first t+1
 // factors is skipped: skip[op] is the first op after that run.
-// Leaf run: for the last op of a node, the number (1..kMaxLeafRun) of
-// consecutive ops from this one that are last ops at the same depth with
-// nodes in the same 16-node chunk: sibling leaves under one parent product,
-// which the walk may evaluate together; 0 for other ops.
-// The device copy of ops and skip carries kOpPad sentinels after the stream.
+// Relation (precomputed so the walk needs no depth compares): 0 the op is
+// the first child of the previous op (its P[t] is the previous product),
+// 1 a sibling (same P[t] as the previous op), 2 otherwise (P[t] is read from
+// the path stack). After a skip of more than one op the target's stream
+// predecessor is deeper, so its relation is 2 and the stack holds P[t].
+// Children: P[t+1] is needed later only when the next op descends from this
+// one, so only such ops store it on the stack.
+// The device copies of ops and skip carry kOpPad sentinels after the stream.
 struct Op
 {
 	uint32_t dim, ctrl, ihi, njhi;
 };
 constexpr uint32_t kOpLast = 16u;
-constexpr int kOpNodeShift = 5;
+constexpr int kOpRelShift = 5;
+constexpr uint32_t kOpKids = 128u;
+constexpr int kOpNodeShift = 8;
 constexpr uint32_t kOpDimMask = 0xffu;
-constexpr int kOpRunShift = 8;
-constexpr int kMaxLeafRun = 4;
+constexpr int kOpSkipShift = 8;
 constexpr int kOpPad = 4;
 
 struct StateDesc

@@ -179,7 +179,7 @@ int build_layout(int dim, int nno, int dofs,
 		nops += std::max(1, F[n] - S[n]);
 	}
 	L.opstart[nno] = (int32_t)nops;
-	if (nops >= ((int64_t)1 << 31))
+	if (nops >= ((int64_t)1 << (32 - kOpSkipShift)))
 		return set_error(HDDM_LB_ERROR_UNSUPPORTED, "upload_state: too many trie edges (%lld)", (long long)nops);
 	L.ops.assign(nops + 1, Op{ 0, 0, 0, 0 });
 	L.skip.assign(nops + 1, 0);
@@ -208,22 +208,22 @@ int build_layout(int dim, int nno, int dofs,
 	L.ops[nops] = Op{ 0u, 0u, 0u, 0u };
 	L.skip[nops] = (uint32_t)nops;
 
-	// Leaf runs: consecutive last ops at one depth (siblings under one parent
-	// product) whose nodes share a 16-node chunk, counted from each op.
-	for (int64_t q = nops - 1; q >= 0; q--)
+	// Relative skip, relation to the stream predecessor and children flag.
+	for (int64_t q = 0; q < nops; q++)
 	{
-		const Op& o = L.ops[q];
-		if (!(o.ctrl & kOpLast)) continue;
-		uint32_t run = 1;
-		if (q + 1 < nops)
+		Op& o = L.ops[q];
+		const uint32_t t = o.ctrl & 15u;
+		uint32_t rel = 2;
+		if (q > 0)
 		{
-			const Op& nx = L.ops[q + 1];
-			const uint32_t nrun = nx.dim >> kOpRunShift;
-			if ((nx.ctrl & kOpLast) && (nx.ctrl & 15u) == (o.ctrl & 15u) &&
-				((nx.ctrl >> kOpNodeShift) >> 4) == ((o.ctrl >> kOpNodeShift) >> 4) && nrun > 0)
-				run = std::min<uint32_t>(nrun + 1, kMaxLeafRun);
+			const uint32_t pt = L.ops[q - 1].ctrl & 15u;
+			if (t == pt + 1) rel = 0;
+			else if (t == pt) rel = 1;
 		}
-		L.ops[q].dim |= run << kOpRunShift;
+		const bool kids = q + 1 < nops && (L.ops[q + 1].ctrl & 15u) == t + 1;
+		o.ctrl |= rel << kOpRelShift;
+		if (kids) o.ctrl |= kOpKids;
+		o.dim |= (L.skip[q] - (uint32_t)q) << kOpSkipShift;
 	}
 
 	// Support masks: bit d of lomask (himask) when the node's support in dim d

@@ -231,11 +231,13 @@ class Layout:
         self.opstart = np.zeros(nno + 1, np.int32)
         _check(lib.hddm_lb_layout_ops(h, _ptr(ops), _ptr(self.skip), _ptr(self.opstart)))
         self.op_dim = (ops[:, 0] & 0xff).astype(np.int64)
-        self.op_run = (ops[:, 0] >> 8).astype(np.int64)
+        self.op_relskip = (ops[:, 0] >> 8).astype(np.int64)
         ctrl = ops[:, 1]
         self.op_depth = (ctrl & 15).astype(np.int64)
         self.op_last = (ctrl >> 4) & 1 == 1
-        self.op_node = (ctrl >> 5).astype(np.int64)
+        self.op_rel = ((ctrl >> 5) & 3).astype(np.int64)
+        self.op_kids = (ctrl >> 7) & 1 == 1
+        self.op_node = (ctrl >> 8).astype(np.int64)
         self.op_i = (ops[:, 2].astype(np.uint64) << np.uint64(32)).view(np.float64)
         self.op_negj = (ops[:, 3].astype(np.uint64) << np.uint64(32)).view(np.float64)
         self.order = np.zeros(nno, np.int32)

@@ -138,3 +138,40 @@ def test_autograd_harness(torch_mod):
     assert normwise_rel_err(x.grad.cpu().numpy(), dxo) <= 1e-10
     assert normwise_rel_err(s.grad.cpu().numpy(), dso) <= 1e-12
     ctx.close()
+
+
+def test_adjoint_d20_100k_nodes(torch_mod):
+    """Differentiated workload shape: D=20 adaptive grid of ~100k nodes (max
+    level 6), dofs=32, 65,536 points in [0.01, 0.99]^20 kept 1e-3 off every
+    kink, gbar ~ N(0, 1). dx is checked against the oracle on a seeded subset
+    of points and against central differences on a few of them; dS against
+    the oracle's B^T g on a 2,048-point slice."""
+    torch = torch_mod
+    ii, jj, lv, _ = hddm_lb.generate_grid(20, 1, 6, 0.3, 404, 100000)
+    S = hddm_lb.synthetic_surplus(lv, 32, 405)
+    n = 65536
+    X = hddm_lb.synthetic_points(n, 20, 406, 0.01, 0.99, kink_level=6, kink_gap=1e-3)
+    G = np.random.default_rng(407).standard_normal((n, 32))
+    ctx = hddm_lb.Context(0)
+    ctx.upload_state(0, ii, jj, S)
+    say("adjoint D=20 nno", ii.shape[0])
+    dx, ds = run_adjoint(torch, ctx, X, G)
+    assert np.isfinite(dx).all() and np.isfinite(ds).all()
+    sub = np.random.default_rng(408).choice(n, size=96, replace=False)
+    dxo, _ = oracle.adjoint(ii, jj, S, X[sub], G[sub])
+    say("  dx err (subset)", normwise_rel_err(dx[sub], dxo))
+    assert normwise_rel_err(dx[sub], dxo) <= 1e-10
+    # dS over a 2,048-point slice of the batch (the oracle's cost grows with the points)
+    _, ds2 = run_adjoint(torch, ctx, X[:2048], G[:2048])
+    _, dso = oracle.adjoint(ii, jj, S, X[:2048], G[:2048])
+    say("  ds err (2048 points)", normwise_rel_err(ds2.reshape(ii.shape[0], 32), dso))
+    assert normwise_rel_err(ds2.reshape(ii.shape[0], 32), dso) <= 1e-12
+    h = 1e-4
+    for p in sub[:4]:
+        for d in (0, 7, 19):
+            xp, xm = X[p:p + 1].copy(), X[p:p + 1].copy()
+            xp[0, d] += h
+            xm[0, d] -= h
+            fd = ((oracle.forward(ii, jj, S, xp) - oracle.forward(ii, jj, S, xm)) * G[p]).sum() / (2 * h)
+            assert abs(dx[p, d] - fd) <= 1e-7 * max(1.0, abs(fd))
+    ctx.close()

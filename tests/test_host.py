@@ -163,6 +163,38 @@ def walk_layout(L, X, S_sorted):
     return y, evals
 
 
+def walk_layout_static(L, X, S_sorted, q0=0, q1=None):
+    """CPU emulation of the scalar-load walk, which selects the parent product
+    by the op's precomputed relation (0 previous product, 1 previous parent,
+    2 path stack), stores P[t+1] only for ops with children and jumps by the
+    relative skip."""
+    n = X.shape[0]
+    q1 = L.nops if q1 is None else q1
+    P = [np.full(n, np.nan) for _ in range(17)]
+    P[0] = np.ones(n)
+    cur = np.ones(n)
+    base_prev = np.ones(n)
+    y = np.zeros((n, L.dofs))
+    q, evals = q0, 0
+    while q < q1:
+        t, rel = L.op_depth[q], L.op_rel[q]
+        bt = cur if rel == 0 else (base_prev if rel == 1 else P[t])
+        assert not np.isnan(bt).any(), "stack read of a slot never written"
+        tt = X[:, L.op_dim[q]] * L.op_i[q] + L.op_negj[q]
+        p = bt * np.maximum(1.0 - np.abs(tt), 0.0)
+        base_prev, cur = bt, p
+        if L.op_kids[q]:
+            P[t + 1] = p
+        evals += 1
+        if not p.any():
+            q += int(L.op_relskip[q])
+            continue
+        if L.op_last[q]:
+            y += p[:, None] * S_sorted[L.op_node[q]][None, :]
+        q += 1
+    return y, evals
+
+
 @pytest.mark.parametrize("dim,start,level,target,seed", [(4, 4, 5, 0, 1), (8, 2, 5, 2000, 2), (16, 2, 7, 6000, 3)])
 def test_layout_trie_walk_matches_oracle(dim, start, level, target, seed):
     ii, jj, lv, ix = hddm_lb.generate_grid(dim, start, level, 0.3, seed, target)
@@ -193,13 +225,16 @@ def test_layout_trie_walk_matches_oracle(dim, start, level, target, seed):
             while e < nno and seqs[e][:t + 1] == seqs[n][:t + 1]:
                 e += 1
             assert L.skip[q] == L.opstart[e]
-    # leaf runs: each op of a run is the last op of its node, at the run's depth, in its chunk
+    # relative skips, relations to the stream predecessor and children flags
     for q in range(L.nops):
-        r = L.op_run[q]
-        assert (r > 0) == bool(L.op_last[q]) and r <= 4
-        for k in range(1, r):
-            assert L.op_last[q + k] and L.op_depth[q + k] == L.op_depth[q]
-            assert L.op_node[q + k] >> 4 == L.op_node[q] >> 4 and L.op_node[q + k] == L.op_node[q] + k
+        assert L.op_relskip[q] == L.skip[q] - q
+        t = L.op_depth[q]
+        if q == 0:
+            assert L.op_rel[q] == 2
+        else:
+            pt = L.op_depth[q - 1]
+            assert L.op_rel[q] == (0 if t == pt + 1 else (1 if t == pt else 2))
+        assert L.op_kids[q] == (q + 1 < L.nops and L.op_depth[q + 1] == t + 1)
     # walks with pruning reproduce the oracle on a clustered and a random block
     rng = np.random.default_rng(seed)
     X = rng.random((4096, dim))
@@ -211,6 +246,9 @@ def test_layout_trie_walk_matches_oracle(dim, start, level, target, seed):
         yw, evals = walk_layout(L, blk, S[L.order])
         yo = oracle.forward(ii, jj, S, blk)
         np.testing.assert_allclose(yw, yo, rtol=1e-12, atol=1e-12 * np.abs(yo).max())
+        yl, evals_l = walk_layout_static(L, blk, S[L.order])
+        assert evals_l == evals
+        np.testing.assert_allclose(yl, yo, rtol=1e-12, atol=1e-12 * np.abs(yo).max())
 
 
 def walk_adjoint(L, X, G, S_sorted):
