@@ -58,6 +58,53 @@ def time_path(ctx, x, y, prec, steps, warmup):
     return (t1 - t0) * 1e3 / steps, tot / max(n, 1), mn, mx
 
 
+def time_adjoint(args, prec):
+    """Forward + adjoint on the differentiated workload: D=20 adaptive grid of
+    ~100k nodes (max level 6), dofs=32, 65,536 points in [0.01, 0.99]^20,
+    upstream gradient N(0, 1). A step is one forward (mixed path) and one FP64
+    adjoint (dx and dsurplus) over the batch."""
+    import numpy as np
+    import torch
+    import hddm_lb
+    ii, jj, lv, _ = hddm_lb.generate_grid(20, 1, 6, 0.3, 404, 100000)
+    S = hddm_lb.synthetic_surplus(lv, 32, 405)
+    n = 65536
+    X = hddm_lb.synthetic_points(n, 20, 406, 0.01, 0.99, kink_level=6, kink_gap=1e-3)
+    G = np.random.default_rng(407).standard_normal((n, 32))
+    ctx = hddm_lb.Context(0)
+    ctx.upload_state(0, ii, jj, S)
+    x = torch.from_numpy(X).cuda()
+    g = torch.from_numpy(G).cuda()
+    y = torch.empty((n, 32), dtype=torch.float64, device="cuda")
+    dx = torch.empty((n, 20), dtype=torch.float64, device="cuda")
+    _, total = ctx.surplus_offset(-1)
+    ds = torch.empty((total,), dtype=torch.float64, device="cuda")
+
+    def step():
+        ctx.forward(x, y, None, prec)
+        ctx.adjoint(x, g, dx, ds, None, hddm_lb.FP64)
+
+    for _ in range(max(1, args.warmup // 2)):
+        step()
+    torch.cuda.synchronize()
+    steps = max(1, args.steps // 2)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.forward(x, y, None, prec)
+    torch.cuda.synchronize()
+    fwd_ms = (time.perf_counter() - t0) * 1e3 / steps
+    ctx.close()
+    return {"workload": "forward + adjoint, adaptive grid D=20, %d nodes, dofs=32, %d points" % (ii.shape[0], n),
+            "ms_per_step": ms, "forward_ms": fwd_ms, "adjoint_ms": ms - fwd_ms,
+            "point_dof_per_s": n * 32 / (ms * 1e-3),
+            "adjoint_precision": "fp64", "forward_precision": args.precision}
+
+
 def normwise_err(y, yref):
     import numpy as np
     den = np.abs(yref).max()
@@ -103,6 +150,8 @@ def main():
     ap.add_argument("--points", type=int, default=1 << 20)
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and error check")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the forward+adjoint measurement on the differentiated workload")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device 3 times (calibrates profiler byte counters)")
     args = ap.parse_args()
@@ -186,6 +235,8 @@ def main():
                      "mfma_tflops": nchunk * 2.0 * kChunkNodes * 64 * 32 * PRODUCTS[args.precision]
                      / (sweep_ms * 1e-3) / 1e12},
     }
+    if not args.no_adjoint:
+        out["forward_adjoint"] = time_adjoint(args, precs[args.precision])
     if not args.no_cpu:
         cb, errs, port_err = cpu_baseline(ii, jj, S, X, ygpu, args.cpu_seconds)
         out["cpu_baseline"] = cb
