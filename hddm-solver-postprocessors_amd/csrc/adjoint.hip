@@ -225,7 +225,7 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 				else if (t == last_t) bt = base_prev;
 				else bt = P[t * kWave + lane];
 				const double iv = f64_from_high(ihi);
-				const double u = fma(xget<XB>(xb0, xb1, xb2, xb3, dim), iv, f64_from_high(njhi));
+				const double u = fma(xget<XB, true>(xb0, xb1, xb2, xb3, dim), iv, f64_from_high(njhi));
 				const double phi = 1.0 - fabs(u);
 				const double p = bt * fmax(phi, 0.0);
 				P[(t + 1) * kWave + lane] = p;

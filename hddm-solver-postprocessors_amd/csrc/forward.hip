@@ -319,7 +319,7 @@ __device__ __forceinline__ void contract_chunk(const SOperands<PREC, DT>& so, in
 //   kWalkSmem  scalar loads from the state's op stream, the sequential
 //              successor read one op ahead; waves are independent
 //   kWalkLds   the workgroup's LDS tile (wave-uniform vector reads)
-template <int XB, typename BT, int WALK, int PM>
+template <int XB, typename BT, int WALK, int PM, bool XSEL>
 struct Walker
 {
 	double* P;          // [PM][64] in LDS
@@ -367,7 +367,7 @@ struct Walker
 		if (t == last_t + 1) bt = cur;
 		else if (t == last_t) bt = base_prev;
 		else bt = pget(t, lane);
-		const double tt = fma(xget<XB>(xb0, xb1, xb2, xb3, dim), f64_from_high(ihi), f64_from_high(njhi));
+		const double tt = fma(xget<XB, XSEL>(xb0, xb1, xb2, xb3, dim), f64_from_high(ihi), f64_from_high(njhi));
 		const double p = bt * fmax(1.0 - fabs(tt), 0.0);
 		pset(t + 1, p, lane);
 		base_prev = bt;
@@ -401,7 +401,7 @@ struct Walker
 				if (rel == 0u) bt = cur;
 				else if (rel == 1u) bt = base_prev;
 				else bt = P[t * kWave + lane];
-				const double tt = fma(xget<XB>(xb0, xb1, xb2, xb3, dimw & kOpDimMask),
+				const double tt = fma(xget<XB, XSEL>(xb0, xb1, xb2, xb3, dimw & kOpDimMask),
 					f64_from_high(op.ihi), f64_from_high(op.njhi));
 				const double p = bt * fmax(1.0 - fabs(tt), 0.0);
 				base_prev = bt;
@@ -538,7 +538,7 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 #pragma unroll
 		for (int k = 0; k < kChunk; k++) Bl[k * kWave + lane] = (BT)0;
 
-		Walker<XB, BT, WALK, PM> w;
+		Walker<XB, BT, WALK, PM, PREC == HDDM_LB_FP64> w;
 		w.init(q0, np > 0 ? q1 : q0, Pl, Bl, lane, gops, gskip);
 		uint32_t stat_chunks = 0;
 		const bool split = nseg > 1;

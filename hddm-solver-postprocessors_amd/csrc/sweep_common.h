@@ -46,17 +46,36 @@ __device__ __forceinline__ void decode_item(const A& a, int64_t item, int G, Ite
 	it.count = ldc(&a.plans[s].count);
 }
 
-// Coordinates of the lane's point are held in XB register banks of 16
-// doubles (separate arrays); a bank is indexed with a wave-uniform dim
-// (s_set_gpr_idx), never through scratch.
-template <int XB>
+// Coordinates of the lane's point are held in XB banks of 16 doubles
+// (separate arrays), indexed with a wave-uniform dim (s_set_gpr_idx).
+// SELECT chooses how more than one bank is read (see below): measured on a
+// D=20 workload, selecting values is faster where registers are left over
+// (FP64 paths), a selected pointer where they are not (MFMA paths).
+template <int XB, bool SELECT>
 __device__ __forceinline__ double xget(const double (&b0)[16], const double (&b1)[16], const double (&b2)[16],
 	const double (&b3)[16], uint32_t dim)
 {
 	const uint32_t idx = dim & 15u;
 	if constexpr (XB == 1) return b0[idx];
+	else if constexpr (SELECT)
+	{
+		// every bank is read with the same index and the value selected: no
+		// pointer into the banks is formed, so they stay in registers (more
+		// VGPRs, no scratch)
+		const uint32_t bank = dim >> 4;
+		const double v0 = b0[idx], v1 = b1[idx];
+		if constexpr (XB == 2) return bank == 0u ? v0 : v1;
+		else
+		{
+			const double v2 = b2[idx], v3 = b3[idx];
+			const double lo = bank == 0u ? v0 : v1, hi = bank == 2u ? v2 : v3;
+			return bank < 2u ? lo : hi;
+		}
+	}
 	else
 	{
+		// one bank read through a selected pointer: the compiler keeps the
+		// banks beyond the first in scratch, which costs fewer VGPRs
 		const uint32_t bank = dim >> 4;
 		if (bank == 0) return b0[idx];
 		if constexpr (XB == 2) return b1[idx];
