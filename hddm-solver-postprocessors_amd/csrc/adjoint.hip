@@ -6,8 +6,8 @@
 // with the piecewise-constant hat derivative d/dx max(0, 1 - |x i - j|) =
 // -i sign(x i - j) inside the support and 0 outside (and 0 at the peak).
 // The basis is recomputed by the same prefix-trie walk as the forward sweep
-// (workgroups of G waves, one 64-point block per wave, op stream through
-// LDS tiles). Along the walk each trie edge e at depth t keeps
+// (independent waves, one 64-point block per wave, the op stream through
+// scalar loads). Along the walk each trie edge e at depth t keeps
 //   R_e = sum over the surviving nodes k below e of w_k B_k,
 // and when the walk leaves e (a sibling or an ancestor's sibling comes next)
 // it adds R_e * phi'_e / phi_e to dx[dim(e)] and R_e to its parent edge:
@@ -105,16 +105,14 @@ __device__ __forceinline__ void contract_ds(const AdjointArgs& a, const StateDes
 template <int XB, int PM>
 __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 {
+	// Waves are independent: each pops its own items (one 64-point block
+	// against one node segment) and reads the op stream through scalar loads.
 	extern __shared__ __attribute__((aligned(16))) double lds[];
 	const int lane = threadIdx.x & 63;
 	const int wave = uniform((int)(threadIdx.x >> 6));
-	const int G = uniform((int)(blockDim.x >> 6));
 	constexpr int DXR = 16 * XB;
 	char* const lbase = (char*)lds;
-	int64_t* const item_slot = (int64_t*)lbase;
-	Op* const tile_ops = (Op*)(lbase + kLdsHeader);
-	uint32_t* const tile_skip = (uint32_t*)(lbase + kLdsHeader + 2 * (size_t)kTileOps * sizeof(Op));
-	double* const P = (double*)(lbase + kLdsHeader + kLdsTiles + (size_t)wave * AdjLds<XB, PM>::kBytes);
+	double* const P = (double*)(lbase + (size_t)wave * AdjLds<XB, PM>::kBytes);
 	double* const Q = P + PM * kWave;
 	double* const R = Q + PM * kWave;
 	double* const DX = R + PM * kWave;
@@ -124,26 +122,25 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 
 	for (;;)
 	{
-		if (threadIdx.x == 0) *item_slot = (int64_t)atomicAdd(&a.hdr->counter, 1ull);
-		__syncthreads();
-		const int64_t item = uniform64((uint64_t)*item_slot);
+		uint64_t iv0 = 0;
+		if (lane == 0) iv0 = atomicAdd(&a.hdr->counter, 1ull);
+		const int64_t item = (int64_t)uniform64(iv0);
 		if (item >= nitems) break;
 		ItemInfo it;
-		decode_item(a, item, G, it);
+		decode_item(a, item, 1, it);
 		const StateDesc* sd = a.states + it.s;
 		const int dofs = ldc(&sd->dofs);
 		const int DP = ldc(&sd->dofs_pad);
-		const int nops_s = ldc(&sd->nops);
 		const Op* const gops = ldc(&sd->ops);
-		const uint32_t* const gskip = ldc(&sd->skip);
 		const int32_t* const seg_op = ldc(&sd->seg_op);
 		const int q0 = ldc(seg_op + it.seg), q1 = ldc(seg_op + it.seg + 1);
 		const int dt0 = a.dt_base;
 		const bool active = 32 * dt0 < dofs;
 
-		const int64_t blk = it.grp * G + wave;
+		const int64_t blk = it.grp;
 		const int64_t rem = it.count - blk * kWave;
 		const int np = !active ? 0 : (rem <= 0 ? 0 : (rem < kWave ? (int)rem : kWave));
+		if (np == 0) continue;
 		const int myp = lane < np ? lane : (np > 0 ? np - 1 : 0);
 		const uint32_t orig = np > 0 ? a.perm[it.start + blk * kWave + myp] : 0u;
 		const double* xrow = a.x + (int64_t)orig * a.ldx;
@@ -180,94 +177,79 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 		const double* const s64 = ldc(&sd->s64) + 32 * dt0;
 		const bool want_ds = a.ds != nullptr;
 		int q = q0;
-		const int qend = np > 0 ? q1 : q0;
 		int top = -1;          // deepest live edge of the path
-		int last_t = -1;
 		double cur = 1.0, base_prev = 1.0;
 		int chunk = -1;
 		uint32_t mask = 0;
-
-		load_tile(gops, gskip, nops_s, q0, tile_ops, tile_skip, wave, G, lane);
-		__syncthreads();
-
-		int kb = 0;
-		for (int tb = q0; tb < q1; tb += kTileOps, kb ^= 1)
+		Op op = ldc_op(gops + q);
+		while (q < q1)
 		{
-			if (tb + kTileOps < q1)
-				load_tile(gops, gskip, nops_s, tb + kTileOps, tile_ops + (kb ^ 1) * kTileOps,
-					tile_skip + (kb ^ 1) * kTileOps, wave, G, lane);
-			const Op* const tops = tile_ops + kb * kTileOps;
-			const uint32_t* const tskip = tile_skip + kb * kTileOps;
-			const int tend = min(tb + kTileOps, qend);
-			while (q < tend)
+			// the stream ends with kOpPad sentinels, so q + 1 is in bounds
+			const Op nop = ldc_op(gops + q + 1);
+			const uint32_t dimw = op.dim, ctrl = op.ctrl;
+			const uint32_t dim = dimw & kOpDimMask;
+			const int t = (int)(ctrl & 15u);
+			// leave the edges at depths >= t
+			for (int tt = top; tt >= t; tt--)
 			{
-				const int sl = q - tb;
-				const u4_t ov = *(const u4_t*)(tops + sl);
-				const uint32_t dim = __builtin_amdgcn_readfirstlane(ov[0]) & kOpDimMask;
-				const uint32_t ctrl = __builtin_amdgcn_readfirstlane(ov[1]);
-				const uint32_t ihi = __builtin_amdgcn_readfirstlane(ov[2]);
-				const uint32_t njhi = __builtin_amdgcn_readfirstlane(ov[3]);
-				const int t = (int)(ctrl & 15u);
-				// leave the edges at depths >= t
-				for (int tt = top; tt >= t; tt--)
+				const double r = R[tt * kWave + lane];
+				if (wave_any(r != 0.0))
 				{
-					const double r = R[tt * kWave + lane];
-					if (wave_any(r != 0.0))
-					{
-						const uint32_t dd = __builtin_amdgcn_readfirstlane(dims[tt]);
-						DX[dd * kWave + lane] += r * Q[tt * kWave + lane];
-						if (tt > 0) R[(tt - 1) * kWave + lane] += r;
-					}
-				}
-				top = t;
-				double bt;
-				if (t == last_t + 1) bt = cur;
-				else if (t == last_t) bt = base_prev;
-				else bt = P[t * kWave + lane];
-				const double iv = f64_from_high(ihi);
-				const double u = fma(xget<XB, true>(xb0, xb1, xb2, xb3, dim), iv, f64_from_high(njhi));
-				const double phi = 1.0 - fabs(u);
-				const double p = bt * fmax(phi, 0.0);
-				P[(t + 1) * kWave + lane] = p;
-				Q[t * kWave + lane] = phi > 0.0 ? (u > 0.0 ? -iv : (u < 0.0 ? iv : 0.0)) / phi : 0.0;
-				R[t * kWave + lane] = 0.0;
-				if (lane == 0) dims[t] = dim;
-				base_prev = bt;
-				cur = p;
-				last_t = t;
-				if (!wave_any(p != 0.0))
-				{
-					q = (int)__builtin_amdgcn_readfirstlane(tskip[sl]);
-					continue;
-				}
-				q++;
-				if (ctrl & kOpLast)
-				{
-					const int n = (int)(ctrl >> kOpNodeShift);
-					const double* srow = s64 + (size_t)n * DP;
-					double w = 0.0;
-#pragma unroll
-					for (int c = 0; c < 32; c++) w = fma(g[c], ldc(srow + c), w);
-					R[t * kWave + lane] = w * p;
-					if (want_ds)
-					{
-						if ((n >> 4) != chunk)
-						{
-							if (chunk >= 0)
-							{
-								contract_ds(a, sd, Bc, mask, chunk, lane, og, ogvalid, dofs);
-#pragma unroll
-								for (int k = 0; k < kChunk; k++) Bc[k * kWave + lane] = 0.0;
-							}
-							chunk = n >> 4;
-							mask = 0;
-						}
-						Bc[(n & 15) * kWave + lane] = p;
-						mask |= 1u << (n & 15);
-					}
+					const uint32_t dd = __builtin_amdgcn_readfirstlane(dims[tt]);
+					DX[dd * kWave + lane] += r * Q[tt * kWave + lane];
+					if (tt > 0) R[(tt - 1) * kWave + lane] += r;
 				}
 			}
-			__syncthreads();
+			top = t;
+			// parent product from the op's relation to its stream predecessor
+			const uint32_t rel = (ctrl >> kOpRelShift) & 3u;
+			double bt;
+			if (rel == 0u) bt = cur;
+			else if (rel == 1u) bt = base_prev;
+			else bt = P[t * kWave + lane];
+			const double iv = f64_from_high(op.ihi);
+			const double u = fma(xget<XB, true>(xb0, xb1, xb2, xb3, dim), iv, f64_from_high(op.njhi));
+			const double phi = 1.0 - fabs(u);
+			const double p = bt * fmax(phi, 0.0);
+			if (ctrl & kOpKids) P[(t + 1) * kWave + lane] = p;
+			Q[t * kWave + lane] = phi > 0.0 ? (u > 0.0 ? -iv : (u < 0.0 ? iv : 0.0)) / phi : 0.0;
+			R[t * kWave + lane] = 0.0;
+			if (lane == 0) dims[t] = dim;
+			base_prev = bt;
+			cur = p;
+			if (!wave_any(p != 0.0))
+			{
+				q += (int)(dimw >> kOpSkipShift);
+				op = ldc_op(gops + q);
+				continue;
+			}
+			q++;
+			op = nop;
+			if (ctrl & kOpLast)
+			{
+				const int n = (int)(ctrl >> kOpNodeShift);
+				const double* srow = s64 + (size_t)n * DP;
+				double w = 0.0;
+#pragma unroll
+				for (int c = 0; c < 32; c++) w = fma(g[c], ldc(srow + c), w);
+				R[t * kWave + lane] = w * p;
+				if (want_ds)
+				{
+					if ((n >> 4) != chunk)
+					{
+						if (chunk >= 0)
+						{
+							contract_ds(a, sd, Bc, mask, chunk, lane, og, ogvalid, dofs);
+#pragma unroll
+							for (int k = 0; k < kChunk; k++) Bc[k * kWave + lane] = 0.0;
+						}
+						chunk = n >> 4;
+						mask = 0;
+					}
+					Bc[(n & 15) * kWave + lane] = p;
+					mask |= 1u << (n & 15);
+				}
+			}
 		}
 		if (want_ds && chunk >= 0) contract_ds(a, sd, Bc, mask, chunk, lane, og, ogvalid, dofs);
 		for (int tt = top; tt >= 0; tt--)
@@ -344,11 +326,10 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 	size_t per_wave = 0;
 	const void* kern = adjoint_kernel_ptr(xbanks_for(ctx->dim), ctx->fmax_all, per_wave);
 	if (!kern) return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: dim %d is not supported", ctx->dim);
-	const size_t fixed = sweep_lds_bytes(HDDM_LB_F32, 0, 0, kWalkLds);
-	int G = (int)std::min<size_t>(4, (ctx->max_lds - fixed) / per_wave);
+	int G = (int)std::min<size_t>(4, ctx->max_lds / per_wave);
 	if (ctx->opt_waves) G = std::min(G, ctx->opt_waves);
 	if (G < 1) return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: LDS of one wave (%zu B) exceeds the CU", per_wave);
-	const size_t lds = fixed + (size_t)G * per_wave;
+	const size_t lds = (size_t)G * per_wave;
 	HIP_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
 	int per_cu = 0;
 	HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, G * kWave, lds));
@@ -361,7 +342,7 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 	const int64_t nb = (int64_t)(ns + 1) << bb;
 	HIP_TRY(launch_fill_u32(sc.hist, nb, 0u, st));
 	HIP_TRY(launch_bucket_key(npoints, x, ldx, state_ids, ctx->d_state_ok, ns, ctx->dim, bb, sc.bkey, sc.hist, st));
-	HIP_TRY(launch_scan_plan(sc.hist, sc.cursor, nb, bb, ns, ctx->d_states, ctx->d_cost_order, sc.plans, sc.hdr, G, st));
+	HIP_TRY(launch_scan_plan(sc.hist, sc.cursor, nb, bb, ns, ctx->d_states, ctx->d_cost_order, sc.plans, sc.hdr, 1, st));
 	HIP_TRY(launch_scatter(npoints, sc.bkey, sc.cursor, sc.perm, st));
 
 	AdjointArgs a;

@@ -55,7 +55,8 @@ typedef struct hddm_lb_grid hddm_lb_grid;
 
 /* options for hddm_lb_set_option */
 #define HDDM_LB_OPT_AUTOTUNE      1  /* 1: time tile candidates on first use of a (D, nno, dofs) key */
-#define HDDM_LB_OPT_SEGMENT_NODES 2  /* node-range length of one work item (0: automatic) */
+#define HDDM_LB_OPT_SEGMENT_NODES 2  /* node-range length of one work item (0: automatic, eighths of states
+                                        of >= 16k nodes; a value >= nno keeps states whole) */
 #define HDDM_LB_OPT_WAVES_PER_WG  3  /* waves per workgroup (1, 2, 4; 0: automatic) */
 #define HDDM_LB_OPT_SORT_POINTS   4  /* 1 (default): spatially sort points before the sweep */
 #define HDDM_LB_OPT_TIMING        5  /* 1: time the sweep kernels with hipEvents (hddm_lb_last_sweep_ms) */
