@@ -235,6 +235,15 @@ def main():
                      "mfma_tflops": nchunk * 2.0 * kChunkNodes * 64 * 32 * PRODUCTS[args.precision]
                      / (sweep_ms * 1e-3) / 1e12},
     }
+    # the same forward through host buffers (what the plugin's interpolate()
+    # uses): x in and y out over PCIe, synchronous
+    yh = np.empty((N, dofs))
+    ctx.forward_host(X, yh, None, precs[args.precision])
+    t0 = time.perf_counter()
+    ctx.forward_host(X, yh, None, precs[args.precision])
+    host_ms = (time.perf_counter() - t0) * 1e3
+    out["host_buffers"] = {"ms_per_step": host_ms, "value": N * dofs / (host_ms * 1e-3),
+                           "note": "x and y in pageable host memory, copies included"}
     if not args.no_adjoint:
         out["forward_adjoint"] = time_adjoint(args, precs[args.precision])
     if not args.no_cpu:

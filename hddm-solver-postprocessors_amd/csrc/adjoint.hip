@@ -212,16 +212,25 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 			const double phi = 1.0 - fabs(u);
 			const double p = bt * fmax(phi, 0.0);
 			if (ctrl & kOpKids) P[(t + 1) * kWave + lane] = p;
-			Q[t * kWave + lane] = phi > 0.0 ? (u > 0.0 ? -iv : (u < 0.0 ? iv : 0.0)) / phi : 0.0;
 			R[t * kWave + lane] = 0.0;
 			if (lane == 0) dims[t] = dim;
 			base_prev = bt;
 			cur = p;
 			if (!wave_any(p != 0.0))
 			{
+				// the edge's sum stays 0, so its phi'/phi is never read
 				q += (int)(dimw >> kOpSkipShift);
 				op = ldc_op(gops + q);
 				continue;
+			}
+			// phi'/phi inside the support: hardware reciprocal refined by two
+			// Newton steps (error ~1 ulp), instead of a full FP64 division
+			{
+				const double ph = phi > 0.0 ? phi : 1.0;
+				double r = __builtin_amdgcn_rcp(ph);
+				r = fma(r, fma(-ph, r, 1.0), r);
+				r = fma(r, fma(-ph, r, 1.0), r);
+				Q[t * kWave + lane] = phi > 0.0 ? (u > 0.0 ? -iv : (u < 0.0 ? iv : 0.0)) * r : 0.0;
 			}
 			q++;
 			op = nop;
