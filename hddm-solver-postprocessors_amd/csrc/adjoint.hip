@@ -184,8 +184,10 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 		Op op = ldc_op(gops + q);
 		while (q < q1)
 		{
-			// the stream ends with kOpPad sentinels, so q + 1 is in bounds
+			// the stream ends with kOpPad sentinels, so q + 1 is in bounds;
+			// the load is issued first (scheduling fence) and stays in flight
 			const Op nop = ldc_op(gops + q + 1);
+			__builtin_amdgcn_sched_barrier(0);
 			const uint32_t dimw = op.dim, ctrl = op.ctrl;
 			const uint32_t dim = dimw & kOpDimMask;
 			const int t = (int)(ctrl & 15u);
@@ -218,9 +220,11 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 			cur = p;
 			if (!wave_any(p != 0.0))
 			{
-				// the edge's sum stays 0, so its phi'/phi is never read
-				q += (int)(dimw >> kOpSkipShift);
-				op = ldc_op(gops + q);
+				// the edge's sum stays 0, so its phi'/phi is never read; a
+				// pruned leaf moves to the already loaded next op
+				const int rs = (int)(dimw >> kOpSkipShift);
+				q += rs;
+				op = rs == 1 ? nop : ldc_op(gops + q);
 				continue;
 			}
 			// phi'/phi inside the support: hardware reciprocal refined by two

@@ -393,14 +393,23 @@ struct Walker
 			while (q < tend)
 			{
 				const uint32_t dimw = op.dim, ctrl = op.ctrl;
-				// the stream ends with kOpPad sentinels, so q + 1 is in bounds
+				// the stream ends with kOpPad sentinels, so q + 1 is in bounds;
+				// the load is issued first and stays in flight over the FP64
+				// chain (the scheduling fence keeps it at the top)
 				const Op nop = ldc_op(gops + q + 1);
+				__builtin_amdgcn_sched_barrier(0);
 				const int t = (int)(ctrl & 15u);
 				const uint32_t rel = (ctrl >> kOpRelShift) & 3u;
 				double bt;
 				if (rel == 0u) bt = cur;
 				else if (rel == 1u) bt = base_prev;
-				else bt = P[t * kWave + lane];
+				else
+				{
+					bt = P[t * kWave + lane];
+					// wait here, on the stack-read path only, so the other
+					// paths do not wait for the load above
+					__builtin_amdgcn_s_waitcnt(0xc07f);
+				}
 				const double tt = fma(xget<XB, XSEL>(xb0, xb1, xb2, xb3, dimw & kOpDimMask),
 					f64_from_high(op.ihi), f64_from_high(op.njhi));
 				const double p = bt * fmax(1.0 - fabs(tt), 0.0);
@@ -411,8 +420,12 @@ struct Walker
 				if (!wave_any(p != 0.0))
 				{
 					nprunes++;
-					q += (int)(dimw >> kOpSkipShift);
-					op = ldc_op(gops + q);
+					// a pruned leaf skips to the next op, which is already
+					// loaded; using nop on both paths also keeps its load at
+					// the top of the iteration, ahead of the FP64 chain
+					const int rs = (int)(dimw >> kOpSkipShift);
+					q += rs;
+					op = rs == 1 ? nop : ldc_op(gops + q);
 					continue;
 				}
 				q++;
