@@ -169,9 +169,6 @@ extern "C" int hddm_lb_set_option(hddm_lb_ctx* ctx, int option, int64_t value)
 	case HDDM_LB_OPT_WALK:
 		if (value < 0 || value > 2) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "set_option: walk must be 0, 1 or 2");
 		ctx->opt_walk = (int)value; break;
-	case HDDM_LB_OPT_FLUSH_LOG2:
-		if (value < 0 || value > 20) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "set_option: flush log2 in [0, 20]");
-		ctx->opt_wlog = (int)value; break;
 	default:
 		return set_error(HDDM_LB_ERROR_INVALID_VALUE, "set_option: unknown option %d", option);
 	}
@@ -411,7 +408,6 @@ TuneConfig default_config(hddm_lb_ctx* ctx, int prec)
 	int slab = ctx->opt_slab;
 	if (!slab || prec == HDDM_LB_FP64) slab = 1;
 	c.slab = slab;
-	c.wlog = ctx->opt_wlog >= 0 ? ctx->opt_wlog : 0;
 	return c;
 }
 
@@ -445,7 +441,7 @@ int enqueue_forward(hddm_lb_ctx* ctx, Scratch& sc, int64_t npoints, const double
 	std::memset(&a, 0, sizeof(a));
 	a.states = ctx->d_states; a.plans = sc.plans; a.cost_order = ctx->d_cost_order; a.hdr = sc.hdr;
 	a.perm = sc.perm; a.x = x; a.ldx = ldx; a.y = y; a.ldy = ldy; a.partial = sc.partial;
-	a.nstates = ns; a.dim = ctx->dim; a.wlog = cfg.wlog;
+	a.nstates = ns; a.dim = ctx->dim;
 	if (ctx->opt_stats && !in_capture)
 	{
 		if (!ctx->d_stats)

@@ -58,7 +58,6 @@ struct TuneConfig
 	int waves = 4;     // waves per workgroup
 	int walk = 1;      // kWalkSmem (1) or kWalkLds (0)
 	int slab = 1;      // dof tiles of 32 per sweep pass
-	int wlog = 0;      // log2 flush window (chunks)
 	int64_t seg = -1;  // node segment length used while tuning (-1: context option)
 };
 
@@ -79,7 +78,6 @@ struct hddm_lb_ctx
 	bool opt_sort = true;
 	bool opt_timing = false;
 	int opt_slab = 0;
-	int opt_wlog = -1;
 	int opt_walk = 0;      // 0 automatic, 1 scalar loads, 2 LDS tiles
 
 	// device tables

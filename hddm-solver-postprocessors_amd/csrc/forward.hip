@@ -319,9 +319,11 @@ __device__ __forceinline__ void contract_chunk(const SOperands<PREC, DT>& so, in
 //   kWalkSmem  scalar loads from the state's op stream, the sequential
 //              successor read one op ahead; waves are independent
 //   kWalkLds   the workgroup's LDS tile (wave-uniform vector reads)
-template <int XB, typename BT, int WALK, int PM, bool XSEL>
+template <int XB, typename BT, int WALK, int PM, bool XSEL, int CB>
 struct Walker
 {
+	static constexpr int kC = 1 << CB;        // nodes per chunk
+	static constexpr int kCM = kC - 1;
 	double* P;          // [PM][64] in LDS
 	BT* B;              // [16][64] in LDS: basis values of the current chunk
 	double base_prev;   // P[last_t]
@@ -433,9 +435,9 @@ struct Walker
 				if (ctrl & kOpLast)
 				{
 					const int n = (int)(ctrl >> kOpNodeShift);
-					if ((n >> 4) != chunk) { node = n; val = p; return true; }
-					B[(n & 15) * kWave + lane] = (BT)p;
-					mask |= 1u << (n & 15);
+					if ((n >> CB) != chunk) { node = n; val = p; return true; }
+					B[(n & kCM) * kWave + lane] = (BT)p;
+					mask |= 1u << (n & kCM);
 				}
 			}
 			return false;
@@ -461,9 +463,9 @@ struct Walker
 				if (ctrl & kOpLast)
 				{
 					const int n = (int)(ctrl >> kOpNodeShift);
-					if ((n >> 4) != chunk) { node = n; val = p; return true; }
-					B[(n & 15) * kWave + lane] = (BT)p;
-					mask |= 1u << (n & 15);
+					if ((n >> CB) != chunk) { node = n; val = p; return true; }
+					B[(n & kCM) * kWave + lane] = (BT)p;
+					mask |= 1u << (n & kCM);
 				}
 			}
 			return false;
@@ -472,18 +474,26 @@ struct Walker
 
 	__device__ __forceinline__ void start_chunk(int node, double val, int lane)
 	{
-		chunk = node >> 4;
-		mask = 1u << (node & 15);
-		B[(node & 15) * kWave + lane] = (BT)val;
+		chunk = node >> CB;
+		mask = 1u << (node & kCM);
+		B[(node & kCM) * kWave + lane] = (BT)val;
 	}
 };
 
-// Per-wave LDS stage: the path stack [PM][64] and the chunk buffer [16][64], doubles.
-template <int PM>
-constexpr size_t wave_stage_bytes() { return (size_t)(PM + kChunk) * kWave * 8; }
+// Node chunk of the walk's basis buffer: 16 nodes (the MFMA K extent) for
+// the MFMA paths, 8 for the FP64 path, which contracts node by node and
+// whose smaller buffer leaves LDS for more resident waves.
+template <int PREC>
+constexpr int chunk_bits() { return PREC == HDDM_LB_FP64 ? 3 : 4; }
+
+// Per-wave LDS stage: the path stack [PM][64] and the chunk buffer [2^CB][64], doubles.
+template <int PM, int CB>
+constexpr size_t wave_stage_bytes() { return (size_t)(PM + (1 << CB)) * kWave * 8; }
 
 template <int PREC, int DT, int XB, int PM, int WALK>
-__global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
+__global__ void __launch_bounds__(256)
+	__attribute__((amdgpu_waves_per_eu(PREC != HDDM_LB_FP64 && XB == 1 && DT == 1 ? 3 : 1)))
+	sweep_kernel(SweepArgs a)
 {
 	extern __shared__ __attribute__((aligned(16))) double lds[];
 	const int lane = threadIdx.x & 63;
@@ -497,7 +507,8 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 	Op* const tile_ops = (Op*)(lbase + kLdsHeader);
 	uint32_t* const tile_skip = (uint32_t*)(lbase + kLdsHeader + 2 * (size_t)kTileOps * sizeof(Op));
 	const size_t stage0 = WALK == kWalkLds ? kLdsHeader + kLdsTiles : 0;
-	double* const Pl = (double*)(lbase + stage0 + (size_t)wave * wave_stage_bytes<PM>());
+	constexpr int CB = chunk_bits<PREC>();
+	double* const Pl = (double*)(lbase + stage0 + (size_t)wave * wave_stage_bytes<PM, CB>());
 	BT* const Bl = (BT*)(Pl + PM * kWave);
 	const int64_t nitems = ldc(&a.hdr->nitems);
 
@@ -549,9 +560,9 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 			xb3[d] = XB > 2 && np > 0 ? xrow[48 + d < a.dim ? 48 + d : 0] : 0.0;
 		}
 #pragma unroll
-		for (int k = 0; k < kChunk; k++) Bl[k * kWave + lane] = (BT)0;
+		for (int k = 0; k < (1 << CB); k++) Bl[k * kWave + lane] = (BT)0;
 
-		Walker<XB, BT, WALK, PM, PREC == HDDM_LB_FP64> w;
+		Walker<XB, BT, WALK, PM, PREC == HDDM_LB_FP64, CB> w;
 		w.init(q0, np > 0 ? q1 : q0, Pl, Bl, lane, gops, gskip);
 		uint32_t stat_chunks = 0;
 		const bool split = nseg > 1;
@@ -578,7 +589,7 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 				{
 					const int kk = __builtin_ctz(m);
 					const double b = Bl[kk * kWave + lane];
-					const double* srow = s64 + (size_t)(w.chunk * kChunk + kk) * DP;
+					const double* srow = s64 + (size_t)((w.chunk << CB) + kk) * DP;
 #pragma unroll
 					for (int c = 0; c < 32; c++) acc[c] = fma(b, ldc(srow + c), acc[c]);
 				}
@@ -615,35 +626,35 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 		}
 		else
 		{
-			// FP32 accumulation in the MFMA accumulators inside fixed windows
-			// of 2^wlog chunks, folded into FP64 at window changes.
-			f16_t acc[2][DT];
+			// Each chunk's MFMA sum starts from zero and is folded into the
+			// FP64 accumulators when the chunk is contracted.
 			double acc64[2][DT][16];
 #pragma unroll
 			for (int t = 0; t < 2; t++)
 #pragma unroll
 				for (int dt = 0; dt < DT; dt++)
 #pragma unroll
-					for (int r = 0; r < 16; r++) { acc[t][dt][r] = 0.0f; acc64[t][dt][r] = 0.0; }
-			int cur_win = -1;
+					for (int r = 0; r < 16; r++) acc64[t][dt][r] = 0.0;
 			SOperands<PREC, DT> so;
 			auto contract = [&]() {
 				if (a.stats) stat_chunks++;
-				const int win = w.chunk >> a.wlog;
-				if (win != cur_win)
-				{
+				f16_t acc[2][DT];
 #pragma unroll
-					for (int t = 0; t < 2; t++)
+				for (int t = 0; t < 2; t++)
 #pragma unroll
-						for (int dt = 0; dt < DT; dt++)
+					for (int dt = 0; dt < DT; dt++)
 #pragma unroll
-							for (int r = 0; r < 16; r++) { acc64[t][dt][r] += (double)acc[t][dt][r]; acc[t][dt][r] = 0.0f; }
-					cur_win = win;
-				}
+						for (int r = 0; r < 16; r++) acc[t][dt][r] = 0.0f;
 				double b[kChunk];
 #pragma unroll
 				for (int kk = 0; kk < kChunk; kk++) { b[kk] = Bl[kk * kWave + lane]; Bl[kk * kWave + lane] = 0.0; }
 				contract_chunk<PREC, DT>(so, ndt, b, acc);
+#pragma unroll
+				for (int t = 0; t < 2; t++)
+#pragma unroll
+					for (int dt = 0; dt < DT; dt++)
+#pragma unroll
+						for (int r = 0; r < 16; r++) acc64[t][dt][r] += (double)acc[t][dt][r];
 			};
 			int k = 0;
 			for (int tb = q0; tb < q1; tb += tstep, k ^= 1)
@@ -667,12 +678,6 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 				if constexpr (WALK == kWalkLds) __syncthreads();
 			}
 			if (w.chunk >= 0) contract();
-#pragma unroll
-			for (int t = 0; t < 2; t++)
-#pragma unroll
-				for (int dt = 0; dt < DT; dt++)
-#pragma unroll
-					for (int r = 0; r < 16; r++) acc64[t][dt][r] += (double)acc[t][dt][r];
 
 			// C layout of v_mfma_f32_32x32x*: register r of lane l holds
 			// row 8*(r/4) + 4*(l/32) + r%4 (the dof) and column l%32 (the point).
@@ -718,8 +723,9 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs a)
 
 size_t sweep_lds_bytes(int prec, int waves, int depth, int walk)
 {
-	(void)prec;
-	const size_t stage = depth < 8 ? wave_stage_bytes<8>() : wave_stage_bytes<16>();
+	const bool c8 = prec == HDDM_LB_FP64;
+	const size_t stage = depth < 8 ? (c8 ? wave_stage_bytes<8, 3>() : wave_stage_bytes<8, 4>())
+		: (c8 ? wave_stage_bytes<16, 3>() : wave_stage_bytes<16, 4>());
 	return (walk == kWalkLds ? kLdsHeader + kLdsTiles : 0) + (size_t)waves * stage;
 }
 

@@ -23,7 +23,6 @@ struct SweepArgs
 	double* partial;           // node-segment partials of split states
 	int nstates;
 	int dim;
-	int wlog;                  // log2 of the FP32 accumulation window, in 16-node chunks
 	int dt_base;               // first dof tile of 32 handled by this pass
 	unsigned long long* stats; // optional: ops evaluated, prunes, chunks contracted, items
 };

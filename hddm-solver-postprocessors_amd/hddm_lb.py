@@ -26,7 +26,7 @@ FP64, F16X2, BF16X3, F32 = 0, 1, 2, 3
 PRECISIONS = {"fp64": FP64, "f16x2": F16X2, "bf16x3": BF16X3, "f32": F32}
 
 OPT_AUTOTUNE, OPT_SEGMENT_NODES, OPT_WAVES_PER_WG, OPT_SORT_POINTS = 1, 2, 3, 4
-OPT_TIMING, OPT_SLAB_TILES, OPT_FLUSH_LOG2, OPT_STATS, OPT_WALK = 5, 6, 7, 8, 9
+OPT_TIMING, OPT_SLAB_TILES, OPT_STATS, OPT_WALK = 5, 6, 8, 9
 
 SUCCESS = 0
 ERROR_NAMES = {1: "INVALID_VALUE", 2: "NO_DEVICE", 3: "OUT_OF_MEMORY", 4: "HIP",

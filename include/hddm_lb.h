@@ -61,7 +61,7 @@ typedef struct hddm_lb_grid hddm_lb_grid;
 #define HDDM_LB_OPT_SORT_POINTS   4  /* 1 (default): spatially sort points before the sweep */
 #define HDDM_LB_OPT_TIMING        5  /* 1: time the sweep kernels with hipEvents (hddm_lb_last_sweep_ms) */
 #define HDDM_LB_OPT_SLAB_TILES    6  /* dof tiles of 32 per sweep pass (1, 2; 0: automatic) */
-#define HDDM_LB_OPT_FLUSH_LOG2    7  /* log2 of the FP32->FP64 flush window in 16-node chunks (default 0: every chunk) */
+/* option 7 is unused: MFMA partial sums are folded into FP64 after every chunk */
 #define HDDM_LB_OPT_STATS         8  /* 1: count walk ops, prunes, contracted chunks and items (hddm_lb_last_stats) */
 #define HDDM_LB_OPT_WALK          9  /* op stream source: 0 automatic, 1 scalar loads per wave, 2 LDS tiles per workgroup */
 
