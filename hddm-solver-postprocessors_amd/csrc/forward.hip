@@ -312,6 +312,112 @@ __device__ __forceinline__ void contract_chunk(const SOperands<PREC, DT>& so, in
 
 } // namespace
 
+typedef double d16_t __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ uint32_t lds_offset(const void* p)
+{
+	return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
+}
+
+// Hand-scheduled scalar-load walk for points whose coordinates fit one bank
+// (D <= 16). Walks ops from byte offset qb until the first surviving node of
+// a chunk other than `chunk` (returned in node and val, not stored; qb is
+// then past its op) or until qb reaches qe (node = -1).
+// Per op: the successor and the skip target are loaded at the top; the
+// parent product comes from the LDS path stack (P[t], lane-strided rows of
+// 512 bytes at pl) and the op's product is stored as P[t+1], so P[k] is the
+// product of the first k factors of the current trie path; the coordinate is
+// read from the bank with an indexed move; a wave-wide zero product skips
+// the op's run, a live last factor emits its node into the chunk buffer at bl.
+// Fixed registers: the op in s[76:79], successor s[80:83], skip target
+// s[84:87], the FP64 i as s[94:95] (low word 0), x in v[64:95], temporaries
+// v[96:103] (v98 = 0 is the low word of -j).
+template <int CB>
+__device__ __forceinline__ int walk_op_stream(const char* ob, uint32_t& qb, uint32_t qe, int chunk,
+	u4_t& opv, uint32_t& mask, uint32_t& nops, uint32_t& nprunes, uint32_t pl, uint32_t bl,
+	const d16_t& xv, double& val)
+{
+	int node;
+	asm volatile(
+		"v_mov_b32 v98, 0\n"
+		"s_mov_b32 s94, 0\n"
+		"L_top_%=:\n"
+		"s_lshr_b32 s90, s76, 4\n"                       // skip bytes = (dimw >> 8) * 16
+		"s_and_b32 s90, s90, 0x0ffffff0\n"
+		"s_add_u32 s91, %[qb], s90\n"
+		"s_load_dwordx4 s[80:83], %[ob], %[qb] offset:0x10\n"
+		"s_load_dwordx4 s[84:87], %[ob], s91\n"
+		"s_lshl_b32 s88, s77, 9\n"                       // t * 512
+		"s_and_b32 s88, s88, 0x1e00\n"
+		"v_add_u32 v102, s88, %[pl]\n"
+		"ds_read_b64 v[100:101], v102\n"                 // P[t]
+		"s_lshl_b32 s89, s76, 1\n"                       // 2 * (dim & 15)
+		"s_and_b32 s89, s89, 30\n"
+		"s_set_gpr_idx_on s89, gpr_idx(SRC0)\n"
+		"v_mov_b32 v96, v64\n"
+		"v_mov_b32 v97, v65\n"
+		"s_set_gpr_idx_off\n"
+		"s_mov_b32 s95, s78\n"
+		"v_mov_b32 v99, s79\n"
+		"v_fma_f64 v[96:97], v[96:97], s[94:95], v[98:99]\n"   // x * i - j
+		"v_add_f64 v[96:97], -|v[96:97]|, 1.0\n"
+		"v_max_f64 v[96:97], v[96:97], 0\n"
+		"s_waitcnt lgkmcnt(0)\n"
+		"v_mul_f64 v[96:97], v[100:101], v[96:97]\n"
+		"ds_write_b64 v102, v[96:97] offset:512\n"       // P[t+1]
+		"s_add_u32 %[nops], %[nops], 1\n"
+		"v_cmp_neq_f64_e32 vcc, 0, v[96:97]\n"
+		"s_cbranch_vccz L_prune_%=\n"
+		"s_bitcmp1_b32 s77, 4\n"
+		"s_cbranch_scc1 L_last_%=\n"
+		"L_adv_%=:\n"
+		"s_add_u32 %[qb], %[qb], 16\n"
+		"s_mov_b64 s[76:77], s[80:81]\n"
+		"s_mov_b64 s[78:79], s[82:83]\n"
+		"s_cmp_lt_u32 %[qb], %[qe]\n"
+		"s_cbranch_scc1 L_top_%=\n"
+		"s_branch L_end_%=\n"
+		"L_last_%=:\n"
+		"s_lshr_b32 s88, s77, 8\n"                       // node
+		"s_lshr_b32 s89, s88, %[cb]\n"
+		"s_cmp_lg_u32 s89, %[chunk]\n"
+		"s_cbranch_scc1 L_new_%=\n"
+		"s_and_b32 s89, s88, %[cm]\n"
+		"s_lshl_b32 s91, 1, s89\n"
+		"s_or_b32 %[mask], %[mask], s91\n"
+		"s_lshl_b32 s89, s89, 9\n"
+		"v_add_u32 v103, s89, %[bl]\n"
+		"ds_write_b64 v103, v[96:97]\n"
+		"s_branch L_adv_%=\n"
+		"L_prune_%=:\n"
+		"s_add_u32 %[nprunes], %[nprunes], 1\n"
+		"s_add_u32 %[qb], %[qb], s90\n"
+		"s_mov_b64 s[76:77], s[84:85]\n"
+		"s_mov_b64 s[78:79], s[86:87]\n"
+		"s_cmp_lt_u32 %[qb], %[qe]\n"
+		"s_cbranch_scc1 L_top_%=\n"
+		"s_branch L_end_%=\n"
+		"L_new_%=:\n"
+		"s_add_u32 %[qb], %[qb], 16\n"
+		"s_mov_b64 s[76:77], s[80:81]\n"
+		"s_mov_b64 s[78:79], s[82:83]\n"
+		"s_mov_b32 %[node], s88\n"
+		"v_mov_b64_e32 %[val], v[96:97]\n"
+		"s_branch L_exit_%=\n"
+		"L_end_%=:\n"
+		"s_mov_b32 %[node], -1\n"
+		"L_exit_%=:\n"
+		"s_waitcnt lgkmcnt(0)\n"
+		: [qb] "+s"(qb), [mask] "+s"(mask), [nops] "+s"(nops), [nprunes] "+s"(nprunes),
+		  [node] "=&s"(node), [val] "=&v"(val), "+{s[76:79]}"(opv)
+		: [ob] "s"(ob), [qe] "s"(qe), [chunk] "s"(chunk), [pl] "v"(pl), [bl] "v"(bl),
+		  [cb] "i"(CB), [cm] "i"((1 << CB) - 1), "{v[64:95]}"(xv)
+		: "s80", "s81", "s82", "s83", "s84", "s85", "s86", "s87", "s88", "s89", "s90", "s91",
+		  "s94", "s95", "v96", "v97", "v98", "v99", "v100", "v101", "v102", "v103",
+		  "vcc", "scc", "memory");
+	return node;
+}
+
 // Walk state of one wave. P[t] (the product of the first t factors of the
 // current trie path) lives in the wave's LDS stage; the two most recent
 // products stay in registers, so only a move up the trie reads LDS.
@@ -387,7 +493,26 @@ struct Walker
 		const double (&xb0)[16], const double (&xb1)[16], const double (&xb2)[16], const double (&xb3)[16],
 		int& node, double& val)
 	{
-		if constexpr (WALK == kWalkSmem)
+		if constexpr (WALK == kWalkSmem && XB == 1)
+		{
+			uint32_t qb = (uint32_t)q * (uint32_t)sizeof(Op);
+			const uint32_t qe = (uint32_t)tend * (uint32_t)sizeof(Op);
+			if (qb >= qe) return false;
+			u4_t opv = { op.dim, op.ctrl, op.ihi, op.njhi };
+			d16_t xv;
+#pragma unroll
+			for (int i = 0; i < 16; i++) xv[i] = xb0[i];
+			double v = 0.0;
+			const int n = walk_op_stream<CB>((const char*)gops, qb, qe, chunk, opv, mask, nops, nprunes,
+				lds_offset(P) + (uint32_t)lane * 8u, lds_offset(B) + (uint32_t)lane * 8u, xv, v);
+			op.dim = opv[0]; op.ctrl = opv[1]; op.ihi = opv[2]; op.njhi = opv[3];
+			q = (int)(qb / (uint32_t)sizeof(Op));
+			if (n < 0) return false;
+			node = n;
+			val = v;
+			return true;
+		}
+		else if constexpr (WALK == kWalkSmem)
 		{
 			// Scalar-load walk on the precomputed relation and children bits:
 			// no depth compares, the stack is read only on a move up the trie
