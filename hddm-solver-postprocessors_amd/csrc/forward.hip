@@ -770,10 +770,37 @@ __global__ void __launch_bounds__(256)
 					for (int dt = 0; dt < DT; dt++)
 #pragma unroll
 						for (int r = 0; r < 16; r++) acc[t][dt][r] = 0.0f;
-				double b[kChunk];
+				if constexpr (PREC == HDDM_LB_F32)
+				{
+					// K = 2 nodes per MFMA step: steps whose two positions
+					// received no node are skipped (wave-uniform mask)
 #pragma unroll
-				for (int kk = 0; kk < kChunk; kk++) { b[kk] = Bl[kk * kWave + lane]; Bl[kk * kWave + lane] = 0.0; }
-				contract_chunk<PREC, DT>(so, ndt, b, acc);
+					for (int q = 0; q < 8; q++)
+						if ((w.mask >> (2 * q)) & 3u)
+						{
+							const double b0 = Bl[(2 * q) * kWave + lane], b1 = Bl[(2 * q + 1) * kWave + lane];
+							Bl[(2 * q) * kWave + lane] = 0.0;
+							Bl[(2 * q + 1) * kWave + lane] = 0.0;
+							auto r = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, (float)b0),
+								__builtin_bit_cast(uint32_t, (float)b1), false, false);
+							const float B0 = __builtin_bit_cast(float, (uint32_t)r[0]);
+							const float B1 = __builtin_bit_cast(float, (uint32_t)r[1]);
+#pragma unroll
+							for (int dt = 0; dt < DT; dt++)
+								if (dt < ndt)
+								{
+									acc[0][dt] = __builtin_amdgcn_mfma_f32_32x32x2f32(so.A[dt][q], B0, acc[0][dt], 0, 0, 0);
+									acc[1][dt] = __builtin_amdgcn_mfma_f32_32x32x2f32(so.A[dt][q], B1, acc[1][dt], 0, 0, 0);
+								}
+						}
+				}
+				else
+				{
+					double b[kChunk];
+#pragma unroll
+					for (int kk = 0; kk < kChunk; kk++) { b[kk] = Bl[kk * kWave + lane]; Bl[kk * kWave + lane] = 0.0; }
+					contract_chunk<PREC, DT>(so, ndt, b, acc);
+				}
 #pragma unroll
 				for (int t = 0; t < 2; t++)
 #pragma unroll
