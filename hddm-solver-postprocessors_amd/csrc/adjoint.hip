@@ -189,7 +189,7 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 			const Op nop = ldc_op(gops + q + 1);
 			__builtin_amdgcn_sched_barrier(0);
 			const uint32_t dimw = op.dim, ctrl = op.ctrl;
-			const uint32_t dim = dimw & kOpDimMask;
+			const uint32_t dim = op_dim(dimw);
 			const int t = (int)(ctrl & 15u);
 			// leave the edges at depths >= t
 			for (int tt = top; tt >= t; tt--)

@@ -323,100 +323,112 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 // (D <= 16). Walks ops from byte offset qb until the first surviving node of
 // a chunk other than `chunk` (returned in node and val, not stored; qb is
 // then past its op) or until qb reaches qe (node = -1).
-// Per op: the successor and the skip target are loaded at the top; the
-// parent product comes from the LDS path stack (P[t], lane-strided rows of
-// 512 bytes at pl) and the op's product is stored as P[t+1], so P[k] is the
-// product of the first k factors of the current trie path; the coordinate is
-// read from the bank with an indexed move; a wave-wide zero product skips
-// the op's run, a live last factor emits its node into the chunk buffer at bl.
-// Fixed registers: the op in s[76:79], successor s[80:83], skip target
-// s[84:87], the FP64 i as s[94:95] (low word 0), x in v[64:95], temporaries
-// v[96:103] (v98 = 0 is the low word of -j).
+// Per op: the successor is loaded at the top; the parent product comes from
+// the LDS path stack (P[t], lane-strided rows of 512 bytes at pl) and the
+// op's product is stored as P[t+1], so P[k] is the product of the first k
+// factors of the current trie path; the coordinate is read from the bank
+// with an indexed move whose index is the op's dim field (twice the
+// dimension); a wave-wide zero product skips the op's run (the skip target
+// is loaded only when it is not the successor); a live last factor emits its
+// node into the chunk buffer at bl.
+// The step is written twice, for the op in s[76:79] (successor into
+// s[80:83]) and for the op in s[80:83] (successor into s[76:79]), so no op
+// is copied between registers; the op handed back is in s[76:79].
+// Fixed registers: the FP64 i as s[94:95] (low word 0), x in v[64:95],
+// temporaries s88-s90, v[96:103] (v98 = 0 is the low word of -j); the
+// indexed moves set m0.
+#define HDDM_WALK_STEP(X, X0, X1, X2, X3, Y, YQ)                                       \
+	"L_top" X "_%=:\n"                                                                 \
+	"s_load_dwordx4 " YQ ", %[ob], %[qb] offset:0x10\n"                                \
+	"s_lshl_b32 s88, " X1 ", 9\n"                                                      \
+	"s_and_b32 s88, s88, 0x1e00\n"                                                     \
+	"v_add_u32 v102, s88, %[pl]\n"                                                     \
+	"ds_read_b64 v[100:101], v102\n"                                                   \
+	"s_set_gpr_idx_on " X0 ", gpr_idx(SRC0)\n"                                         \
+	"v_mov_b32 v96, v64\n"                                                             \
+	"v_mov_b32 v97, v65\n"                                                             \
+	"s_set_gpr_idx_off\n"                                                              \
+	"s_mov_b32 s95, " X2 "\n"                                                          \
+	"v_mov_b32 v99, " X3 "\n"                                                          \
+	"v_fma_f64 v[96:97], v[96:97], s[94:95], v[98:99]\n"                               \
+	"v_add_f64 v[96:97], -|v[96:97]|, 1.0\n"                                           \
+	"v_max_f64 v[96:97], v[96:97], 0\n"                                                \
+	"s_waitcnt lgkmcnt(0)\n"                                                           \
+	"v_mul_f64 v[96:97], v[100:101], v[96:97]\n"                                       \
+	"ds_write_b64 v102, v[96:97] offset:512\n"                                         \
+	"s_add_u32 %[nops], %[nops], 1\n"                                                  \
+	"v_cmp_neq_f64_e32 vcc, 0, v[96:97]\n"                                             \
+	"s_cbranch_vccz L_prune" X "_%=\n"                                                 \
+	"s_bitcmp0_b32 " X1 ", 4\n"                                                        \
+	"s_cbranch_scc1 L_adv" X "_%=\n"                                                   \
+	"s_lshr_b32 s89, " X1 ", %[cs]\n"                                                  \
+	"s_cmp_lg_u32 s89, %[chunk]\n"                                                     \
+	"s_cbranch_scc1 L_new" X "_%=\n"                                                   \
+	"s_bfe_u32 s89, " X1 ", %[cf]\n"                                                   \
+	"s_bitset1_b32 %[mask], s89\n"                                                     \
+	"s_lshl_b32 s89, s89, 9\n"                                                         \
+	"v_add_u32 v103, s89, %[bl]\n"                                                     \
+	"ds_write_b64 v103, v[96:97]\n"                                                    \
+	"L_adv" X "_%=:\n"                                                                 \
+	"s_add_u32 %[qb], %[qb], 16\n"                                                     \
+	"s_cmp_lt_u32 %[qb], %[qe]\n"                                                      \
+	"s_cbranch_scc1 L_top" Y "_%=\n"                                                   \
+	"s_branch L_end" Y "_%=\n"                                                         \
+	"L_prune" X "_%=:\n"                                                               \
+	"s_add_u32 %[nprunes], %[nprunes], 1\n"                                            \
+	"s_lshr_b32 s90, " X0 ", 8\n"                                                      \
+	"s_cmp_eq_u32 s90, 1\n"                                                            \
+	"s_cbranch_scc1 L_adv" X "_%=\n"                                                   \
+	"s_lshl_b32 s90, s90, 4\n"                                                         \
+	"s_add_u32 %[qb], %[qb], s90\n"                                                    \
+	"s_load_dwordx4 " YQ ", %[ob], %[qb]\n"                                            \
+	"s_waitcnt lgkmcnt(0)\n"                                                           \
+	"s_cmp_lt_u32 %[qb], %[qe]\n"                                                      \
+	"s_cbranch_scc1 L_top" Y "_%=\n"                                                   \
+	"s_branch L_end" Y "_%=\n"                                                         \
+	"L_new" X "_%=:\n"                                                                 \
+	"s_add_u32 %[qb], %[qb], 16\n"                                                     \
+	"s_lshr_b32 %[node], " X1 ", 8\n"                                                  \
+	"v_mov_b64_e32 %[val], v[96:97]\n"                                                 \
+	"s_branch L_exit" Y "_%=\n"
+
+#define HDDM_WALK_ASM                                                                  \
+	"v_mov_b32 v98, 0\n"                                                               \
+	"s_mov_b32 s94, 0\n"                                                               \
+	HDDM_WALK_STEP("A", "s76", "s77", "s78", "s79", "B", "s[80:83]")                   \
+	HDDM_WALK_STEP("B", "s80", "s81", "s82", "s83", "A", "s[76:79]")                   \
+	"L_endA_%=:\n"                                                                     \
+	"s_mov_b32 %[node], -1\n"                                                          \
+	"s_branch L_exitA_%=\n"                                                            \
+	"L_endB_%=:\n"                                                                     \
+	"s_mov_b32 %[node], -1\n"                                                          \
+	"L_exitB_%=:\n"                                                                    \
+	"s_mov_b64 s[76:77], s[80:81]\n"                                                   \
+	"s_mov_b64 s[78:79], s[82:83]\n"                                                   \
+	"L_exitA_%=:\n"                                                                    \
+	"s_waitcnt lgkmcnt(0)\n"
+
+#define HDDM_WALK_OPERANDS                                                             \
+	: [qb] "+s"(qb), [mask] "+s"(mask), [nops] "+s"(nops), [nprunes] "+s"(nprunes),    \
+	  [node] "=&s"(node), [val] "=&v"(val), "+{s[76:79]}"(opv)                          \
+	: [ob] "s"(ob), [qe] "s"(qe), [chunk] "s"(chunk), [pl] "v"(pl), [bl] "v"(bl),      \
+	  [cs] "i"(8 + CB), [cf] "i"((CB << 16) | 8), "{v[64:95]}"(xv)                     \
+	: "s80", "s81", "s82", "s83", "s88", "s89", "s90", "s94", "s95",                   \
+	  "v96", "v97", "v98", "v99", "v100", "v101", "v102", "v103", "vcc", "scc", "m0", "memory"
+
 template <int CB>
 __device__ __forceinline__ int walk_op_stream(const char* ob, uint32_t& qb, uint32_t qe, int chunk,
 	u4_t& opv, uint32_t& mask, uint32_t& nops, uint32_t& nprunes, uint32_t pl, uint32_t bl,
 	const d16_t& xv, double& val)
 {
 	int node;
-	asm volatile(
-		"v_mov_b32 v98, 0\n"
-		"s_mov_b32 s94, 0\n"
-		"L_top_%=:\n"
-		"s_lshr_b32 s90, s76, 4\n"                       // skip bytes = (dimw >> 8) * 16
-		"s_and_b32 s90, s90, 0x0ffffff0\n"
-		"s_add_u32 s91, %[qb], s90\n"
-		"s_load_dwordx4 s[80:83], %[ob], %[qb] offset:0x10\n"
-		"s_load_dwordx4 s[84:87], %[ob], s91\n"
-		"s_lshl_b32 s88, s77, 9\n"                       // t * 512
-		"s_and_b32 s88, s88, 0x1e00\n"
-		"v_add_u32 v102, s88, %[pl]\n"
-		"ds_read_b64 v[100:101], v102\n"                 // P[t]
-		"s_lshl_b32 s89, s76, 1\n"                       // 2 * (dim & 15)
-		"s_and_b32 s89, s89, 30\n"
-		"s_set_gpr_idx_on s89, gpr_idx(SRC0)\n"
-		"v_mov_b32 v96, v64\n"
-		"v_mov_b32 v97, v65\n"
-		"s_set_gpr_idx_off\n"
-		"s_mov_b32 s95, s78\n"
-		"v_mov_b32 v99, s79\n"
-		"v_fma_f64 v[96:97], v[96:97], s[94:95], v[98:99]\n"   // x * i - j
-		"v_add_f64 v[96:97], -|v[96:97]|, 1.0\n"
-		"v_max_f64 v[96:97], v[96:97], 0\n"
-		"s_waitcnt lgkmcnt(0)\n"
-		"v_mul_f64 v[96:97], v[100:101], v[96:97]\n"
-		"ds_write_b64 v102, v[96:97] offset:512\n"       // P[t+1]
-		"s_add_u32 %[nops], %[nops], 1\n"
-		"v_cmp_neq_f64_e32 vcc, 0, v[96:97]\n"
-		"s_cbranch_vccz L_prune_%=\n"
-		"s_bitcmp1_b32 s77, 4\n"
-		"s_cbranch_scc1 L_last_%=\n"
-		"L_adv_%=:\n"
-		"s_add_u32 %[qb], %[qb], 16\n"
-		"s_mov_b64 s[76:77], s[80:81]\n"
-		"s_mov_b64 s[78:79], s[82:83]\n"
-		"s_cmp_lt_u32 %[qb], %[qe]\n"
-		"s_cbranch_scc1 L_top_%=\n"
-		"s_branch L_end_%=\n"
-		"L_last_%=:\n"
-		"s_lshr_b32 s88, s77, 8\n"                       // node
-		"s_lshr_b32 s89, s88, %[cb]\n"
-		"s_cmp_lg_u32 s89, %[chunk]\n"
-		"s_cbranch_scc1 L_new_%=\n"
-		"s_and_b32 s89, s88, %[cm]\n"
-		"s_lshl_b32 s91, 1, s89\n"
-		"s_or_b32 %[mask], %[mask], s91\n"
-		"s_lshl_b32 s89, s89, 9\n"
-		"v_add_u32 v103, s89, %[bl]\n"
-		"ds_write_b64 v103, v[96:97]\n"
-		"s_branch L_adv_%=\n"
-		"L_prune_%=:\n"
-		"s_add_u32 %[nprunes], %[nprunes], 1\n"
-		"s_add_u32 %[qb], %[qb], s90\n"
-		"s_mov_b64 s[76:77], s[84:85]\n"
-		"s_mov_b64 s[78:79], s[86:87]\n"
-		"s_cmp_lt_u32 %[qb], %[qe]\n"
-		"s_cbranch_scc1 L_top_%=\n"
-		"s_branch L_end_%=\n"
-		"L_new_%=:\n"
-		"s_add_u32 %[qb], %[qb], 16\n"
-		"s_mov_b64 s[76:77], s[80:81]\n"
-		"s_mov_b64 s[78:79], s[82:83]\n"
-		"s_mov_b32 %[node], s88\n"
-		"v_mov_b64_e32 %[val], v[96:97]\n"
-		"s_branch L_exit_%=\n"
-		"L_end_%=:\n"
-		"s_mov_b32 %[node], -1\n"
-		"L_exit_%=:\n"
-		"s_waitcnt lgkmcnt(0)\n"
-		: [qb] "+s"(qb), [mask] "+s"(mask), [nops] "+s"(nops), [nprunes] "+s"(nprunes),
-		  [node] "=&s"(node), [val] "=&v"(val), "+{s[76:79]}"(opv)
-		: [ob] "s"(ob), [qe] "s"(qe), [chunk] "s"(chunk), [pl] "v"(pl), [bl] "v"(bl),
-		  [cb] "i"(CB), [cm] "i"((1 << CB) - 1), "{v[64:95]}"(xv)
-		: "s80", "s81", "s82", "s83", "s84", "s85", "s86", "s87", "s88", "s89", "s90", "s91",
-		  "s94", "s95", "v96", "v97", "v98", "v99", "v100", "v101", "v102", "v103",
-		  "vcc", "scc", "memory");
+	asm volatile(HDDM_WALK_ASM HDDM_WALK_OPERANDS);
 	return node;
 }
+
+#undef HDDM_WALK_OPERANDS
+#undef HDDM_WALK_ASM
+#undef HDDM_WALK_STEP
 
 // Walk state of one wave. P[t] (the product of the first t factors of the
 // current trie path) lives in the wave's LDS stage; the two most recent
@@ -503,8 +515,8 @@ struct Walker
 #pragma unroll
 			for (int i = 0; i < 16; i++) xv[i] = xb0[i];
 			double v = 0.0;
-			const int n = walk_op_stream<CB>((const char*)gops, qb, qe, chunk, opv, mask, nops, nprunes,
-				lds_offset(P) + (uint32_t)lane * 8u, lds_offset(B) + (uint32_t)lane * 8u, xv, v);
+			const uint32_t pl = lds_offset(P) + (uint32_t)lane * 8u, bl = lds_offset(B) + (uint32_t)lane * 8u;
+			const int n = walk_op_stream<CB>((const char*)gops, qb, qe, chunk, opv, mask, nops, nprunes, pl, bl, xv, v);
 			op.dim = opv[0]; op.ctrl = opv[1]; op.ihi = opv[2]; op.njhi = opv[3];
 			q = (int)(qb / (uint32_t)sizeof(Op));
 			if (n < 0) return false;
@@ -537,7 +549,7 @@ struct Walker
 					// paths do not wait for the load above
 					__builtin_amdgcn_s_waitcnt(0xc07f);
 				}
-				const double tt = fma(xget<XB, XSEL>(xb0, xb1, xb2, xb3, dimw & kOpDimMask),
+				const double tt = fma(xget<XB, XSEL>(xb0, xb1, xb2, xb3, op_dim(dimw)),
 					f64_from_high(op.ihi), f64_from_high(op.njhi));
 				const double p = bt * fmax(1.0 - fabs(tt), 0.0);
 				base_prev = bt;
@@ -573,7 +585,7 @@ struct Walker
 			{
 				const int sl = q - tb;
 				const u4_t ov = *(const u4_t*)(tops + sl);
-				const uint32_t dim = __builtin_amdgcn_readfirstlane(ov[0]) & kOpDimMask;
+				const uint32_t dim = op_dim(__builtin_amdgcn_readfirstlane(ov[0]));
 				const uint32_t ctrl = __builtin_amdgcn_readfirstlane(ov[1]);
 				const uint32_t ihi = __builtin_amdgcn_readfirstlane(ov[2]);
 				const uint32_t njhi = __builtin_amdgcn_readfirstlane(ov[3]);

@@ -18,7 +18,9 @@ constexpr int kMortonBits = 40;
 // One factor evaluation of the prefix-trie walk ("op"), 16 bytes, read by
 // one scalar load. The ops of all nodes are stored in trie order; node n
 // contributes one op per factor it does not share with node n-1.
-//   dim    dimension of x the factor reads (bits 0-7) | (skip[op] - op) << 8
+//   dim    twice the dimension of x the factor reads (bits 0-7; the index of
+//          its first 32-bit register in a bank of FP64 coordinates)
+//          | (skip[op] - op) << 8
 //   ctrl   depth t (bits 0-3) | last factor of its node (bit 4)
 //          | relation to the previous op in the stream (bits 5-6)
 //          | the op has children (bit 7) | node index << 8
@@ -46,6 +48,9 @@ constexpr int kOpNodeShift = 8;
 constexpr uint32_t kOpDimMask = 0xffu;
 constexpr int kOpSkipShift = 8;
 constexpr int kOpPad = 4;
+
+// Dimension of x read by an op with dim word w (constexpr: host and device).
+constexpr uint32_t op_dim(uint32_t w) { return (w & kOpDimMask) >> 1; }
 
 struct StateDesc
 {

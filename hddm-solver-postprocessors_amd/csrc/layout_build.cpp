@@ -196,7 +196,7 @@ int build_layout(int dim, int nno, int dofs,
 		for (int t = S[n]; t < F[n]; t++, q++)
 		{
 			Op op;
-			op.dim = (uint32_t)cur[t].d;
+			op.dim = (uint32_t)cur[t].d << 1;
 			op.ctrl = (uint32_t)t | (t == F[n] - 1 ? kOpLast : 0u) | ((uint32_t)n << kOpNodeShift);
 			f64_high_word(cur[t].i, op.ihi);
 			f64_high_word(-(int64_t)cur[t].j, op.njhi);

@@ -230,7 +230,7 @@ class Layout:
         self.skip = np.zeros(self.nops, np.uint32)
         self.opstart = np.zeros(nno + 1, np.int32)
         _check(lib.hddm_lb_layout_ops(h, _ptr(ops), _ptr(self.skip), _ptr(self.opstart)))
-        self.op_dim = (ops[:, 0] & 0xff).astype(np.int64)
+        self.op_dim = ((ops[:, 0] & 0xff) >> 1).astype(np.int64)
         self.op_relskip = (ops[:, 0] >> 8).astype(np.int64)
         ctrl = ops[:, 1]
         self.op_depth = (ctrl & 15).astype(np.int64)
