@@ -333,11 +333,13 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 // node into the chunk buffer at bl.
 // The step is written twice, for the op in s[76:79] (successor into
 // s[80:83]) and for the op in s[80:83] (successor into s[76:79]), so no op
-// is copied between registers; the op handed back is in s[76:79].
+// is copied between registers; the op handed back is in s[76:79]. The
+// second step follows the first in the code, so the common path takes one
+// branch per two ops; prunes and node emission into a new chunk branch out.
 // Fixed registers: the FP64 i as s[94:95] (low word 0), x in v[64:95],
 // temporaries s88-s90, v[96:103] (v98 = 0 is the low word of -j); the
 // indexed moves set m0.
-#define HDDM_WALK_STEP(X, X0, X1, X2, X3, Y, YQ)                                       \
+#define HDDM_WALK_MAIN(X, X0, X1, X2, X3, Y, YQ, NEXT)                                \
 	"L_top" X "_%=:\n"                                                                 \
 	"s_load_dwordx4 " YQ ", %[ob], %[qb] offset:0x10\n"                                \
 	"s_lshl_b32 s88, " X1 ", 9\n"                                                      \
@@ -372,8 +374,13 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 	"L_adv" X "_%=:\n"                                                                 \
 	"s_add_u32 %[qb], %[qb], 16\n"                                                     \
 	"s_cmp_lt_u32 %[qb], %[qe]\n"                                                      \
-	"s_cbranch_scc1 L_top" Y "_%=\n"                                                   \
-	"s_branch L_end" Y "_%=\n"                                                         \
+	NEXT
+
+// continue with step Y: falls through when Y's step follows, else branches back
+#define HDDM_WALK_FALL(Y) "s_cbranch_scc0 L_end" Y "_%=\n"
+#define HDDM_WALK_LOOP(Y) "s_cbranch_scc1 L_top" Y "_%=\n" "s_branch L_end" Y "_%=\n"
+
+#define HDDM_WALK_SIDE(X, X0, X1, Y, YQ)                                               \
 	"L_prune" X "_%=:\n"                                                               \
 	"s_add_u32 %[nprunes], %[nprunes], 1\n"                                            \
 	"s_lshr_b32 s90, " X0 ", 8\n"                                                      \
@@ -395,8 +402,10 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 #define HDDM_WALK_ASM                                                                  \
 	"v_mov_b32 v98, 0\n"                                                               \
 	"s_mov_b32 s94, 0\n"                                                               \
-	HDDM_WALK_STEP("A", "s76", "s77", "s78", "s79", "B", "s[80:83]")                   \
-	HDDM_WALK_STEP("B", "s80", "s81", "s82", "s83", "A", "s[76:79]")                   \
+	HDDM_WALK_MAIN("A", "s76", "s77", "s78", "s79", "B", "s[80:83]", HDDM_WALK_FALL("B")) \
+	HDDM_WALK_MAIN("B", "s80", "s81", "s82", "s83", "A", "s[76:79]", HDDM_WALK_LOOP("A")) \
+	HDDM_WALK_SIDE("A", "s76", "s77", "B", "s[80:83]")                                 \
+	HDDM_WALK_SIDE("B", "s80", "s81", "A", "s[76:79]")                                 \
 	"L_endA_%=:\n"                                                                     \
 	"s_mov_b32 %[node], -1\n"                                                          \
 	"s_branch L_exitA_%=\n"                                                            \
@@ -428,7 +437,10 @@ __device__ __forceinline__ int walk_op_stream(const char* ob, uint32_t& qb, uint
 
 #undef HDDM_WALK_OPERANDS
 #undef HDDM_WALK_ASM
-#undef HDDM_WALK_STEP
+#undef HDDM_WALK_MAIN
+#undef HDDM_WALK_SIDE
+#undef HDDM_WALK_FALL
+#undef HDDM_WALK_LOOP
 
 // Walk state of one wave. P[t] (the product of the first t factors of the
 // current trie path) lives in the wave's LDS stage; the two most recent
