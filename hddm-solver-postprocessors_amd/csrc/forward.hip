@@ -323,25 +323,26 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 // (D <= 16). Walks ops from byte offset qb until the first surviving node of
 // a chunk other than `chunk` (returned in node and val, not stored; qb is
 // then past its op) or until qb reaches qe (node = -1).
-// Per op: the successor is loaded at the top; the parent product comes from
-// the LDS path stack (P[t], lane-strided rows of 512 bytes at pl) and the
-// op's product is stored as P[t+1], so P[k] is the product of the first k
-// factors of the current trie path; the coordinate is read from the bank
-// with an indexed move whose index is the op's dim field (twice the
-// dimension); a wave-wide zero product skips the op's run (the skip target
-// is loaded only when it is not the successor); a live last factor emits its
+// Per op: the parent product comes from the LDS path stack (P[t],
+// lane-strided rows of 512 bytes at pl) and the op's product is stored as
+// P[t+1], so P[k] is the product of the first k factors of the current trie
+// path; the coordinate is read from the bank with an indexed move whose
+// index is the op's dim field (twice the dimension). A wave-wide zero
+// product skips the op's run (the skip target and its successor are loaded
+// only when the target is not the successor); a live last factor emits its
 // node into the chunk buffer at bl.
-// The step is written twice, for the op in s[76:79] (successor into
-// s[80:83]) and for the op in s[80:83] (successor into s[76:79]), so no op
-// is copied between registers; the op handed back is in s[76:79]. The
-// second step follows the first in the code, so the common path takes one
-// branch per two ops; prunes and node emission into a new chunk branch out.
+// Ops rotate through three SGPR sets s[76:79], s[80:83], s[84:87]: while op
+// q is evaluated, op q+1 is already in flight (issued one step earlier) and
+// op q+2 is issued right after the step's one wait, so a scalar-cache miss
+// has a whole step to return. The three steps follow each other in the
+// code: the common path takes one branch per three ops, prunes and the
+// emission of a node of a new chunk branch out. The op handed back is in
+// s[76:79].
 // Fixed registers: the FP64 i as s[94:95] (low word 0), x in v[64:95],
 // temporaries s88-s90, v[96:103] (v98 = 0 is the low word of -j); the
 // indexed moves set m0.
-#define HDDM_WALK_MAIN(X, X0, X1, X2, X3, Y, YQ, NEXT)                                \
+#define HDDM_WALK_MAIN(X, X0, X1, X2, X3, ZQ, NEXT)                                    \
 	"L_top" X "_%=:\n"                                                                 \
-	"s_load_dwordx4 " YQ ", %[ob], %[qb] offset:0x10\n"                                \
 	"s_lshl_b32 s88, " X1 ", 9\n"                                                      \
 	"s_and_b32 s88, s88, 0x1e00\n"                                                     \
 	"v_add_u32 v102, s88, %[pl]\n"                                                     \
@@ -356,6 +357,7 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 	"v_add_f64 v[96:97], -|v[96:97]|, 1.0\n"                                           \
 	"v_max_f64 v[96:97], v[96:97], 0\n"                                                \
 	"s_waitcnt lgkmcnt(0)\n"                                                           \
+	"s_load_dwordx4 " ZQ ", %[ob], %[qb] offset:0x20\n"                                \
 	"v_mul_f64 v[96:97], v[100:101], v[96:97]\n"                                       \
 	"ds_write_b64 v102, v[96:97] offset:512\n"                                         \
 	"s_add_u32 %[nops], %[nops], 1\n"                                                  \
@@ -380,7 +382,7 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 #define HDDM_WALK_FALL(Y) "s_cbranch_scc0 L_end" Y "_%=\n"
 #define HDDM_WALK_LOOP(Y) "s_cbranch_scc1 L_top" Y "_%=\n" "s_branch L_end" Y "_%=\n"
 
-#define HDDM_WALK_SIDE(X, X0, X1, Y, YQ)                                               \
+#define HDDM_WALK_SIDE(X, X0, X1, Y, YQ, ZQ)                                           \
 	"L_prune" X "_%=:\n"                                                               \
 	"s_add_u32 %[nprunes], %[nprunes], 1\n"                                            \
 	"s_lshr_b32 s90, " X0 ", 8\n"                                                      \
@@ -388,7 +390,9 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 	"s_cbranch_scc1 L_adv" X "_%=\n"                                                   \
 	"s_lshl_b32 s90, s90, 4\n"                                                         \
 	"s_add_u32 %[qb], %[qb], s90\n"                                                    \
+	"s_waitcnt lgkmcnt(0)\n"                                                           \
 	"s_load_dwordx4 " YQ ", %[ob], %[qb]\n"                                            \
+	"s_load_dwordx4 " ZQ ", %[ob], %[qb] offset:0x10\n"                                \
 	"s_waitcnt lgkmcnt(0)\n"                                                           \
 	"s_cmp_lt_u32 %[qb], %[qe]\n"                                                      \
 	"s_cbranch_scc1 L_top" Y "_%=\n"                                                   \
@@ -402,28 +406,41 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 #define HDDM_WALK_ASM                                                                  \
 	"v_mov_b32 v98, 0\n"                                                               \
 	"s_mov_b32 s94, 0\n"                                                               \
-	HDDM_WALK_MAIN("A", "s76", "s77", "s78", "s79", "B", "s[80:83]", HDDM_WALK_FALL("B")) \
-	HDDM_WALK_MAIN("B", "s80", "s81", "s82", "s83", "A", "s[76:79]", HDDM_WALK_LOOP("A")) \
-	HDDM_WALK_SIDE("A", "s76", "s77", "B", "s[80:83]")                                 \
-	HDDM_WALK_SIDE("B", "s80", "s81", "A", "s[76:79]")                                 \
-	"L_endA_%=:\n"                                                                     \
+	"s_load_dwordx4 s[80:83], %[ob], %[qb] offset:0x10\n"                              \
+	HDDM_WALK_MAIN("0", "s76", "s77", "s78", "s79", "s[84:87]", HDDM_WALK_FALL("1"))   \
+	HDDM_WALK_MAIN("1", "s80", "s81", "s82", "s83", "s[76:79]", HDDM_WALK_FALL("2"))   \
+	HDDM_WALK_MAIN("2", "s84", "s85", "s86", "s87", "s[80:83]", HDDM_WALK_LOOP("0"))   \
+	HDDM_WALK_SIDE("0", "s76", "s77", "1", "s[80:83]", "s[84:87]")                     \
+	HDDM_WALK_SIDE("1", "s80", "s81", "2", "s[84:87]", "s[76:79]")                     \
+	HDDM_WALK_SIDE("2", "s84", "s85", "0", "s[76:79]", "s[80:83]")                     \
+	"L_end0_%=:\n"                                                                     \
 	"s_mov_b32 %[node], -1\n"                                                          \
-	"s_branch L_exitA_%=\n"                                                            \
-	"L_endB_%=:\n"                                                                     \
+	"L_exit0_%=:\n"                                                                    \
+	"s_waitcnt lgkmcnt(0)\n"                                                           \
+	"s_branch L_done_%=\n"                                                             \
+	"L_end1_%=:\n"                                                                     \
 	"s_mov_b32 %[node], -1\n"                                                          \
-	"L_exitB_%=:\n"                                                                    \
+	"L_exit1_%=:\n"                                                                    \
+	"s_waitcnt lgkmcnt(0)\n"                                                           \
 	"s_mov_b64 s[76:77], s[80:81]\n"                                                   \
 	"s_mov_b64 s[78:79], s[82:83]\n"                                                   \
-	"L_exitA_%=:\n"                                                                    \
-	"s_waitcnt lgkmcnt(0)\n"
+	"s_branch L_done_%=\n"                                                             \
+	"L_end2_%=:\n"                                                                     \
+	"s_mov_b32 %[node], -1\n"                                                          \
+	"L_exit2_%=:\n"                                                                    \
+	"s_waitcnt lgkmcnt(0)\n"                                                           \
+	"s_mov_b64 s[76:77], s[84:85]\n"                                                   \
+	"s_mov_b64 s[78:79], s[86:87]\n"                                                   \
+	"L_done_%=:\n"
 
 #define HDDM_WALK_OPERANDS                                                             \
 	: [qb] "+s"(qb), [mask] "+s"(mask), [nops] "+s"(nops), [nprunes] "+s"(nprunes),    \
 	  [node] "=&s"(node), [val] "=&v"(val), "+{s[76:79]}"(opv)                          \
 	: [ob] "s"(ob), [qe] "s"(qe), [chunk] "s"(chunk), [pl] "v"(pl), [bl] "v"(bl),      \
 	  [cs] "i"(8 + CB), [cf] "i"((CB << 16) | 8), "{v[64:95]}"(xv)                     \
-	: "s80", "s81", "s82", "s83", "s88", "s89", "s90", "s94", "s95",                   \
-	  "v96", "v97", "v98", "v99", "v100", "v101", "v102", "v103", "vcc", "scc", "m0", "memory"
+	: "s80", "s81", "s82", "s83", "s84", "s85", "s86", "s87", "s88", "s89", "s90",     \
+	  "s94", "s95", "v96", "v97", "v98", "v99", "v100", "v101", "v102", "v103",        \
+	  "vcc", "scc", "m0", "memory"
 
 template <int CB>
 __device__ __forceinline__ int walk_op_stream(const char* ob, uint32_t& qb, uint32_t qe, int chunk,
