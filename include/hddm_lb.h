@@ -153,17 +153,18 @@ typedef struct hddm_lb_layout hddm_lb_layout;
 /* Builds the layout hddm_lb_upload_state would upload: nodes in prefix-trie
  * order (lexicographic by their ascending-dim factor sequences) and the op
  * stream of the walk, one op per factor a node does not share with its
- * predecessor. Op q is 4 words: dim, ctrl = depth | (last factor of its node)
- * << 4 | node << 5, and the high words of the FP64 values of i and -j (low
- * words 0); skip[q] is the first op after the run of nodes sharing op q's
- * prefix; opstart[n] (n <= nno) is node n's first op. */
+ * predecessor. Op q is 4 words: 2 * dim | (skip[q] - q) << 8; ctrl = depth
+ * | (last factor of its node) << 4 | (relation to op q-1: 0 first child,
+ * 1 sibling, 2 other) << 5 | (op has children) << 7 | node << 8; and the high
+ * words of the FP64 values of i and -j (low words 0). skip[q] is the first op
+ * after the run of nodes sharing op q's prefix; opstart[n] (n <= nno) is node
+ * n's first op. */
 int hddm_lb_layout_build(int dim, int nno, int dofs,
 	const int32_t* index_i, const int32_t* index_j, int64_t ld_index,
 	const double* surplus, int64_t ld_surplus, hddm_lb_layout** layout);
 int hddm_lb_layout_info(const hddm_lb_layout* layout, int* max_factors, int* nops,
 	int* nchunks, int* dofs_pad);
-/* ops: 4 * nops words per op {dim | leaf run << 8, depth | last << 4 | node << 5,
- * high word of FP64 i, high word of FP64 -j}; skip: nops; opstart: nno + 1 */
+/* ops: 4 words per op as above (4 * nops); skip: nops; opstart: nno + 1 */
 int hddm_lb_layout_ops(const hddm_lb_layout* layout, uint32_t* ops, uint32_t* skip, int32_t* opstart);
 /* order[n]: uploaded index of the node at trie position n */
 int hddm_lb_layout_order(const hddm_lb_layout* layout, int32_t* order);
