@@ -927,8 +927,12 @@ __global__ void reset_counter_kernel(PlanHeader* hdr)
 	if (threadIdx.x == 0) hdr->counter = 0;
 }
 
-__global__ void reduce_partials_kernel(const StateDesc* __restrict__ states, const StatePlan* __restrict__ plans,
-	const uint32_t* __restrict__ perm, const double* __restrict__ partial, double* __restrict__ y, int64_t ldy)
+// Sums the node-segment partials of every point in segment order. Block
+// (32, 8): threadIdx.x walks the dofs, threadIdx.y picks one of 8 points;
+// blocks stride over the state's points. No 64-bit division per element.
+__global__ void __launch_bounds__(256) reduce_partials_kernel(const StateDesc* __restrict__ states,
+	const StatePlan* __restrict__ plans, const uint32_t* __restrict__ perm, const double* __restrict__ partial,
+	double* __restrict__ y, int64_t ldy)
 {
 	const int s = blockIdx.y;
 	const int nseg = states[s].nseg;
@@ -936,14 +940,17 @@ __global__ void reduce_partials_kernel(const StateDesc* __restrict__ states, con
 	const int dofs = states[s].dofs, DP = states[s].dofs_pad;
 	const int64_t count = plans[s].count, start = plans[s].start;
 	const double* base = partial + plans[s].partial_base;
-	const int64_t total = count * dofs;
-	for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x)
+	const int64_t gstride = count * DP;
+	for (int64_t lp = (int64_t)blockIdx.x * 8 + threadIdx.y; lp < count; lp += (int64_t)gridDim.x * 8)
 	{
-		const int64_t lp = e / dofs;
-		const int c = (int)(e - lp * dofs);
-		double sum = 0.0;
-		for (int g = 0; g < nseg; g++) sum += base[((int64_t)g * count + lp) * DP + c];
-		y[(int64_t)perm[start + lp] * ldy + c] = sum;
+		const uint32_t orow = perm[start + lp];
+		const double* src = base + lp * DP;
+		for (int c = threadIdx.x; c < dofs; c += 32)
+		{
+			double sum = 0.0;
+			for (int g = 0; g < nseg; g++) sum += src[g * gstride + c];
+			y[(int64_t)orow * ldy + c] = sum;
+		}
 	}
 }
 
@@ -1045,7 +1052,7 @@ hipError_t launch_sweep(const void* kern, int grid, int waves, size_t lds, const
 hipError_t launch_reduce(const StateDesc* states, const StatePlan* plans, const uint32_t* perm,
 	const double* partial, double* y, int64_t ldy, int nstates, hipStream_t st)
 {
-	hipLaunchKernelGGL(reduce_partials_kernel, dim3(512, nstates), dim3(256), 0, st, states, plans, perm, partial, y, ldy);
+	hipLaunchKernelGGL(reduce_partials_kernel, dim3(2048, nstates), dim3(32, 8), 0, st, states, plans, perm, partial, y, ldy);
 	return hipGetLastError();
 }
 
