@@ -17,6 +17,7 @@
 // contracted per 16-node chunk with v_mfma_f64_16x16x4f64 (dofs x nodes,
 // K = 4 points per step) and added to the output with FP64 atomics.
 #include "sweep_common.h"
+#include "walk.h"
 #include "kernels.h"
 #include "context.h"
 #include "errors.h"
@@ -47,14 +48,19 @@ struct AdjointArgs
 	int dt_base;       // dof slab of 32 handled by this launch
 };
 
-// Per-wave LDS of the adjoint: P, Q (phi'/phi), R [PM][64]; dx [16 XB][64];
-// chunk basis [16][64] (doubles); dims of the path [PM] (uint32).
-template <int XB, int PM>
+// Per-wave LDS of the adjoint walk: P, Q (phi'/phi), R [PM][64]; dx
+// [16 XB][64]; with dS, the chunk basis [16][64] (doubles); dims of the path
+// [PM] (uint32).
+template <int XB, int PM, bool DS>
 struct AdjLds
 {
-	static constexpr size_t kDoubles = (size_t)(3 * PM + 16 * XB + kChunk) * kWave;
+	static constexpr size_t kDoubles = (size_t)(3 * PM + 16 * XB + (DS ? kChunk : 0)) * kWave;
 	static constexpr size_t kBytes = kDoubles * 8 + ((PM * 4 + 15) / 16) * 16;
 };
+
+// Per-wave LDS of the dS-only sweep: path stack [PM][64] and chunk basis [16][64].
+template <int PM>
+constexpr size_t ds_stage_bytes() { return (size_t)(PM + kChunk) * kWave * 8; }
 
 namespace {
 
@@ -102,7 +108,8 @@ __device__ __forceinline__ void contract_ds(const AdjointArgs& a, const StateDes
 
 } // namespace
 
-template <int XB, int PM>
+// dx (and, with DS, dS) along the edge-sum walk.
+template <int XB, int PM, bool DS>
 __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 {
 	// Waves are independent: each pops its own items (one 64-point block
@@ -112,12 +119,12 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 	const int wave = uniform((int)(threadIdx.x >> 6));
 	constexpr int DXR = 16 * XB;
 	char* const lbase = (char*)lds;
-	double* const P = (double*)(lbase + (size_t)wave * AdjLds<XB, PM>::kBytes);
+	double* const P = (double*)(lbase + (size_t)wave * AdjLds<XB, PM, DS>::kBytes);
 	double* const Q = P + PM * kWave;
 	double* const R = Q + PM * kWave;
 	double* const DX = R + PM * kWave;
 	double* const Bc = DX + DXR * kWave;
-	uint32_t* const dims = (uint32_t*)(Bc + kChunk * kWave);
+	uint32_t* const dims = (uint32_t*)(Bc + (DS ? kChunk : 0) * kWave);
 	const int64_t nitems = ldc(&a.hdr->nitems);
 
 	for (;;)
@@ -161,21 +168,24 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 		// points whose gradient rows feed the dS operands of this lane
 		uint32_t og[16];
 		uint32_t ogvalid = 0;
-#pragma unroll
-		for (int s = 0; s < 16; s++)
+		if constexpr (DS)
 		{
-			const int pt = 4 * s + (lane >> 4);
-			og[s] = (uint32_t)__shfl((int)orig, pt, 64);
-			if (pt < np) ogvalid |= 1u << s;
+#pragma unroll
+			for (int s = 0; s < 16; s++)
+			{
+				const int pt = 4 * s + (lane >> 4);
+				og[s] = (uint32_t)__shfl((int)orig, pt, 64);
+				if (pt < np) ogvalid |= 1u << s;
+			}
+#pragma unroll
+			for (int k = 0; k < kChunk; k++) Bc[k * kWave + lane] = 0.0;
 		}
 #pragma unroll
 		for (int d = 0; d < DXR; d++) DX[d * kWave + lane] = 0.0;
-#pragma unroll
-		for (int k = 0; k < kChunk; k++) Bc[k * kWave + lane] = 0.0;
 		P[lane] = 1.0;
 
 		const double* const s64 = ldc(&sd->s64) + 32 * dt0;
-		const bool want_ds = a.ds != nullptr;
+		constexpr bool want_ds = DS;
 		int q = q0;
 		int top = -1;          // deepest live edge of the path
 		double cur = 1.0, base_prev = 1.0;
@@ -246,7 +256,7 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 #pragma unroll
 				for (int c = 0; c < 32; c++) w = fma(g[c], ldc(srow + c), w);
 				R[t * kWave + lane] = w * p;
-				if (want_ds)
+				if constexpr (want_ds)
 				{
 					if ((n >> 4) != chunk)
 					{
@@ -264,7 +274,8 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 				}
 			}
 		}
-		if (want_ds && chunk >= 0) contract_ds(a, sd, Bc, mask, chunk, lane, og, ogvalid, dofs);
+		if constexpr (want_ds)
+			if (chunk >= 0) contract_ds(a, sd, Bc, mask, chunk, lane, og, ogvalid, dofs);
 		for (int tt = top; tt >= 0; tt--)
 		{
 			const double r = R[tt * kWave + lane];
@@ -286,17 +297,104 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 	}
 }
 
+// dS alone: the forward's walk (nodes emitted into 16-node chunks of the
+// wave's LDS stage) with the dS contraction per finished chunk; no dx state,
+// so it needs far fewer registers and LDS than the edge-sum walk.
 template <int XB, int PM>
-static const void* adjoint_ptr() { return (const void*)&adjoint_kernel<XB, PM>; }
+__global__ void __launch_bounds__(256) adjoint_ds_kernel(AdjointArgs a)
+{
+	extern __shared__ __attribute__((aligned(16))) double lds[];
+	const int lane = threadIdx.x & 63;
+	const int wave = uniform((int)(threadIdx.x >> 6));
+	double* const Pl = (double*)((char*)lds + (size_t)wave * ds_stage_bytes<PM>());
+	double* const Bl = Pl + PM * kWave;
+	const int64_t nitems = ldc(&a.hdr->nitems);
 
-static const void* adjoint_kernel_ptr(int xbanks, int depth, size_t& per_wave)
+	for (;;)
+	{
+		uint64_t iv0 = 0;
+		if (lane == 0) iv0 = atomicAdd(&a.hdr->counter, 1ull);
+		const int64_t item = (int64_t)uniform64(iv0);
+		if (item >= nitems) break;
+		ItemInfo it;
+		decode_item(a, item, 1, it);
+		const StateDesc* sd = a.states + it.s;
+		const int dofs = ldc(&sd->dofs);
+		const Op* const gops = ldc(&sd->ops);
+		const uint32_t* const gskip = ldc(&sd->skip);
+		const int32_t* const seg_op = ldc(&sd->seg_op);
+		const int q0 = ldc(seg_op + it.seg), q1 = ldc(seg_op + it.seg + 1);
+		if (32 * a.dt_base >= dofs) continue;
+		const int64_t rem = it.count - it.grp * kWave;
+		const int np = rem <= 0 ? 0 : (rem < kWave ? (int)rem : kWave);
+		if (np == 0) continue;
+		const int myp = lane < np ? lane : np - 1;
+		const uint32_t orig = a.perm[it.start + it.grp * kWave + myp];
+		const double* xrow = a.x + (int64_t)orig * a.ldx;
+		double xb0[16], xb1[16], xb2[16], xb3[16];
+#pragma unroll
+		for (int d = 0; d < 16; d++)
+		{
+			xb0[d] = xrow[d < a.dim ? d : 0];
+			xb1[d] = XB > 1 ? xrow[16 + d < a.dim ? 16 + d : 0] : 0.0;
+			xb2[d] = XB > 2 ? xrow[32 + d < a.dim ? 32 + d : 0] : 0.0;
+			xb3[d] = XB > 2 ? xrow[48 + d < a.dim ? 48 + d : 0] : 0.0;
+		}
+		uint32_t og[16];
+		uint32_t ogvalid = 0;
+#pragma unroll
+		for (int s = 0; s < 16; s++)
+		{
+			const int pt = 4 * s + (lane >> 4);
+			og[s] = (uint32_t)__shfl((int)orig, pt, 64);
+			if (pt < np) ogvalid |= 1u << s;
+		}
+#pragma unroll
+		for (int k = 0; k < kChunk; k++) Bl[k * kWave + lane] = 0.0;
+
+		Walker<XB, double, kWalkSmem, PM, true, 4> w;
+		w.init(q0, q1, Pl, Bl, lane, gops, gskip);
+		for (;;)
+		{
+			int node = 0;
+			double val = 0.0;
+			if (!w.run(nullptr, nullptr, q0, q1, lane, xb0, xb1, xb2, xb3, node, val)) break;
+			if (w.chunk >= 0)
+			{
+				contract_ds(a, sd, Bl, w.mask, w.chunk, lane, og, ogvalid, dofs);
+#pragma unroll
+				for (int k = 0; k < kChunk; k++) Bl[k * kWave + lane] = 0.0;
+			}
+			w.start_chunk(node, val, lane);
+		}
+		if (w.chunk >= 0)
+		{
+			contract_ds(a, sd, Bl, w.mask, w.chunk, lane, og, ogvalid, dofs);
+#pragma unroll
+			for (int k = 0; k < kChunk; k++) Bl[k * kWave + lane] = 0.0;
+		}
+	}
+}
+
+enum AdjMode { kAdjDx = 0, kAdjBoth = 1, kAdjDs = 2 };
+
+template <int XB, int PM>
+static const void* adjoint_ptr(int mode, size_t& per_wave)
+{
+	if (mode == kAdjDs) { per_wave = ds_stage_bytes<PM>(); return (const void*)&adjoint_ds_kernel<XB, PM>; }
+	if (mode == kAdjBoth) { per_wave = AdjLds<XB, PM, true>::kBytes; return (const void*)&adjoint_kernel<XB, PM, true>; }
+	per_wave = AdjLds<XB, PM, false>::kBytes;
+	return (const void*)&adjoint_kernel<XB, PM, false>;
+}
+
+static const void* adjoint_kernel_ptr(int xbanks, int depth, int mode, size_t& per_wave)
 {
 	const bool deep = depth >= 8;
 	switch (xbanks)
 	{
-	case 1: per_wave = deep ? AdjLds<1, 16>::kBytes : AdjLds<1, 8>::kBytes; return deep ? adjoint_ptr<1, 16>() : adjoint_ptr<1, 8>();
-	case 2: per_wave = deep ? AdjLds<2, 16>::kBytes : AdjLds<2, 8>::kBytes; return deep ? adjoint_ptr<2, 16>() : adjoint_ptr<2, 8>();
-	case 3: case 4: per_wave = deep ? AdjLds<4, 16>::kBytes : AdjLds<4, 8>::kBytes; return deep ? adjoint_ptr<4, 16>() : adjoint_ptr<4, 8>();
+	case 1: return deep ? adjoint_ptr<1, 16>(mode, per_wave) : adjoint_ptr<1, 8>(mode, per_wave);
+	case 2: return deep ? adjoint_ptr<2, 16>(mode, per_wave) : adjoint_ptr<2, 8>(mode, per_wave);
+	case 3: case 4: return deep ? adjoint_ptr<4, 16>(mode, per_wave) : adjoint_ptr<4, 8>(mode, per_wave);
 	}
 	return nullptr;
 }
@@ -336,18 +434,28 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 	if (dx && npoints) HIP_TRY(launch_fill_u32((uint32_t*)dx, 2 * (npoints - 1) * lddx + 2 * ctx->dim, 0u, st));
 	if (npoints == 0 || (!dx && !dsurplus)) return HDDM_LB_SUCCESS;
 
-	size_t per_wave = 0;
-	const void* kern = adjoint_kernel_ptr(xbanks_for(ctx->dim), ctx->fmax_all, per_wave);
-	if (!kern) return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: dim %d is not supported", ctx->dim);
-	int G = (int)std::min<size_t>(4, ctx->max_lds / per_wave);
-	if (ctx->opt_waves) G = std::min(G, ctx->opt_waves);
-	if (G < 1) return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: LDS of one wave (%zu B) exceeds the CU", per_wave);
-	const size_t lds = (size_t)G * per_wave;
-	HIP_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-	int per_cu = 0;
-	HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, G * kWave, lds));
-	if (per_cu < 1) return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: kernel does not fit a CU");
-	const int grid = per_cu * ctx->num_cus;
+	// dx and dS together: the edge-sum walk for dx, then the forward's walk
+	// for dS (two lighter kernels run faster than one that carries both)
+	int modes[2] = { dx ? kAdjDx : kAdjDs, -1 };
+	if (dx && dsurplus) modes[1] = kAdjDs;
+	const void* kerns[2] = { nullptr, nullptr };
+	int grids[2] = { 0, 0 }, groups[2] = { 0, 0 };
+	size_t ldss[2] = { 0, 0 };
+	for (int m = 0; m < 2 && modes[m] >= 0; m++)
+	{
+		size_t per_wave = 0;
+		const void* kern = adjoint_kernel_ptr(xbanks_for(ctx->dim), ctx->fmax_all, modes[m], per_wave);
+		if (!kern) return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: dim %d is not supported", ctx->dim);
+		int G = (int)std::min<size_t>(4, ctx->max_lds / per_wave);
+		if (ctx->opt_waves) G = std::min(G, ctx->opt_waves);
+		if (G < 1) return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: LDS of one wave (%zu B) exceeds the CU", per_wave);
+		const size_t lds = (size_t)G * per_wave;
+		HIP_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+		int per_cu = 0;
+		HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, G * kWave, lds));
+		if (per_cu < 1) return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: kernel does not fit a CU");
+		kerns[m] = kern; grids[m] = per_cu * ctx->num_cus; groups[m] = G; ldss[m] = lds;
+	}
 
 	Scratch& sc = ctx->scratch;
 	const int ns = ctx->tables_n;
@@ -363,12 +471,15 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 	a.states = ctx->d_states; a.plans = sc.plans; a.cost_order = ctx->d_cost_order; a.hdr = sc.hdr;
 	a.perm = sc.perm; a.x = x; a.ldx = ldx; a.g = gbar; a.ldg = ldg; a.dx = dx; a.lddx = lddx;
 	a.ds = dsurplus; a.nstates = ns; a.dim = ctx->dim;
-	for (int sl = 0; sl < ctx->dt_needed; sl++)
-	{
-		a.dt_base = sl;
-		if (sl > 0) HIP_TRY(launch_reset_counter(sc.hdr, st));
-		void* params[] = { (void*)&a };
-		HIP_TRY(hipLaunchKernel(kern, dim3(grid), dim3(G * kWave), params, lds, st));
-	}
+	bool first = true;
+	for (int m = 0; m < 2 && kerns[m]; m++)
+		for (int sl = 0; sl < ctx->dt_needed; sl++)
+		{
+			a.dt_base = sl;
+			if (!first) HIP_TRY(launch_reset_counter(sc.hdr, st));
+			first = false;
+			void* params[] = { (void*)&a };
+			HIP_TRY(hipLaunchKernel(kerns[m], dim3(grids[m]), dim3(groups[m] * kWave), params, ldss[m], st));
+		}
 	return HDDM_LB_SUCCESS;
 }

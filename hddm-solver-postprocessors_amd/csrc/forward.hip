@@ -25,6 +25,7 @@
 #include "sweep_common.h"
 #include <type_traits>
 #include "kernels.h"
+#include "walk.h"
 
 namespace hddm_lb {
 
@@ -312,339 +313,6 @@ __device__ __forceinline__ void contract_chunk(const SOperands<PREC, DT>& so, in
 
 } // namespace
 
-typedef double d16_t __attribute__((ext_vector_type(16)));
-
-__device__ __forceinline__ uint32_t lds_offset(const void* p)
-{
-	return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
-}
-
-// Hand-scheduled scalar-load walk for points whose coordinates fit one bank
-// (D <= 16). Walks ops from byte offset qb until the first surviving node of
-// a chunk other than `chunk` (returned in node and val, not stored; qb is
-// then past its op) or until qb reaches qe (node = -1).
-// Per op: the parent product comes from the LDS path stack (P[t],
-// lane-strided rows of 512 bytes at pl) and the op's product is stored as
-// P[t+1], so P[k] is the product of the first k factors of the current trie
-// path; the coordinate is read from the bank with an indexed move whose
-// index is the op's dim field (twice the dimension). A wave-wide zero
-// product skips the op's run (the skip target and its successor are loaded
-// only when the target is not the successor); a live last factor emits its
-// node into the chunk buffer at bl.
-// Ops rotate through three SGPR sets s[76:79], s[80:83], s[84:87]: while op
-// q is evaluated, op q+1 is already in flight (issued one step earlier) and
-// op q+2 is issued right after the step's one wait, so a scalar-cache miss
-// has a whole step to return. The three steps follow each other in the
-// code: the common path takes one branch per three ops, prunes and the
-// emission of a node of a new chunk branch out. The op handed back is in
-// s[76:79].
-// Fixed registers: the FP64 i as s[94:95] (low word 0), x in v[64:95],
-// temporaries s88-s90, v[96:103] (v98 = 0 is the low word of -j); the
-// indexed moves set m0.
-#define HDDM_WALK_MAIN(X, X0, X1, X2, X3, ZQ, NEXT)                                    \
-	"L_top" X "_%=:\n"                                                                 \
-	"s_lshl_b32 s88, " X1 ", 9\n"                                                      \
-	"s_and_b32 s88, s88, 0x1e00\n"                                                     \
-	"v_add_u32 v102, s88, %[pl]\n"                                                     \
-	"ds_read_b64 v[100:101], v102\n"                                                   \
-	"s_set_gpr_idx_on " X0 ", gpr_idx(SRC0)\n"                                         \
-	"v_mov_b32 v96, v64\n"                                                             \
-	"v_mov_b32 v97, v65\n"                                                             \
-	"s_set_gpr_idx_off\n"                                                              \
-	"s_mov_b32 s95, " X2 "\n"                                                          \
-	"v_mov_b32 v99, " X3 "\n"                                                          \
-	"v_fma_f64 v[96:97], v[96:97], s[94:95], v[98:99]\n"                               \
-	"v_add_f64 v[96:97], -|v[96:97]|, 1.0\n"                                           \
-	"v_max_f64 v[96:97], v[96:97], 0\n"                                                \
-	"s_waitcnt lgkmcnt(0)\n"                                                           \
-	"s_load_dwordx4 " ZQ ", %[ob], %[qb] offset:0x20\n"                                \
-	"v_mul_f64 v[96:97], v[100:101], v[96:97]\n"                                       \
-	"ds_write_b64 v102, v[96:97] offset:512\n"                                         \
-	"s_add_u32 %[nops], %[nops], 1\n"                                                  \
-	"v_cmp_neq_f64_e32 vcc, 0, v[96:97]\n"                                             \
-	"s_cbranch_vccz L_prune" X "_%=\n"                                                 \
-	"s_bitcmp0_b32 " X1 ", 4\n"                                                        \
-	"s_cbranch_scc1 L_adv" X "_%=\n"                                                   \
-	"s_lshr_b32 s89, " X1 ", %[cs]\n"                                                  \
-	"s_cmp_lg_u32 s89, %[chunk]\n"                                                     \
-	"s_cbranch_scc1 L_new" X "_%=\n"                                                   \
-	"s_bfe_u32 s89, " X1 ", %[cf]\n"                                                   \
-	"s_bitset1_b32 %[mask], s89\n"                                                     \
-	"s_lshl_b32 s89, s89, 9\n"                                                         \
-	"v_add_u32 v103, s89, %[bl]\n"                                                     \
-	"ds_write_b64 v103, v[96:97]\n"                                                    \
-	"L_adv" X "_%=:\n"                                                                 \
-	"s_add_u32 %[qb], %[qb], 16\n"                                                     \
-	"s_cmp_lt_u32 %[qb], %[qe]\n"                                                      \
-	NEXT
-
-// continue with step Y: falls through when Y's step follows, else branches back
-#define HDDM_WALK_FALL(Y) "s_cbranch_scc0 L_end" Y "_%=\n"
-#define HDDM_WALK_LOOP(Y) "s_cbranch_scc1 L_top" Y "_%=\n" "s_branch L_end" Y "_%=\n"
-
-#define HDDM_WALK_SIDE(X, X0, X1, Y, YQ, ZQ)                                           \
-	"L_prune" X "_%=:\n"                                                               \
-	"s_add_u32 %[nprunes], %[nprunes], 1\n"                                            \
-	"s_lshr_b32 s90, " X0 ", 8\n"                                                      \
-	"s_cmp_eq_u32 s90, 1\n"                                                            \
-	"s_cbranch_scc1 L_adv" X "_%=\n"                                                   \
-	"s_lshl_b32 s90, s90, 4\n"                                                         \
-	"s_add_u32 %[qb], %[qb], s90\n"                                                    \
-	"s_waitcnt lgkmcnt(0)\n"                                                           \
-	"s_load_dwordx4 " YQ ", %[ob], %[qb]\n"                                            \
-	"s_load_dwordx4 " ZQ ", %[ob], %[qb] offset:0x10\n"                                \
-	"s_waitcnt lgkmcnt(0)\n"                                                           \
-	"s_cmp_lt_u32 %[qb], %[qe]\n"                                                      \
-	"s_cbranch_scc1 L_top" Y "_%=\n"                                                   \
-	"s_branch L_end" Y "_%=\n"                                                         \
-	"L_new" X "_%=:\n"                                                                 \
-	"s_add_u32 %[qb], %[qb], 16\n"                                                     \
-	"s_lshr_b32 %[node], " X1 ", 8\n"                                                  \
-	"v_mov_b64_e32 %[val], v[96:97]\n"                                                 \
-	"s_branch L_exit" Y "_%=\n"
-
-#define HDDM_WALK_ASM                                                                  \
-	"v_mov_b32 v98, 0\n"                                                               \
-	"s_mov_b32 s94, 0\n"                                                               \
-	"s_load_dwordx4 s[80:83], %[ob], %[qb] offset:0x10\n"                              \
-	HDDM_WALK_MAIN("0", "s76", "s77", "s78", "s79", "s[84:87]", HDDM_WALK_FALL("1"))   \
-	HDDM_WALK_MAIN("1", "s80", "s81", "s82", "s83", "s[76:79]", HDDM_WALK_FALL("2"))   \
-	HDDM_WALK_MAIN("2", "s84", "s85", "s86", "s87", "s[80:83]", HDDM_WALK_LOOP("0"))   \
-	HDDM_WALK_SIDE("0", "s76", "s77", "1", "s[80:83]", "s[84:87]")                     \
-	HDDM_WALK_SIDE("1", "s80", "s81", "2", "s[84:87]", "s[76:79]")                     \
-	HDDM_WALK_SIDE("2", "s84", "s85", "0", "s[76:79]", "s[80:83]")                     \
-	"L_end0_%=:\n"                                                                     \
-	"s_mov_b32 %[node], -1\n"                                                          \
-	"L_exit0_%=:\n"                                                                    \
-	"s_waitcnt lgkmcnt(0)\n"                                                           \
-	"s_branch L_done_%=\n"                                                             \
-	"L_end1_%=:\n"                                                                     \
-	"s_mov_b32 %[node], -1\n"                                                          \
-	"L_exit1_%=:\n"                                                                    \
-	"s_waitcnt lgkmcnt(0)\n"                                                           \
-	"s_mov_b64 s[76:77], s[80:81]\n"                                                   \
-	"s_mov_b64 s[78:79], s[82:83]\n"                                                   \
-	"s_branch L_done_%=\n"                                                             \
-	"L_end2_%=:\n"                                                                     \
-	"s_mov_b32 %[node], -1\n"                                                          \
-	"L_exit2_%=:\n"                                                                    \
-	"s_waitcnt lgkmcnt(0)\n"                                                           \
-	"s_mov_b64 s[76:77], s[84:85]\n"                                                   \
-	"s_mov_b64 s[78:79], s[86:87]\n"                                                   \
-	"L_done_%=:\n"
-
-#define HDDM_WALK_OPERANDS                                                             \
-	: [qb] "+s"(qb), [mask] "+s"(mask), [nops] "+s"(nops), [nprunes] "+s"(nprunes),    \
-	  [node] "=&s"(node), [val] "=&v"(val), "+{s[76:79]}"(opv)                          \
-	: [ob] "s"(ob), [qe] "s"(qe), [chunk] "s"(chunk), [pl] "v"(pl), [bl] "v"(bl),      \
-	  [cs] "i"(8 + CB), [cf] "i"((CB << 16) | 8), "{v[64:95]}"(xv)                     \
-	: "s80", "s81", "s82", "s83", "s84", "s85", "s86", "s87", "s88", "s89", "s90",     \
-	  "s94", "s95", "v96", "v97", "v98", "v99", "v100", "v101", "v102", "v103",        \
-	  "vcc", "scc", "m0", "memory"
-
-template <int CB>
-__device__ __forceinline__ int walk_op_stream(const char* ob, uint32_t& qb, uint32_t qe, int chunk,
-	u4_t& opv, uint32_t& mask, uint32_t& nops, uint32_t& nprunes, uint32_t pl, uint32_t bl,
-	const d16_t& xv, double& val)
-{
-	int node;
-	asm volatile(HDDM_WALK_ASM HDDM_WALK_OPERANDS);
-	return node;
-}
-
-#undef HDDM_WALK_OPERANDS
-#undef HDDM_WALK_ASM
-#undef HDDM_WALK_MAIN
-#undef HDDM_WALK_SIDE
-#undef HDDM_WALK_FALL
-#undef HDDM_WALK_LOOP
-
-// Walk state of one wave. P[t] (the product of the first t factors of the
-// current trie path) lives in the wave's LDS stage; the two most recent
-// products stay in registers, so only a move up the trie reads LDS.
-// WALK selects where ops come from:
-//   kWalkSmem  scalar loads from the state's op stream, the sequential
-//              successor read one op ahead; waves are independent
-//   kWalkLds   the workgroup's LDS tile (wave-uniform vector reads)
-template <int XB, typename BT, int WALK, int PM, bool XSEL, int CB>
-struct Walker
-{
-	static constexpr int kC = 1 << CB;        // nodes per chunk
-	static constexpr int kCM = kC - 1;
-	double* P;          // [PM][64] in LDS
-	BT* B;              // [16][64] in LDS: basis values of the current chunk
-	double base_prev;   // P[last_t]
-	double cur;         // P[last_t + 1]
-	int last_t;
-	int q, q1;
-	int chunk;          // chunk whose positions B holds (-1: none)
-	uint32_t mask;      // positions of B written in this chunk
-	uint32_t nops, nprunes;  // counts for measurement
-	const Op* gops;     // kWalkSmem: the op stream and skips
-	const uint32_t* gskip;
-	Op op;              // kWalkSmem: op q, in SGPRs
-
-	__device__ __forceinline__ void init(int q0, int qend, double* Pl, BT* Bl, int lane,
-		const Op* ops, const uint32_t* skip)
-	{
-		q = q0;
-		q1 = qend;
-		P = Pl;
-		B = Bl;
-		P[lane] = 1.0;
-		base_prev = 1.0;
-		cur = 1.0;
-		last_t = -1;
-		chunk = -1;
-		mask = 0;
-		nops = 0;
-		nprunes = 0;
-		gops = ops;
-		gskip = skip;
-		if constexpr (WALK == kWalkSmem) op = ldc_op(ops + q);
-	}
-
-	__device__ __forceinline__ double pget(int t, int lane) { return P[t * kWave + lane]; }
-
-	__device__ __forceinline__ void pset(int t, double v, int lane) { P[t * kWave + lane] = v; }
-
-	// One factor at depth t: the new prefix product.
-	__device__ __forceinline__ double step(int t, uint32_t dim, uint32_t ihi, uint32_t njhi, int lane,
-		const double (&xb0)[16], const double (&xb1)[16], const double (&xb2)[16], const double (&xb3)[16])
-	{
-		double bt;
-		if (t == last_t + 1) bt = cur;
-		else if (t == last_t) bt = base_prev;
-		else bt = pget(t, lane);
-		const double tt = fma(xget<XB, XSEL>(xb0, xb1, xb2, xb3, dim), f64_from_high(ihi), f64_from_high(njhi));
-		const double p = bt * fmax(1.0 - fabs(tt), 0.0);
-		pset(t + 1, p, lane);
-		base_prev = bt;
-		cur = p;
-		last_t = t;
-		nops++;
-		return p;
-	}
-
-	// Walks ops below tend (the end of the resident tile or of the segment),
-	// writing basis values of nodes of `chunk` into B. Stops at the first
-	// surviving node of another chunk, returned in (node, val) and not yet
-	// stored (returns true), or when q reaches tend (returns false).
-	__device__ __forceinline__ bool run(const Op* tops, const uint32_t* tskip, int tb, int tend, int lane,
-		const double (&xb0)[16], const double (&xb1)[16], const double (&xb2)[16], const double (&xb3)[16],
-		int& node, double& val)
-	{
-		if constexpr (WALK == kWalkSmem && XB == 1)
-		{
-			uint32_t qb = (uint32_t)q * (uint32_t)sizeof(Op);
-			const uint32_t qe = (uint32_t)tend * (uint32_t)sizeof(Op);
-			if (qb >= qe) return false;
-			u4_t opv = { op.dim, op.ctrl, op.ihi, op.njhi };
-			d16_t xv;
-#pragma unroll
-			for (int i = 0; i < 16; i++) xv[i] = xb0[i];
-			double v = 0.0;
-			const uint32_t pl = lds_offset(P) + (uint32_t)lane * 8u, bl = lds_offset(B) + (uint32_t)lane * 8u;
-			const int n = walk_op_stream<CB>((const char*)gops, qb, qe, chunk, opv, mask, nops, nprunes, pl, bl, xv, v);
-			op.dim = opv[0]; op.ctrl = opv[1]; op.ihi = opv[2]; op.njhi = opv[3];
-			q = (int)(qb / (uint32_t)sizeof(Op));
-			if (n < 0) return false;
-			node = n;
-			val = v;
-			return true;
-		}
-		else if constexpr (WALK == kWalkSmem)
-		{
-			// Scalar-load walk on the precomputed relation and children bits:
-			// no depth compares, the stack is read only on a move up the trie
-			// and written only by ops with children.
-			while (q < tend)
-			{
-				const uint32_t dimw = op.dim, ctrl = op.ctrl;
-				// the stream ends with kOpPad sentinels, so q + 1 is in bounds;
-				// the load is issued first and stays in flight over the FP64
-				// chain (the scheduling fence keeps it at the top)
-				const Op nop = ldc_op(gops + q + 1);
-				__builtin_amdgcn_sched_barrier(0);
-				const int t = (int)(ctrl & 15u);
-				const uint32_t rel = (ctrl >> kOpRelShift) & 3u;
-				double bt;
-				if (rel == 0u) bt = cur;
-				else if (rel == 1u) bt = base_prev;
-				else
-				{
-					bt = P[t * kWave + lane];
-					// wait here, on the stack-read path only, so the other
-					// paths do not wait for the load above
-					__builtin_amdgcn_s_waitcnt(0xc07f);
-				}
-				const double tt = fma(xget<XB, XSEL>(xb0, xb1, xb2, xb3, op_dim(dimw)),
-					f64_from_high(op.ihi), f64_from_high(op.njhi));
-				const double p = bt * fmax(1.0 - fabs(tt), 0.0);
-				base_prev = bt;
-				cur = p;
-				if (ctrl & kOpKids) P[(t + 1) * kWave + lane] = p;
-				nops++;
-				if (!wave_any(p != 0.0))
-				{
-					nprunes++;
-					// a pruned leaf skips to the next op, which is already
-					// loaded; using nop on both paths also keeps its load at
-					// the top of the iteration, ahead of the FP64 chain
-					const int rs = (int)(dimw >> kOpSkipShift);
-					q += rs;
-					op = rs == 1 ? nop : ldc_op(gops + q);
-					continue;
-				}
-				q++;
-				op = nop;
-				if (ctrl & kOpLast)
-				{
-					const int n = (int)(ctrl >> kOpNodeShift);
-					if ((n >> CB) != chunk) { node = n; val = p; return true; }
-					B[(n & kCM) * kWave + lane] = (BT)p;
-					mask |= 1u << (n & kCM);
-				}
-			}
-			return false;
-		}
-		else
-		{
-			while (q < tend)
-			{
-				const int sl = q - tb;
-				const u4_t ov = *(const u4_t*)(tops + sl);
-				const uint32_t dim = op_dim(__builtin_amdgcn_readfirstlane(ov[0]));
-				const uint32_t ctrl = __builtin_amdgcn_readfirstlane(ov[1]);
-				const uint32_t ihi = __builtin_amdgcn_readfirstlane(ov[2]);
-				const uint32_t njhi = __builtin_amdgcn_readfirstlane(ov[3]);
-				const double p = step((int)(ctrl & 15u), dim, ihi, njhi, lane, xb0, xb1, xb2, xb3);
-				if (!wave_any(p != 0.0))
-				{
-					nprunes++;
-					q = (int)__builtin_amdgcn_readfirstlane(tskip[sl]);
-					continue;
-				}
-				q++;
-				if (ctrl & kOpLast)
-				{
-					const int n = (int)(ctrl >> kOpNodeShift);
-					if ((n >> CB) != chunk) { node = n; val = p; return true; }
-					B[(n & kCM) * kWave + lane] = (BT)p;
-					mask |= 1u << (n & kCM);
-				}
-			}
-			return false;
-		}
-	}
-
-	__device__ __forceinline__ void start_chunk(int node, double val, int lane)
-	{
-		chunk = node >> CB;
-		mask = 1u << (node & kCM);
-		B[(node & kCM) * kWave + lane] = (BT)val;
-	}
-};
 
 // Node chunk of the walk's basis buffer: 16 nodes (the MFMA K extent) for
 // the MFMA paths, 8 for the FP64 path, which contracts node by node and
