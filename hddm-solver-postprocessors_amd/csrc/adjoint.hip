@@ -7,15 +7,17 @@
 // -i sign(x i - j) inside the support and 0 outside (and 0 at the peak).
 // The basis is recomputed by the same prefix-trie walk as the forward sweep
 // (independent waves, one 64-point block per wave, the op stream through
-// scalar loads). Along the walk each trie edge e at depth t keeps
-//   R_e = sum over the surviving nodes k below e of w_k B_k,
-// and when the walk leaves e (a sibling or an ancestor's sibling comes next)
-// it adds R_e * phi'_e / phi_e to dx[dim(e)] and R_e to its parent edge:
-// dB_k/dx_d = B_k phi'_d / phi_d for the one factor of k in dim d, so every
-// node's contribution reaches each of its factors exactly once. w_pk is an
-// FP64 dot product against the node's surplus row at emission; dS is
-// contracted per 16-node chunk with v_mfma_f64_16x16x4f64 (dofs x nodes,
-// K = 4 points per step) and added to the output with FP64 atomics.
+// scalar loads). Two kernels, each with only the state it needs:
+// * dx: along the walk each trie edge e at depth t keeps
+//     R_e = sum over the surviving nodes k below e of w_k B_k,
+//   and when the walk leaves e (a sibling or an ancestor's sibling comes
+//   next) it adds R_e * phi'_e / phi_e to dx[dim(e)] and R_e to its parent
+//   edge: dB_k/dx_d = B_k phi'_d / phi_d for the one factor of k in dim d, so
+//   every node's contribution reaches each of its factors exactly once. w_pk
+//   is an FP64 dot product against the node's surplus row at emission.
+// * dS: the forward's walk emits basis values into 16-node chunks; each
+//   finished chunk is contracted with v_mfma_f64_16x16x4f64 (dofs x nodes,
+//   K = 4 points per step) and added to the output with FP64 atomics.
 #include "sweep_common.h"
 #include "walk.h"
 #include "kernels.h"
@@ -48,13 +50,12 @@ struct AdjointArgs
 	int dt_base;       // dof slab of 32 handled by this launch
 };
 
-// Per-wave LDS of the adjoint walk: P, Q (phi'/phi), R [PM][64]; dx
-// [16 XB][64]; with dS, the chunk basis [16][64] (doubles); dims of the path
-// [PM] (uint32).
-template <int XB, int PM, bool DS>
+// Per-wave LDS of the dx walk: P, Q (phi'/phi), R [PM][64]; dx [16 XB][64]
+// (doubles); dims of the path [PM] (uint32).
+template <int XB, int PM>
 struct AdjLds
 {
-	static constexpr size_t kDoubles = (size_t)(3 * PM + 16 * XB + (DS ? kChunk : 0)) * kWave;
+	static constexpr size_t kDoubles = (size_t)(3 * PM + 16 * XB) * kWave;
 	static constexpr size_t kBytes = kDoubles * 8 + ((PM * 4 + 15) / 16) * 16;
 };
 
@@ -67,9 +68,10 @@ namespace {
 // dS of one finished chunk: D[dof][node] += sum_pt g[pt][dof] B[node][pt],
 // 16 dofs x 16 nodes per v_mfma_f64_16x16x4f64 tile, 16 steps of 4 points.
 // Operand layouts: A[i][k] from lane i + 16k, B[k][j] from lane j + 16k;
-// D[i][j] in lane j + 16 (i % 4), element i / 4.
+// D[i][j] in lane j + 16 (i % 4), element i / 4. The point of A's row k in
+// step s is 4s + k, whose original index comes from its lane by a shuffle.
 __device__ __forceinline__ void contract_ds(const AdjointArgs& a, const StateDesc* sd, const double* Bc,
-	uint32_t mask, int chunk, int lane, const uint32_t (&og)[16], uint32_t ogvalid, int dofs)
+	uint32_t mask, int chunk, int lane, uint32_t orig, int np, int dofs)
 {
 	double* const ds = a.ds + ldc(&sd->ds_offset);
 	const int32_t* order = ldc(&sd->order);
@@ -83,16 +85,24 @@ __device__ __forceinline__ void contract_ds(const AdjointArgs& a, const StateDes
 		const int dbase = (a.dt_base * 2 + tile) * 16;
 		if (dbase >= dofs) break;
 		const int dofA = dbase + col;
-		double av[16];
-#pragma unroll
-		for (int s = 0; s < 16; s++)
-			av[s] = ((ogvalid >> s) & 1u) && dofA < dofs ? a.g[(int64_t)og[s] * a.ldg + dofA] : 0.0;
 		d4_t acc = { 0.0, 0.0, 0.0, 0.0 };
 #pragma unroll
-		for (int s = 0; s < 16; s++)
+		for (int s0 = 0; s0 < 16; s0 += 4)
 		{
-			const double bv = Bc[col * kWave + 4 * s + kq];
-			acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s], bv, acc, 0, 0, 0);
+			double av[4];
+#pragma unroll
+			for (int j = 0; j < 4; j++)
+			{
+				const int pt = 4 * (s0 + j) + kq;
+				const uint32_t og = (uint32_t)__shfl((int)orig, pt, 64);
+				av[j] = pt < np && dofA < dofs ? a.g[(int64_t)og * a.ldg + dofA] : 0.0;
+			}
+#pragma unroll
+			for (int j = 0; j < 4; j++)
+			{
+				const double bv = Bc[col * kWave + 4 * (s0 + j) + kq];
+				acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[j], bv, acc, 0, 0, 0);
+			}
 		}
 		if (live)
 		{
@@ -108,8 +118,8 @@ __device__ __forceinline__ void contract_ds(const AdjointArgs& a, const StateDes
 
 } // namespace
 
-// dx (and, with DS, dS) along the edge-sum walk.
-template <int XB, int PM, bool DS>
+// dx along the edge-sum walk.
+template <int XB, int PM>
 __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 {
 	// Waves are independent: each pops its own items (one 64-point block
@@ -119,12 +129,11 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 	const int wave = uniform((int)(threadIdx.x >> 6));
 	constexpr int DXR = 16 * XB;
 	char* const lbase = (char*)lds;
-	double* const P = (double*)(lbase + (size_t)wave * AdjLds<XB, PM, DS>::kBytes);
+	double* const P = (double*)(lbase + (size_t)wave * AdjLds<XB, PM>::kBytes);
 	double* const Q = P + PM * kWave;
 	double* const R = Q + PM * kWave;
 	double* const DX = R + PM * kWave;
-	double* const Bc = DX + DXR * kWave;
-	uint32_t* const dims = (uint32_t*)(Bc + (DS ? kChunk : 0) * kWave);
+	uint32_t* const dims = (uint32_t*)(DX + DXR * kWave);
 	const int64_t nitems = ldc(&a.hdr->nitems);
 
 	for (;;)
@@ -165,32 +174,14 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 		const double* grow = a.g + (int64_t)orig * a.ldg + 32 * dt0;
 #pragma unroll
 		for (int c = 0; c < 32; c++) g[c] = lane < np && 32 * dt0 + c < dofs ? grow[c] : 0.0;
-		// points whose gradient rows feed the dS operands of this lane
-		uint32_t og[16];
-		uint32_t ogvalid = 0;
-		if constexpr (DS)
-		{
-#pragma unroll
-			for (int s = 0; s < 16; s++)
-			{
-				const int pt = 4 * s + (lane >> 4);
-				og[s] = (uint32_t)__shfl((int)orig, pt, 64);
-				if (pt < np) ogvalid |= 1u << s;
-			}
-#pragma unroll
-			for (int k = 0; k < kChunk; k++) Bc[k * kWave + lane] = 0.0;
-		}
 #pragma unroll
 		for (int d = 0; d < DXR; d++) DX[d * kWave + lane] = 0.0;
 		P[lane] = 1.0;
 
 		const double* const s64 = ldc(&sd->s64) + 32 * dt0;
-		constexpr bool want_ds = DS;
 		int q = q0;
 		int top = -1;          // deepest live edge of the path
 		double cur = 1.0, base_prev = 1.0;
-		int chunk = -1;
-		uint32_t mask = 0;
 		Op op = ldc_op(gops + q);
 		while (q < q1)
 		{
@@ -256,26 +247,8 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 #pragma unroll
 				for (int c = 0; c < 32; c++) w = fma(g[c], ldc(srow + c), w);
 				R[t * kWave + lane] = w * p;
-				if constexpr (want_ds)
-				{
-					if ((n >> 4) != chunk)
-					{
-						if (chunk >= 0)
-						{
-							contract_ds(a, sd, Bc, mask, chunk, lane, og, ogvalid, dofs);
-#pragma unroll
-							for (int k = 0; k < kChunk; k++) Bc[k * kWave + lane] = 0.0;
-						}
-						chunk = n >> 4;
-						mask = 0;
-					}
-					Bc[(n & 15) * kWave + lane] = p;
-					mask |= 1u << (n & 15);
-				}
 			}
 		}
-		if constexpr (want_ds)
-			if (chunk >= 0) contract_ds(a, sd, Bc, mask, chunk, lane, og, ogvalid, dofs);
 		for (int tt = top; tt >= 0; tt--)
 		{
 			const double r = R[tt * kWave + lane];
@@ -301,7 +274,9 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 // wave's LDS stage) with the dS contraction per finished chunk; no dx state,
 // so it needs far fewer registers and LDS than the edge-sum walk.
 template <int XB, int PM>
-__global__ void __launch_bounds__(256) adjoint_ds_kernel(AdjointArgs a)
+__global__ void __launch_bounds__(256)
+	__attribute__((amdgpu_waves_per_eu(XB == 1 ? 3 : (XB == 2 ? 2 : 1))))
+	adjoint_ds_kernel(AdjointArgs a)
 {
 	extern __shared__ __attribute__((aligned(16))) double lds[];
 	const int lane = threadIdx.x & 63;
@@ -340,15 +315,6 @@ __global__ void __launch_bounds__(256) adjoint_ds_kernel(AdjointArgs a)
 			xb2[d] = XB > 2 ? xrow[32 + d < a.dim ? 32 + d : 0] : 0.0;
 			xb3[d] = XB > 2 ? xrow[48 + d < a.dim ? 48 + d : 0] : 0.0;
 		}
-		uint32_t og[16];
-		uint32_t ogvalid = 0;
-#pragma unroll
-		for (int s = 0; s < 16; s++)
-		{
-			const int pt = 4 * s + (lane >> 4);
-			og[s] = (uint32_t)__shfl((int)orig, pt, 64);
-			if (pt < np) ogvalid |= 1u << s;
-		}
 #pragma unroll
 		for (int k = 0; k < kChunk; k++) Bl[k * kWave + lane] = 0.0;
 
@@ -361,7 +327,7 @@ __global__ void __launch_bounds__(256) adjoint_ds_kernel(AdjointArgs a)
 			if (!w.run(nullptr, nullptr, q0, q1, lane, xb0, xb1, xb2, xb3, node, val)) break;
 			if (w.chunk >= 0)
 			{
-				contract_ds(a, sd, Bl, w.mask, w.chunk, lane, og, ogvalid, dofs);
+				contract_ds(a, sd, Bl, w.mask, w.chunk, lane, orig, np, dofs);
 #pragma unroll
 				for (int k = 0; k < kChunk; k++) Bl[k * kWave + lane] = 0.0;
 			}
@@ -369,22 +335,21 @@ __global__ void __launch_bounds__(256) adjoint_ds_kernel(AdjointArgs a)
 		}
 		if (w.chunk >= 0)
 		{
-			contract_ds(a, sd, Bl, w.mask, w.chunk, lane, og, ogvalid, dofs);
+			contract_ds(a, sd, Bl, w.mask, w.chunk, lane, orig, np, dofs);
 #pragma unroll
 			for (int k = 0; k < kChunk; k++) Bl[k * kWave + lane] = 0.0;
 		}
 	}
 }
 
-enum AdjMode { kAdjDx = 0, kAdjBoth = 1, kAdjDs = 2 };
+enum AdjMode { kAdjDx = 0, kAdjDs = 1 };
 
 template <int XB, int PM>
 static const void* adjoint_ptr(int mode, size_t& per_wave)
 {
 	if (mode == kAdjDs) { per_wave = ds_stage_bytes<PM>(); return (const void*)&adjoint_ds_kernel<XB, PM>; }
-	if (mode == kAdjBoth) { per_wave = AdjLds<XB, PM, true>::kBytes; return (const void*)&adjoint_kernel<XB, PM, true>; }
-	per_wave = AdjLds<XB, PM, false>::kBytes;
-	return (const void*)&adjoint_kernel<XB, PM, false>;
+	per_wave = AdjLds<XB, PM>::kBytes;
+	return (const void*)&adjoint_kernel<XB, PM>;
 }
 
 static const void* adjoint_kernel_ptr(int xbanks, int depth, int mode, size_t& per_wave)
