@@ -50,14 +50,13 @@ struct AdjointArgs
 	int dt_base;       // dof slab of 32 handled by this launch
 };
 
-// Per-wave LDS of the dx walk: P, Q (phi'/phi), R [PM][64]; dx [16 XB][64]
-// (doubles); dims of the path [PM] (uint32).
-template <int XB, int PM>
-struct AdjLds
+// Per-wave LDS of the dx walk: P, Q (phi'/phi), R [PM][64]; dx [dim rounded
+// to even][64] (doubles); dims of the path [PM] (uint32).
+__host__ __device__ constexpr int adj_dx_rows(int dim) { return (dim + 1) & ~1; }
+__host__ __device__ constexpr size_t adj_lds_bytes(int pm, int dim)
 {
-	static constexpr size_t kDoubles = (size_t)(3 * PM + 16 * XB) * kWave;
-	static constexpr size_t kBytes = kDoubles * 8 + ((PM * 4 + 15) / 16) * 16;
-};
+	return (size_t)(3 * pm + adj_dx_rows(dim)) * kWave * 8 + ((pm * 4 + 15) / 16) * 16;
+}
 
 // Per-wave LDS of the dS-only sweep: path stack [PM][64] and chunk basis [16][64].
 template <int PM>
@@ -129,11 +128,11 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 	const int wave = uniform((int)(threadIdx.x >> 6));
 	constexpr int DXR = 16 * XB;
 	char* const lbase = (char*)lds;
-	double* const P = (double*)(lbase + (size_t)wave * AdjLds<XB, PM>::kBytes);
+	double* const P = (double*)(lbase + (size_t)wave * adj_lds_bytes(PM, a.dim));
 	double* const Q = P + PM * kWave;
 	double* const R = Q + PM * kWave;
 	double* const DX = R + PM * kWave;
-	uint32_t* const dims = (uint32_t*)(DX + DXR * kWave);
+	uint32_t* const dims = (uint32_t*)(DX + adj_dx_rows(a.dim) * kWave);
 	const int64_t nitems = ldc(&a.hdr->nitems);
 
 	for (;;)
@@ -175,7 +174,8 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 #pragma unroll
 		for (int c = 0; c < 32; c++) g[c] = lane < np && 32 * dt0 + c < dofs ? grow[c] : 0.0;
 #pragma unroll
-		for (int d = 0; d < DXR; d++) DX[d * kWave + lane] = 0.0;
+		for (int d = 0; d < DXR; d++)
+			if (d < a.dim) DX[d * kWave + lane] = 0.0;
 		P[lane] = 1.0;
 
 		const double* const s64 = ldc(&sd->s64) + 32 * dt0;
@@ -345,21 +345,21 @@ __global__ void __launch_bounds__(256)
 enum AdjMode { kAdjDx = 0, kAdjDs = 1 };
 
 template <int XB, int PM>
-static const void* adjoint_ptr(int mode, size_t& per_wave)
+static const void* adjoint_ptr(int mode, int dim, size_t& per_wave)
 {
 	if (mode == kAdjDs) { per_wave = ds_stage_bytes<PM>(); return (const void*)&adjoint_ds_kernel<XB, PM>; }
-	per_wave = AdjLds<XB, PM>::kBytes;
+	per_wave = adj_lds_bytes(PM, dim);
 	return (const void*)&adjoint_kernel<XB, PM>;
 }
 
-static const void* adjoint_kernel_ptr(int xbanks, int depth, int mode, size_t& per_wave)
+static const void* adjoint_kernel_ptr(int dim, int depth, int mode, size_t& per_wave)
 {
 	const bool deep = depth >= 8;
-	switch (xbanks)
+	switch (xbanks_for(dim))
 	{
-	case 1: return deep ? adjoint_ptr<1, 16>(mode, per_wave) : adjoint_ptr<1, 8>(mode, per_wave);
-	case 2: return deep ? adjoint_ptr<2, 16>(mode, per_wave) : adjoint_ptr<2, 8>(mode, per_wave);
-	case 3: case 4: return deep ? adjoint_ptr<4, 16>(mode, per_wave) : adjoint_ptr<4, 8>(mode, per_wave);
+	case 1: return deep ? adjoint_ptr<1, 16>(mode, dim, per_wave) : adjoint_ptr<1, 8>(mode, dim, per_wave);
+	case 2: return deep ? adjoint_ptr<2, 16>(mode, dim, per_wave) : adjoint_ptr<2, 8>(mode, dim, per_wave);
+	case 3: case 4: return deep ? adjoint_ptr<4, 16>(mode, dim, per_wave) : adjoint_ptr<4, 8>(mode, dim, per_wave);
 	}
 	return nullptr;
 }
@@ -409,17 +409,27 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 	for (int m = 0; m < 2 && modes[m] >= 0; m++)
 	{
 		size_t per_wave = 0;
-		const void* kern = adjoint_kernel_ptr(xbanks_for(ctx->dim), ctx->fmax_all, modes[m], per_wave);
+		const void* kern = adjoint_kernel_ptr(ctx->dim, ctx->fmax_all, modes[m], per_wave);
 		if (!kern) return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: dim %d is not supported", ctx->dim);
-		int G = (int)std::min<size_t>(4, ctx->max_lds / per_wave);
-		if (ctx->opt_waves) G = std::min(G, ctx->opt_waves);
-		if (G < 1) return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: LDS of one wave (%zu B) exceeds the CU", per_wave);
-		const size_t lds = (size_t)G * per_wave;
+		if (per_wave > ctx->max_lds)
+			return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: LDS of one wave (%zu B) exceeds the CU", per_wave);
+		// waves are independent: take the workgroup size (1, 2 or 4 waves)
+		// that keeps the most waves resident per CU (LDS rarely divides evenly)
+		int bestG = 0, best_waves = 0, best_per_cu = 0;
+		for (int G = 1; G <= 4; G *= 2)
+		{
+			if (ctx->opt_waves && G > ctx->opt_waves) break;
+			const size_t lds = (size_t)G * per_wave;
+			if (lds > ctx->max_lds) break;
+			HIP_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+			int per_cu = 0;
+			HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, G * kWave, lds));
+			if (per_cu * G >= best_waves && per_cu > 0) { bestG = G; best_waves = per_cu * G; best_per_cu = per_cu; }
+		}
+		if (bestG < 1) return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: kernel does not fit a CU");
+		const size_t lds = (size_t)bestG * per_wave;
 		HIP_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-		int per_cu = 0;
-		HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, G * kWave, lds));
-		if (per_cu < 1) return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: kernel does not fit a CU");
-		kerns[m] = kern; grids[m] = per_cu * ctx->num_cus; groups[m] = G; ldss[m] = lds;
+		kerns[m] = kern; grids[m] = best_per_cu * ctx->num_cus; groups[m] = bestG; ldss[m] = lds;
 	}
 
 	Scratch& sc = ctx->scratch;
