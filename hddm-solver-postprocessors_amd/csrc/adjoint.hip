@@ -50,12 +50,13 @@ struct AdjointArgs
 	int dt_base;       // dof slab of 32 handled by this launch
 };
 
-// Per-wave LDS of the dx walk: P, Q (phi'/phi), R [PM][64]; dx [dim rounded
-// to even][64] (doubles); dims of the path [PM] (uint32).
+// Per-wave LDS of the dx walk: P, R [PM][64]; dx [dim rounded to even][64]
+// (doubles); per depth of the path the edge's op words {dim, i, -j, 0}
+// (uint32 x 4), from which phi'/phi is recomputed when the edge is left.
 __host__ __device__ constexpr int adj_dx_rows(int dim) { return (dim + 1) & ~1; }
 __host__ __device__ constexpr size_t adj_lds_bytes(int pm, int dim)
 {
-	return (size_t)(3 * pm + adj_dx_rows(dim)) * kWave * 8 + ((pm * 4 + 15) / 16) * 16;
+	return (size_t)(2 * pm + adj_dx_rows(dim)) * kWave * 8 + (size_t)pm * 16;
 }
 
 // Per-wave LDS of the dS-only sweep: path stack [PM][64] and chunk basis [16][64].
@@ -117,6 +118,27 @@ __device__ __forceinline__ void contract_ds(const AdjointArgs& a, const StateDes
 
 } // namespace
 
+// Leaves the path edge at depth tt with sum r: dx[dim] += r * phi'/phi and
+// r goes to the parent edge. phi'/phi = -i sign(x i - j) / phi inside the
+// support (0 outside and at the peak), from the hardware reciprocal refined
+// by two Newton steps (error ~1 ulp) instead of a full FP64 division.
+template <int XB>
+__device__ __forceinline__ void leave_edge(const u4_t& e, double r, const double (&xb0)[16], const double (&xb1)[16],
+	const double (&xb2)[16], const double (&xb3)[16], double* DX, double* R, int tt, int lane)
+{
+	const uint32_t dd = __builtin_amdgcn_readfirstlane(e[0]);
+	const double iv = f64_from_high(__builtin_amdgcn_readfirstlane(e[1]));
+	const double u = fma(xget<XB, true>(xb0, xb1, xb2, xb3, dd), iv, f64_from_high(__builtin_amdgcn_readfirstlane(e[2])));
+	const double phi = 1.0 - fabs(u);
+	const double ph = phi > 0.0 ? phi : 1.0;
+	double rc = __builtin_amdgcn_rcp(ph);
+	rc = fma(rc, fma(-ph, rc, 1.0), rc);
+	rc = fma(rc, fma(-ph, rc, 1.0), rc);
+	const double qv = phi > 0.0 ? (u > 0.0 ? -iv : (u < 0.0 ? iv : 0.0)) * rc : 0.0;
+	DX[dd * kWave + lane] += r * qv;
+	if (tt > 0) R[(tt - 1) * kWave + lane] += r;
+}
+
 // dx along the edge-sum walk.
 template <int XB, int PM>
 __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
@@ -129,10 +151,9 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 	constexpr int DXR = 16 * XB;
 	char* const lbase = (char*)lds;
 	double* const P = (double*)(lbase + (size_t)wave * adj_lds_bytes(PM, a.dim));
-	double* const Q = P + PM * kWave;
-	double* const R = Q + PM * kWave;
+	double* const R = P + PM * kWave;
 	double* const DX = R + PM * kWave;
-	uint32_t* const dims = (uint32_t*)(DX + adj_dx_rows(a.dim) * kWave);
+	u4_t* const edges = (u4_t*)(DX + adj_dx_rows(a.dim) * kWave);
 	const int64_t nitems = ldc(&a.hdr->nitems);
 
 	for (;;)
@@ -196,12 +217,7 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 			for (int tt = top; tt >= t; tt--)
 			{
 				const double r = R[tt * kWave + lane];
-				if (wave_any(r != 0.0))
-				{
-					const uint32_t dd = __builtin_amdgcn_readfirstlane(dims[tt]);
-					DX[dd * kWave + lane] += r * Q[tt * kWave + lane];
-					if (tt > 0) R[(tt - 1) * kWave + lane] += r;
-				}
+				if (wave_any(r != 0.0)) leave_edge<XB>(edges[tt], r, xb0, xb1, xb2, xb3, DX, R, tt, lane);
 			}
 			top = t;
 			// parent product from the op's relation to its stream predecessor
@@ -216,26 +232,17 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 			const double p = bt * fmax(phi, 0.0);
 			if (ctrl & kOpKids) P[(t + 1) * kWave + lane] = p;
 			R[t * kWave + lane] = 0.0;
-			if (lane == 0) dims[t] = dim;
+			if (lane == 0) edges[t] = u4_t{ dim, op.ihi, op.njhi, 0u };
 			base_prev = bt;
 			cur = p;
 			if (!wave_any(p != 0.0))
 			{
-				// the edge's sum stays 0, so its phi'/phi is never read; a
-				// pruned leaf moves to the already loaded next op
+				// the edge's sum stays 0, so it is never left with a
+				// contribution; a pruned leaf moves to the already loaded next op
 				const int rs = (int)(dimw >> kOpSkipShift);
 				q += rs;
 				op = rs == 1 ? nop : ldc_op(gops + q);
 				continue;
-			}
-			// phi'/phi inside the support: hardware reciprocal refined by two
-			// Newton steps (error ~1 ulp), instead of a full FP64 division
-			{
-				const double ph = phi > 0.0 ? phi : 1.0;
-				double r = __builtin_amdgcn_rcp(ph);
-				r = fma(r, fma(-ph, r, 1.0), r);
-				r = fma(r, fma(-ph, r, 1.0), r);
-				Q[t * kWave + lane] = phi > 0.0 ? (u > 0.0 ? -iv : (u < 0.0 ? iv : 0.0)) * r : 0.0;
 			}
 			q++;
 			op = nop;
@@ -252,9 +259,7 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 		for (int tt = top; tt >= 0; tt--)
 		{
 			const double r = R[tt * kWave + lane];
-			const uint32_t dd = __builtin_amdgcn_readfirstlane(dims[tt]);
-			DX[dd * kWave + lane] += r * Q[tt * kWave + lane];
-			if (tt > 0) R[(tt - 1) * kWave + lane] += r;
+			if (wave_any(r != 0.0)) leave_edge<XB>(edges[tt], r, xb0, xb1, xb2, xb3, DX, R, tt, lane);
 		}
 		if (a.dx && lane < np)
 		{
