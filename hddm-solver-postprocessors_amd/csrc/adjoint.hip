@@ -65,13 +65,38 @@ constexpr size_t ds_stage_bytes() { return (size_t)(PM + kChunk) * kWave * 8; }
 
 namespace {
 
+// A operands of the dS contraction for one 64-point block: the upstream
+// gradient in the v_mfma_f64_16x16x4f64 A layout, A[i][k] from lane i + 16k
+// (i: dof of the 16-dof tile, k: point 4s + k of step s). Loaded once per
+// block; lanes past the block hold 0.
+struct GradTiles
+{
+	double v[2][16];
+
+	__device__ __forceinline__ void load(const AdjointArgs& a, uint32_t orig, int np, int dofs, int lane)
+	{
+		const int col = lane & 15, kq = lane >> 4;
+#pragma unroll
+		for (int tile = 0; tile < 2; tile++)
+		{
+			const int dofA = (a.dt_base * 2 + tile) * 16 + col;
+#pragma unroll
+			for (int s = 0; s < 16; s++)
+			{
+				const int pt = 4 * s + kq;
+				const uint32_t og = (uint32_t)__shfl((int)orig, pt, 64);
+				v[tile][s] = pt < np && dofA < dofs ? a.g[(int64_t)og * a.ldg + dofA] : 0.0;
+			}
+		}
+	}
+};
+
 // dS of one finished chunk: D[dof][node] += sum_pt g[pt][dof] B[node][pt],
 // 16 dofs x 16 nodes per v_mfma_f64_16x16x4f64 tile, 16 steps of 4 points.
 // Operand layouts: A[i][k] from lane i + 16k, B[k][j] from lane j + 16k;
-// D[i][j] in lane j + 16 (i % 4), element i / 4. The point of A's row k in
-// step s is 4s + k, whose original index comes from its lane by a shuffle.
+// D[i][j] in lane j + 16 (i % 4), element i / 4.
 __device__ __forceinline__ void contract_ds(const AdjointArgs& a, const StateDesc* sd, const double* Bc,
-	uint32_t mask, int chunk, int lane, uint32_t orig, int np, int dofs)
+	uint32_t mask, int chunk, int lane, const GradTiles& gt, int dofs)
 {
 	double* const ds = a.ds + ldc(&sd->ds_offset);
 	const int32_t* order = ldc(&sd->order);
@@ -79,40 +104,25 @@ __device__ __forceinline__ void contract_ds(const AdjointArgs& a, const StateDes
 	const int node = chunk * kChunk + col;
 	const bool live = (mask >> col) & 1u;
 	const int orow = live ? order[node] : 0;
+	// the two dof tiles' chains are interleaved (independent accumulators)
+	d4_t acc[2] = { { 0.0, 0.0, 0.0, 0.0 }, { 0.0, 0.0, 0.0, 0.0 } };
 #pragma unroll
-	for (int tile = 0; tile < 2; tile++)
+	for (int s = 0; s < 16; s++)
 	{
-		const int dbase = (a.dt_base * 2 + tile) * 16;
-		if (dbase >= dofs) break;
-		const int dofA = dbase + col;
-		d4_t acc = { 0.0, 0.0, 0.0, 0.0 };
+		const double bv = Bc[col * kWave + 4 * s + kq];
+		acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(gt.v[0][s], bv, acc[0], 0, 0, 0);
+		acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(gt.v[1][s], bv, acc[1], 0, 0, 0);
+	}
+	if (live)
+	{
 #pragma unroll
-		for (int s0 = 0; s0 < 16; s0 += 4)
-		{
-			double av[4];
-#pragma unroll
-			for (int j = 0; j < 4; j++)
-			{
-				const int pt = 4 * (s0 + j) + kq;
-				const uint32_t og = (uint32_t)__shfl((int)orig, pt, 64);
-				av[j] = pt < np && dofA < dofs ? a.g[(int64_t)og * a.ldg + dofA] : 0.0;
-			}
-#pragma unroll
-			for (int j = 0; j < 4; j++)
-			{
-				const double bv = Bc[col * kWave + 4 * (s0 + j) + kq];
-				acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[j], bv, acc, 0, 0, 0);
-			}
-		}
-		if (live)
-		{
+		for (int tile = 0; tile < 2; tile++)
 #pragma unroll
 			for (int r = 0; r < 4; r++)
 			{
-				const int dof = dbase + kq + 4 * r;
-				if (dof < dofs) unsafeAtomicAdd(ds + (int64_t)orow * dofs + dof, acc[r]);
+				const int dof = (a.dt_base * 2 + tile) * 16 + kq + 4 * r;
+				if (dof < dofs) unsafeAtomicAdd(ds + (int64_t)orow * dofs + dof, acc[tile][r]);
 			}
-		}
 	}
 }
 
@@ -322,6 +332,8 @@ __global__ void __launch_bounds__(256)
 		}
 #pragma unroll
 		for (int k = 0; k < kChunk; k++) Bl[k * kWave + lane] = 0.0;
+		GradTiles gt;
+		gt.load(a, orig, np, dofs, lane);
 
 		Walker<XB, double, kWalkSmem, PM, true, 4> w;
 		w.init(q0, q1, Pl, Bl, lane, gops, gskip);
@@ -332,7 +344,7 @@ __global__ void __launch_bounds__(256)
 			if (!w.run(nullptr, nullptr, q0, q1, lane, xb0, xb1, xb2, xb3, node, val)) break;
 			if (w.chunk >= 0)
 			{
-				contract_ds(a, sd, Bl, w.mask, w.chunk, lane, orig, np, dofs);
+				contract_ds(a, sd, Bl, w.mask, w.chunk, lane, gt, dofs);
 #pragma unroll
 				for (int k = 0; k < kChunk; k++) Bl[k * kWave + lane] = 0.0;
 			}
@@ -340,7 +352,7 @@ __global__ void __launch_bounds__(256)
 		}
 		if (w.chunk >= 0)
 		{
-			contract_ds(a, sd, Bl, w.mask, w.chunk, lane, orig, np, dofs);
+			contract_ds(a, sd, Bl, w.mask, w.chunk, lane, gt, dofs);
 #pragma unroll
 			for (int k = 0; k < kChunk; k++) Bl[k * kWave + lane] = 0.0;
 		}
