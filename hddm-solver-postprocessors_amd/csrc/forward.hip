@@ -326,7 +326,7 @@ constexpr size_t wave_stage_bytes() { return (size_t)(PM + (1 << CB)) * kWave * 
 
 template <int PREC, int DT, int XB, int PM, int WALK>
 __global__ void __launch_bounds__(256)
-	__attribute__((amdgpu_waves_per_eu(PREC != HDDM_LB_FP64 && XB == 1 && DT == 1 ? 3 : 1)))
+	__attribute__((amdgpu_waves_per_eu(PREC != HDDM_LB_FP64 && DT == 1 ? (XB == 1 ? 3 : (XB == 2 ? 2 : 1)) : 1)))
 	sweep_kernel(SweepArgs a)
 {
 	extern __shared__ __attribute__((aligned(16))) double lds[];

@@ -34,30 +34,30 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 // code: the common path takes one branch per three ops, prunes and the
 // emission of a node of a new chunk branch out. The op handed back is in
 // s[76:79].
-// Fixed registers: the FP64 i as s[94:95] (low word 0), x in v[64:95],
-// temporaries s88-s90, v[96:103] (v98 = 0 is the low word of -j); the
-// indexed moves set m0.
+// Fixed registers: the FP64 i as s[94:95] (low word 0), x from v64 (one or
+// two banks), temporaries s88-s90 and eight VGPRs after x (HW_*; the low word
+// of -j is kept 0 in HW_B0); the indexed moves set m0.
 #define HDDM_WALK_MAIN(X, X0, X1, X2, X3, ZQ, NEXT)                                    \
 	"L_top" X "_%=:\n"                                                                 \
 	"s_lshl_b32 s88, " X1 ", 9\n"                                                      \
 	"s_and_b32 s88, s88, 0x1e00\n"                                                     \
-	"v_add_u32 v102, s88, %[pl]\n"                                                     \
-	"ds_read_b64 v[100:101], v102\n"                                                   \
+	"v_add_u32 " HW_D ", s88, %[pl]\n"                                                     \
+	"ds_read_b64 " HW_C ", " HW_D "\n"                                                   \
 	"s_set_gpr_idx_on " X0 ", gpr_idx(SRC0)\n"                                         \
-	"v_mov_b32 v96, v64\n"                                                             \
-	"v_mov_b32 v97, v65\n"                                                             \
+	"v_mov_b32 " HW_A0 ", v64\n"                                                             \
+	"v_mov_b32 " HW_A1 ", v65\n"                                                             \
 	"s_set_gpr_idx_off\n"                                                              \
 	"s_mov_b32 s95, " X2 "\n"                                                          \
-	"v_mov_b32 v99, " X3 "\n"                                                          \
-	"v_fma_f64 v[96:97], v[96:97], s[94:95], v[98:99]\n"                               \
-	"v_add_f64 v[96:97], -|v[96:97]|, 1.0\n"                                           \
-	"v_max_f64 v[96:97], v[96:97], 0\n"                                                \
+	"v_mov_b32 " HW_B1 ", " X3 "\n"                                                          \
+	"v_fma_f64 " HW_A ", " HW_A ", s[94:95], " HW_B "\n"                               \
+	"v_add_f64 " HW_A ", -|" HW_A "|, 1.0\n"                                           \
+	"v_max_f64 " HW_A ", " HW_A ", 0\n"                                                \
 	"s_waitcnt lgkmcnt(0)\n"                                                           \
 	"s_load_dwordx4 " ZQ ", %[ob], %[qb] offset:0x20\n"                                \
-	"v_mul_f64 v[96:97], v[100:101], v[96:97]\n"                                       \
-	"ds_write_b64 v102, v[96:97] offset:512\n"                                         \
+	"v_mul_f64 " HW_A ", " HW_C ", " HW_A "\n"                                       \
+	"ds_write_b64 " HW_D ", " HW_A " offset:512\n"                                         \
 	"s_add_u32 %[nops], %[nops], 1\n"                                                  \
-	"v_cmp_neq_f64_e32 vcc, 0, v[96:97]\n"                                             \
+	"v_cmp_neq_f64_e32 vcc, 0, " HW_A "\n"                                             \
 	"s_cbranch_vccz L_prune" X "_%=\n"                                                 \
 	"s_bitcmp0_b32 " X1 ", 4\n"                                                        \
 	"s_cbranch_scc1 L_adv" X "_%=\n"                                                   \
@@ -67,8 +67,8 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 	"s_bfe_u32 s89, " X1 ", %[cf]\n"                                                   \
 	"s_bitset1_b32 %[mask], s89\n"                                                     \
 	"s_lshl_b32 s89, s89, 9\n"                                                         \
-	"v_add_u32 v103, s89, %[bl]\n"                                                     \
-	"ds_write_b64 v103, v[96:97]\n"                                                    \
+	"v_add_u32 " HW_E ", s89, %[bl]\n"                                                     \
+	"ds_write_b64 " HW_E ", " HW_A "\n"                                                    \
 	"L_adv" X "_%=:\n"                                                                 \
 	"s_add_u32 %[qb], %[qb], 16\n"                                                     \
 	"s_cmp_lt_u32 %[qb], %[qe]\n"                                                      \
@@ -96,11 +96,11 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 	"L_new" X "_%=:\n"                                                                 \
 	"s_add_u32 %[qb], %[qb], 16\n"                                                     \
 	"s_lshr_b32 %[node], " X1 ", 8\n"                                                  \
-	"v_mov_b64_e32 %[val], v[96:97]\n"                                                 \
+	"v_mov_b64_e32 %[val], " HW_A "\n"                                                 \
 	"s_branch L_exit" Y "_%=\n"
 
 #define HDDM_WALK_ASM                                                                  \
-	"v_mov_b32 v98, 0\n"                                                               \
+	"v_mov_b32 " HW_B0 ", 0\n"                                                               \
 	"s_mov_b32 s94, 0\n"                                                               \
 	"s_load_dwordx4 s[80:83], %[ob], %[qb] offset:0x10\n"                              \
 	HDDM_WALK_MAIN("0", "s76", "s77", "s78", "s79", "s[84:87]", HDDM_WALK_FALL("1"))   \
@@ -129,26 +129,80 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 	"s_mov_b64 s[78:79], s[86:87]\n"                                                   \
 	"L_done_%=:\n"
 
-#define HDDM_WALK_OPERANDS                                                             \
+#define HDDM_WALK_OUTPUTS                                                              \
 	: [qb] "+s"(qb), [mask] "+s"(mask), [nops] "+s"(nops), [nprunes] "+s"(nprunes),    \
-	  [node] "=&s"(node), [val] "=&v"(val), "+{s[76:79]}"(opv)                          \
+	  [node] "=&s"(node), [val] "=&v"(val), "+{s[76:79]}"(opv)
+#define HDDM_WALK_INPUTS                                                               \
 	: [ob] "s"(ob), [qe] "s"(qe), [chunk] "s"(chunk), [pl] "v"(pl), [bl] "v"(bl),      \
-	  [cs] "i"(8 + CB), [cf] "i"((CB << 16) | 8), "{v[64:95]}"(xv)                     \
-	: "s80", "s81", "s82", "s83", "s84", "s85", "s86", "s87", "s88", "s89", "s90",     \
-	  "s94", "s95", "v96", "v97", "v98", "v99", "v100", "v101", "v102", "v103",        \
-	  "vcc", "scc", "m0", "memory"
+	  [cs] "i"(8 + CB), [cf] "i"((CB << 16) | 8)
+#define HDDM_WALK_SCLOBBERS                                                            \
+	"s80", "s81", "s82", "s83", "s84", "s85", "s86", "s87", "s88", "s89", "s90",       \
+	"s94", "s95", "vcc", "scc", "m0", "memory"
 
+// One coordinate bank (D <= 16): x in v[64:95], temporaries v[96:103].
 template <int CB>
 __device__ __forceinline__ int walk_op_stream(const char* ob, uint32_t& qb, uint32_t qe, int chunk,
 	u4_t& opv, uint32_t& mask, uint32_t& nops, uint32_t& nprunes, uint32_t pl, uint32_t bl,
 	const d16_t& xv, double& val)
 {
 	int node;
-	asm volatile(HDDM_WALK_ASM HDDM_WALK_OPERANDS);
+#define HW_A "v[96:97]"
+#define HW_A0 "v96"
+#define HW_A1 "v97"
+#define HW_B "v[98:99]"
+#define HW_B0 "v98"
+#define HW_B1 "v99"
+#define HW_C "v[100:101]"
+#define HW_D "v102"
+#define HW_E "v103"
+	asm volatile(HDDM_WALK_ASM HDDM_WALK_OUTPUTS HDDM_WALK_INPUTS, "{v[64:95]}"(xv)
+		: HDDM_WALK_SCLOBBERS, "v96", "v97", "v98", "v99", "v100", "v101", "v102", "v103");
+#undef HW_A
+#undef HW_A0
+#undef HW_A1
+#undef HW_B
+#undef HW_B0
+#undef HW_B1
+#undef HW_C
+#undef HW_D
+#undef HW_E
 	return node;
 }
 
-#undef HDDM_WALK_OPERANDS
+// Two coordinate banks (16 < D <= 32): x in v[64:127] (the indexed move
+// reaches dims 16-31 through the same index), temporaries v[128:135].
+template <int CB>
+__device__ __forceinline__ int walk_op_stream2(const char* ob, uint32_t& qb, uint32_t qe, int chunk,
+	u4_t& opv, uint32_t& mask, uint32_t& nops, uint32_t& nprunes, uint32_t pl, uint32_t bl,
+	const d16_t& xv0, const d16_t& xv1, double& val)
+{
+	int node;
+#define HW_A "v[128:129]"
+#define HW_A0 "v128"
+#define HW_A1 "v129"
+#define HW_B "v[130:131]"
+#define HW_B0 "v130"
+#define HW_B1 "v131"
+#define HW_C "v[132:133]"
+#define HW_D "v134"
+#define HW_E "v135"
+	asm volatile(HDDM_WALK_ASM HDDM_WALK_OUTPUTS HDDM_WALK_INPUTS, "{v[64:95]}"(xv0), "{v[96:127]}"(xv1)
+		: HDDM_WALK_SCLOBBERS, "v128", "v129", "v130", "v131", "v132", "v133", "v134", "v135");
+#undef HW_A
+#undef HW_A0
+#undef HW_A1
+#undef HW_B
+#undef HW_B0
+#undef HW_B1
+#undef HW_C
+#undef HW_D
+#undef HW_E
+	return node;
+}
+
+#undef HDDM_WALK_OUTPUTS
+#undef HDDM_WALK_INPUTS
+#undef HDDM_WALK_SCLOBBERS
 #undef HDDM_WALK_ASM
 #undef HDDM_WALK_MAIN
 #undef HDDM_WALK_SIDE
@@ -230,18 +284,22 @@ struct Walker
 		const double (&xb0)[16], const double (&xb1)[16], const double (&xb2)[16], const double (&xb3)[16],
 		int& node, double& val)
 	{
-		if constexpr (WALK == kWalkSmem && XB == 1)
+		if constexpr (WALK == kWalkSmem && XB <= 2)
 		{
 			uint32_t qb = (uint32_t)q * (uint32_t)sizeof(Op);
 			const uint32_t qe = (uint32_t)tend * (uint32_t)sizeof(Op);
 			if (qb >= qe) return false;
 			u4_t opv = { op.dim, op.ctrl, op.ihi, op.njhi };
-			d16_t xv;
+			d16_t xv0, xv1;
 #pragma unroll
-			for (int i = 0; i < 16; i++) xv[i] = xb0[i];
+			for (int i = 0; i < 16; i++) { xv0[i] = xb0[i]; xv1[i] = xb1[i]; }
 			double v = 0.0;
 			const uint32_t pl = lds_offset(P) + (uint32_t)lane * 8u, bl = lds_offset(B) + (uint32_t)lane * 8u;
-			const int n = walk_op_stream<CB>((const char*)gops, qb, qe, chunk, opv, mask, nops, nprunes, pl, bl, xv, v);
+			int n;
+			if constexpr (XB == 1)
+				n = walk_op_stream<CB>((const char*)gops, qb, qe, chunk, opv, mask, nops, nprunes, pl, bl, xv0, v);
+			else
+				n = walk_op_stream2<CB>((const char*)gops, qb, qe, chunk, opv, mask, nops, nprunes, pl, bl, xv0, xv1, v);
 			op.dim = opv[0]; op.ctrl = opv[1]; op.ihi = opv[2]; op.njhi = opv[3];
 			q = (int)(qb / (uint32_t)sizeof(Op));
 			if (n < 0) return false;
