@@ -1,6 +1,6 @@
 // The per-wave prefix-trie walk shared by the forward sweep and the adjoint
-// (gfx950 only): the hand-scheduled scalar-load walk for one coordinate bank
-// and the walk state (path stack, chunk buffer, op position).
+// (gfx950 only): the hand-scheduled scalar-load walk for one or two
+// coordinate banks and the walk state (path stack, chunk buffer, op position).
 #pragma once
 
 #include "sweep_common.h"
@@ -15,8 +15,8 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 	return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
 }
 
-// Hand-scheduled scalar-load walk for points whose coordinates fit one bank
-// (D <= 16). Walks ops from byte offset qb until the first surviving node of
+// Hand-scheduled scalar-load walk for points whose coordinates fit one or two
+// banks (D <= 32). Walks ops from byte offset qb until the first surviving node of
 // a chunk other than `chunk` (returned in node and val, not stored; qb is
 // then past its op) or until qb reaches qe (node = -1).
 // Per op: the parent product comes from the LDS path stack (P[t],
