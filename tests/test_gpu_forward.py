@@ -89,6 +89,22 @@ def test_config2_grid_d16_subset(ctx, prec):
     assert err <= TOL[prec]
 
 
+@pytest.mark.parametrize("dim", [15, 16, 17, 24, 32])
+def test_coordinate_bank_edges(ctx, dim):
+    """Dimensions at and around the one/two coordinate-bank boundary (the
+    walk reads x through a register index of twice the dimension), every
+    precision, against the oracle."""
+    ii, jj, lv, ix = hddm_lb.generate_grid(dim, 2, 5, 0.4, 40 + dim, 4000)
+    S = hddm_lb.synthetic_surplus(lv, 32, 41 + dim)
+    ctx.upload_state(0, ii, jj, S)
+    X = hddm_lb.synthetic_points(700, dim, 42 + dim)
+    yo = oracle.forward(ii, jj, S, X)
+    for prec in ALL_PRECISIONS:
+        say("bank edge dim", dim, "nno", ii.shape[0], "precision", prec)
+        y = run_forward(ctx, X, precision=prec)
+        assert normwise_rel_err(y, yo) <= TOL[prec]
+
+
 @pytest.mark.parametrize("n", [1, 2, 63, 65, 200])
 def test_ragged_point_counts(ctx, n):
     ii, jj, lv, ix = hddm_lb.generate_grid(6, 2, 5, 0.5, 5, 3000)
