@@ -149,13 +149,14 @@ __device__ __forceinline__ V as_vec8(uint32_t a, uint32_t b, uint32_t c, uint32_
 	return __builtin_bit_cast(V, u);
 }
 
-// S^T operands (the MFMA A side) of one 16-node chunk, loaded ahead of use.
+// S^T operands (the MFMA A side) of one chunk (16 nodes; 32 for F32, two
+// consecutive 16-node surplus chunks), loaded ahead of use.
 template <int PREC, int DT>
 struct SOperands
 {
 	typedef typename std::conditional<PREC == HDDM_LB_F16X2, h8_t,
 		typename std::conditional<PREC == HDDM_LB_BF16X3, b8_t, float>::type>::type V;
-	static constexpr int kTerms = PREC == HDDM_LB_F16X2 ? 2 : (PREC == HDDM_LB_BF16X3 ? 3 : 8);
+	static constexpr int kTerms = PREC == HDDM_LB_F16X2 ? 2 : (PREC == HDDM_LB_BF16X3 ? 3 : 16);
 	V A[DT][kTerms];
 
 	__device__ __forceinline__ void load(const StateDesc* sd, int chunk, int DP, int dt0, int ndt, int lane)
@@ -182,16 +183,26 @@ struct SOperands
 				}
 				else
 				{
-					const float* ap = ldc(&sd->s32) + (size_t)chunk * DP * kChunk + ((size_t)dr * 2 + khalf) * 8;
+					// K-step q (nodes 2q, 2q+1 of the 32) is step q % 8 of
+					// surplus chunk 2 chunk + q / 8; past the last chunk: 0
+					const int nch = ldc(&sd->nchunks);
 #pragma unroll
-					for (int q = 0; q < 8; q++) A[dt][q] = ap[q];
+					for (int h = 0; h < 2; h++)
+					{
+						const int sc = 2 * chunk + h;
+						const float* ap = ldc(&sd->s32) + (size_t)(sc < nch ? sc : 0) * DP * kChunk
+							+ ((size_t)dr * 2 + khalf) * 8;
+#pragma unroll
+						for (int q = 0; q < 8; q++) A[dt][8 * h + q] = sc < nch ? ap[q] : 0.0f;
+					}
 				}
 			}
 	}
 };
 
-// MFMA contraction of one 16-node chunk: acc[t][dt] (points 32t.., dofs
-// 32(dt0+dt)..) += S^T[dofs x 16 nodes] * B^T[16 nodes x 32 points].
+// Split-precision MFMA contraction of one 16-node chunk (F16X2, BF16X3):
+// acc[t][dt] (points 32t.., dofs 32(dt0+dt)..) += S^T[dofs x 16 nodes] *
+// B^T[16 nodes x 32 points]. The F32 path contracts in the sweep kernel.
 constexpr float kBScale = 4096.0f;
 
 template <int PREC, int DT>
@@ -286,43 +297,29 @@ __device__ __forceinline__ void contract_chunk(const SOperands<PREC, DT>& so, in
 				}
 			}
 	}
-	else  // HDDM_LB_F32: v_mfma_f32_32x32x2_f32, K = 2 nodes per step
-	{
-		float B0[8], B1[8];
-#pragma unroll
-		for (int q = 0; q < 8; q++)
-		{
-			auto r = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, (float)b[2 * q]),
-				__builtin_bit_cast(uint32_t, (float)b[2 * q + 1]), false, false);
-			B0[q] = __builtin_bit_cast(float, (uint32_t)r[0]);
-			B1[q] = __builtin_bit_cast(float, (uint32_t)r[1]);
-		}
-#pragma unroll
-		for (int dt = 0; dt < DT; dt++)
-			if (dt < ndt)
-			{
-#pragma unroll
-				for (int q = 0; q < 8; q++)
-				{
-					acc[0][dt] = __builtin_amdgcn_mfma_f32_32x32x2f32(so.A[dt][q], B0[q], acc[0][dt], 0, 0, 0);
-					acc[1][dt] = __builtin_amdgcn_mfma_f32_32x32x2f32(so.A[dt][q], B1[q], acc[1][dt], 0, 0, 0);
-				}
-			}
-	}
 }
 
 } // namespace
 
 
 // Node chunk of the walk's basis buffer: 16 nodes (the MFMA K extent) for
-// the MFMA paths, 8 for the FP64 path, which contracts node by node and
-// whose smaller buffer leaves LDS for more resident waves.
+// the split MFMA paths; 32 for F32, whose buffer holds floats (half as many
+// contractions, each folding into FP64 once); 8 for the FP64 path, which
+// contracts node by node and whose smaller buffer leaves LDS for more
+// resident waves.
 template <int PREC>
-constexpr int chunk_bits() { return PREC == HDDM_LB_FP64 ? 3 : 4; }
+constexpr int chunk_bits() { return PREC == HDDM_LB_FP64 ? 3 : (PREC == HDDM_LB_F32 ? 5 : 4); }
 
-// Per-wave LDS stage: the path stack [PM][64] and the chunk buffer [2^CB][64], doubles.
-template <int PM, int CB>
-constexpr size_t wave_stage_bytes() { return (size_t)(PM + (1 << CB)) * kWave * 8; }
+// Bytes of one basis value in the chunk buffer.
+template <int PREC>
+constexpr int chunk_value_bytes() { return PREC == HDDM_LB_F32 ? 4 : 8; }
+
+// Per-wave LDS stage: the path stack [PM][64] doubles and the chunk buffer [2^CB][64].
+template <int PREC, int PM>
+constexpr size_t wave_stage_bytes()
+{
+	return (size_t)PM * kWave * 8 + ((size_t)1 << chunk_bits<PREC>()) * kWave * chunk_value_bytes<PREC>();
+}
 
 template <int PREC, int DT, int XB, int PM, int WALK>
 __global__ void __launch_bounds__(256)
@@ -335,14 +332,15 @@ __global__ void __launch_bounds__(256)
 	const int wave = uniform((int)(threadIdx.x >> 6));
 	// blocks per item: the workgroup's waves with LDS tiles, one otherwise
 	const int G = WALK == kWalkLds ? uniform((int)(blockDim.x >> 6)) : 1;
-	typedef double BT;  // basis values of the chunk stay FP64 until the operand split
+	// basis values of the chunk: FP64 until the operand split, F32 for the F32 path
+	typedef typename std::conditional<PREC == HDDM_LB_F32, float, double>::type BT;
 	char* const lbase = (char*)lds;
 	int64_t* const item_slot = (int64_t*)lbase;
 	Op* const tile_ops = (Op*)(lbase + kLdsHeader);
 	uint32_t* const tile_skip = (uint32_t*)(lbase + kLdsHeader + 2 * (size_t)kTileOps * sizeof(Op));
 	const size_t stage0 = WALK == kWalkLds ? kLdsHeader + kLdsTiles : 0;
 	constexpr int CB = chunk_bits<PREC>();
-	double* const Pl = (double*)(lbase + stage0 + (size_t)wave * wave_stage_bytes<PM, CB>());
+	double* const Pl = (double*)(lbase + stage0 + (size_t)wave * wave_stage_bytes<PREC, PM>());
 	BT* const Bl = (BT*)(Pl + PM * kWave);
 	const int64_t nitems = ldc(&a.hdr->nitems);
 
@@ -481,17 +479,18 @@ __global__ void __launch_bounds__(256)
 						for (int r = 0; r < 16; r++) acc[t][dt][r] = 0.0f;
 				if constexpr (PREC == HDDM_LB_F32)
 				{
-					// K = 2 nodes per MFMA step: steps whose two positions
-					// received no node are skipped (wave-uniform mask)
+					// K = 2 nodes per MFMA step, 16 steps over the 32-node
+					// chunk: steps whose two positions received no node are
+					// skipped (wave-uniform mask)
 #pragma unroll
-					for (int q = 0; q < 8; q++)
+					for (int q = 0; q < 16; q++)
 						if ((w.mask >> (2 * q)) & 3u)
 						{
-							const double b0 = Bl[(2 * q) * kWave + lane], b1 = Bl[(2 * q + 1) * kWave + lane];
-							Bl[(2 * q) * kWave + lane] = 0.0;
-							Bl[(2 * q + 1) * kWave + lane] = 0.0;
-							auto r = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, (float)b0),
-								__builtin_bit_cast(uint32_t, (float)b1), false, false);
+							const float b0 = Bl[(2 * q) * kWave + lane], b1 = Bl[(2 * q + 1) * kWave + lane];
+							Bl[(2 * q) * kWave + lane] = 0.0f;
+							Bl[(2 * q + 1) * kWave + lane] = 0.0f;
+							auto r = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, b0),
+								__builtin_bit_cast(uint32_t, b1), false, false);
 							const float B0 = __builtin_bit_cast(float, (uint32_t)r[0]);
 							const float B1 = __builtin_bit_cast(float, (uint32_t)r[1]);
 #pragma unroll
@@ -584,9 +583,13 @@ __global__ void __launch_bounds__(256)
 
 size_t sweep_lds_bytes(int prec, int waves, int depth, int walk)
 {
-	const bool c8 = prec == HDDM_LB_FP64;
-	const size_t stage = depth < 8 ? (c8 ? wave_stage_bytes<8, 3>() : wave_stage_bytes<8, 4>())
-		: (c8 ? wave_stage_bytes<16, 3>() : wave_stage_bytes<16, 4>());
+	size_t stage;
+	switch (prec)
+	{
+	case HDDM_LB_FP64: stage = depth < 8 ? wave_stage_bytes<HDDM_LB_FP64, 8>() : wave_stage_bytes<HDDM_LB_FP64, 16>(); break;
+	case HDDM_LB_F32: stage = depth < 8 ? wave_stage_bytes<HDDM_LB_F32, 8>() : wave_stage_bytes<HDDM_LB_F32, 16>(); break;
+	default: stage = depth < 8 ? wave_stage_bytes<HDDM_LB_F16X2, 8>() : wave_stage_bytes<HDDM_LB_F16X2, 16>(); break;
+	}
 	return (walk == kWalkLds ? kLdsHeader + kLdsTiles : 0) + (size_t)waves * stage;
 }
 

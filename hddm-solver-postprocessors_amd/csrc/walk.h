@@ -66,9 +66,9 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 	"s_cbranch_scc1 L_new" X "_%=\n"                                                   \
 	"s_bfe_u32 s89, " X1 ", %[cf]\n"                                                   \
 	"s_bitset1_b32 %[mask], s89\n"                                                     \
-	"s_lshl_b32 s89, s89, 9\n"                                                         \
-	"v_add_u32 " HW_E ", s89, %[bl]\n"                                                     \
-	"ds_write_b64 " HW_E ", " HW_A "\n"                                                    \
+	"s_lshl_b32 s89, s89, " HW_BSH "\n"                                                \
+	"v_add_u32 " HW_E ", s89, %[bl]\n"                                                 \
+	HW_BST                                                                             \
 	"L_adv" X "_%=:\n"                                                                 \
 	"s_add_u32 %[qb], %[qb], 16\n"                                                     \
 	"s_cmp_lt_u32 %[qb], %[qe]\n"                                                      \
@@ -140,7 +140,8 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 	"s94", "s95", "vcc", "scc", "m0", "memory"
 
 // One coordinate bank (D <= 16): x in v[64:95], temporaries v[96:103].
-template <int CB>
+// BF: the chunk buffer holds floats (rows of 256 bytes), else doubles (512).
+template <int CB, bool BF>
 __device__ __forceinline__ int walk_op_stream(const char* ob, uint32_t& qb, uint32_t qe, int chunk,
 	u4_t& opv, uint32_t& mask, uint32_t& nops, uint32_t& nprunes, uint32_t pl, uint32_t bl,
 	const d16_t& xv, double& val)
@@ -155,8 +156,24 @@ __device__ __forceinline__ int walk_op_stream(const char* ob, uint32_t& qb, uint
 #define HW_C "v[100:101]"
 #define HW_D "v102"
 #define HW_E "v103"
-	asm volatile(HDDM_WALK_ASM HDDM_WALK_OUTPUTS HDDM_WALK_INPUTS, "{v[64:95]}"(xv)
-		: HDDM_WALK_SCLOBBERS, "v96", "v97", "v98", "v99", "v100", "v101", "v102", "v103");
+	if constexpr (BF)
+	{
+#define HW_BSH "8"
+#define HW_BST "v_cvt_f32_f64 " HW_A0 ", " HW_A "\n" "ds_write_b32 " HW_E ", " HW_A0 "\n"
+		asm volatile(HDDM_WALK_ASM HDDM_WALK_OUTPUTS HDDM_WALK_INPUTS, "{v[64:95]}"(xv)
+			: HDDM_WALK_SCLOBBERS, "v96", "v97", "v98", "v99", "v100", "v101", "v102", "v103");
+#undef HW_BSH
+#undef HW_BST
+	}
+	else
+	{
+#define HW_BSH "9"
+#define HW_BST "ds_write_b64 " HW_E ", " HW_A "\n"
+		asm volatile(HDDM_WALK_ASM HDDM_WALK_OUTPUTS HDDM_WALK_INPUTS, "{v[64:95]}"(xv)
+			: HDDM_WALK_SCLOBBERS, "v96", "v97", "v98", "v99", "v100", "v101", "v102", "v103");
+#undef HW_BSH
+#undef HW_BST
+	}
 #undef HW_A
 #undef HW_A0
 #undef HW_A1
@@ -171,7 +188,7 @@ __device__ __forceinline__ int walk_op_stream(const char* ob, uint32_t& qb, uint
 
 // Two coordinate banks (16 < D <= 32): x in v[64:127] (the indexed move
 // reaches dims 16-31 through the same index), temporaries v[128:135].
-template <int CB>
+template <int CB, bool BF>
 __device__ __forceinline__ int walk_op_stream2(const char* ob, uint32_t& qb, uint32_t qe, int chunk,
 	u4_t& opv, uint32_t& mask, uint32_t& nops, uint32_t& nprunes, uint32_t pl, uint32_t bl,
 	const d16_t& xv0, const d16_t& xv1, double& val)
@@ -186,8 +203,24 @@ __device__ __forceinline__ int walk_op_stream2(const char* ob, uint32_t& qb, uin
 #define HW_C "v[132:133]"
 #define HW_D "v134"
 #define HW_E "v135"
-	asm volatile(HDDM_WALK_ASM HDDM_WALK_OUTPUTS HDDM_WALK_INPUTS, "{v[64:95]}"(xv0), "{v[96:127]}"(xv1)
-		: HDDM_WALK_SCLOBBERS, "v128", "v129", "v130", "v131", "v132", "v133", "v134", "v135");
+	if constexpr (BF)
+	{
+#define HW_BSH "8"
+#define HW_BST "v_cvt_f32_f64 " HW_A0 ", " HW_A "\n" "ds_write_b32 " HW_E ", " HW_A0 "\n"
+		asm volatile(HDDM_WALK_ASM HDDM_WALK_OUTPUTS HDDM_WALK_INPUTS, "{v[64:95]}"(xv0), "{v[96:127]}"(xv1)
+			: HDDM_WALK_SCLOBBERS, "v128", "v129", "v130", "v131", "v132", "v133", "v134", "v135");
+#undef HW_BSH
+#undef HW_BST
+	}
+	else
+	{
+#define HW_BSH "9"
+#define HW_BST "ds_write_b64 " HW_E ", " HW_A "\n"
+		asm volatile(HDDM_WALK_ASM HDDM_WALK_OUTPUTS HDDM_WALK_INPUTS, "{v[64:95]}"(xv0), "{v[96:127]}"(xv1)
+			: HDDM_WALK_SCLOBBERS, "v128", "v129", "v130", "v131", "v132", "v133", "v134", "v135");
+#undef HW_BSH
+#undef HW_BST
+	}
 #undef HW_A
 #undef HW_A0
 #undef HW_A1
@@ -294,12 +327,15 @@ struct Walker
 #pragma unroll
 			for (int i = 0; i < 16; i++) { xv0[i] = xb0[i]; xv1[i] = xb1[i]; }
 			double v = 0.0;
-			const uint32_t pl = lds_offset(P) + (uint32_t)lane * 8u, bl = lds_offset(B) + (uint32_t)lane * 8u;
+			const uint32_t pl = lds_offset(P) + (uint32_t)lane * 8u;
+			const uint32_t bl = lds_offset(B) + (uint32_t)lane * (uint32_t)sizeof(BT);
 			int n;
 			if constexpr (XB == 1)
-				n = walk_op_stream<CB>((const char*)gops, qb, qe, chunk, opv, mask, nops, nprunes, pl, bl, xv0, v);
+				n = walk_op_stream<CB, sizeof(BT) == 4>((const char*)gops, qb, qe, chunk, opv, mask, nops, nprunes,
+					pl, bl, xv0, v);
 			else
-				n = walk_op_stream2<CB>((const char*)gops, qb, qe, chunk, opv, mask, nops, nprunes, pl, bl, xv0, xv1, v);
+				n = walk_op_stream2<CB, sizeof(BT) == 4>((const char*)gops, qb, qe, chunk, opv, mask, nops, nprunes,
+					pl, bl, xv0, xv1, v);
 			op.dim = opv[0]; op.ctrl = opv[1]; op.ihi = opv[2]; op.njhi = opv[3];
 			q = (int)(qb / (uint32_t)sizeof(Op));
 			if (n < 0) return false;
