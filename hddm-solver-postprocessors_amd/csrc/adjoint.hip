@@ -213,6 +213,12 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 		int q = q0;
 		int top = -1;          // deepest live edge of the path
 		double cur = 1.0, base_prev = 1.0;
+		// An op without children is left by the very next op, so the edge
+		// sum and phi'/phi of such a top edge stay in registers; edges of
+		// ops with children go through the LDS path (R, op words).
+		bool top_leaf = false;
+		double r_top = 0.0, q_top = 0.0;
+		uint32_t dim_top = 0;
 		Op op = ldc_op(gops + q);
 		while (q < q1)
 		{
@@ -224,12 +230,23 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 			const uint32_t dim = op_dim(dimw);
 			const int t = (int)(ctrl & 15u);
 			// leave the edges at depths >= t
-			for (int tt = top; tt >= t; tt--)
+			int tt = top;
+			if (top_leaf && tt >= t)
+			{
+				if (wave_any(r_top != 0.0))
+				{
+					DX[dim_top * kWave + lane] += r_top * q_top;
+					if (tt > 0) R[(tt - 1) * kWave + lane] += r_top;
+				}
+				tt--;
+			}
+			for (; tt >= t; tt--)
 			{
 				const double r = R[tt * kWave + lane];
 				if (wave_any(r != 0.0)) leave_edge<XB>(edges[tt], r, xb0, xb1, xb2, xb3, DX, R, tt, lane);
 			}
 			top = t;
+			top_leaf = !(ctrl & kOpKids);
 			// parent product from the op's relation to its stream predecessor
 			const uint32_t rel = (ctrl >> kOpRelShift) & 3u;
 			double bt;
@@ -240,9 +257,17 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 			const double u = fma(xget<XB, true>(xb0, xb1, xb2, xb3, dim), iv, f64_from_high(op.njhi));
 			const double phi = 1.0 - fabs(u);
 			const double p = bt * fmax(phi, 0.0);
-			if (ctrl & kOpKids) P[(t + 1) * kWave + lane] = p;
-			R[t * kWave + lane] = 0.0;
-			if (lane == 0) edges[t] = u4_t{ dim, op.ihi, op.njhi, 0u };
+			if (top_leaf)
+			{
+				r_top = 0.0;
+				dim_top = dim;
+			}
+			else
+			{
+				P[(t + 1) * kWave + lane] = p;
+				R[t * kWave + lane] = 0.0;
+				if (lane == 0) edges[t] = u4_t{ dim, op.ihi, op.njhi, 0u };
+			}
 			base_prev = bt;
 			cur = p;
 			if (!wave_any(p != 0.0))
@@ -256,6 +281,15 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 			}
 			q++;
 			op = nop;
+			if (top_leaf)
+			{
+				// phi'/phi of the edge, for the next op that leaves it
+				const double ph = phi > 0.0 ? phi : 1.0;
+				double rc = __builtin_amdgcn_rcp(ph);
+				rc = fma(rc, fma(-ph, rc, 1.0), rc);
+				rc = fma(rc, fma(-ph, rc, 1.0), rc);
+				q_top = phi > 0.0 ? (u > 0.0 ? -iv : (u < 0.0 ? iv : 0.0)) * rc : 0.0;
+			}
 			if (ctrl & kOpLast)
 			{
 				const int n = (int)(ctrl >> kOpNodeShift);
@@ -263,10 +297,21 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 				double w = 0.0;
 #pragma unroll
 				for (int c = 0; c < 32; c++) w = fma(g[c], ldc(srow + c), w);
-				R[t * kWave + lane] = w * p;
+				if (top_leaf) r_top = w * p;
+				else R[t * kWave + lane] = w * p;
 			}
 		}
-		for (int tt = top; tt >= 0; tt--)
+		int tt = top;
+		if (top_leaf && tt >= 0)
+		{
+			if (wave_any(r_top != 0.0))
+			{
+				DX[dim_top * kWave + lane] += r_top * q_top;
+				if (tt > 0) R[(tt - 1) * kWave + lane] += r_top;
+			}
+			tt--;
+		}
+		for (; tt >= 0; tt--)
 		{
 			const double r = R[tt * kWave + lane];
 			if (wave_any(r != 0.0)) leave_edge<XB>(edges[tt], r, xb0, xb1, xb2, xb3, DX, R, tt, lane);
