@@ -45,6 +45,7 @@ struct AdjointArgs
 	double* dx;        // may be null
 	int64_t lddx;
 	double* ds;        // may be null; state s starts at states[s].ds_offset
+	int64_t ds_xcd;    // > 0: ds holds one copy of this many elements per XCD
 	int nstates;
 	int dim;
 	int dt_base;       // dof slab of 32 handled by this launch
@@ -64,6 +65,24 @@ template <int PM>
 constexpr size_t ds_stage_bytes() { return (size_t)(PM + kChunk) * kWave * 8; }
 
 namespace {
+
+// Copies of dS for the per-XCD accumulation (MI355X has 8 XCDs).
+constexpr int kXcdCopies = 8;
+// Largest per-XCD dS scratch, in elements of all copies together; beyond it
+// dS is accumulated in place.
+constexpr int64_t kXcdCopiesMaxElems = (int64_t)1 << 27;
+
+// out[i] = sum of the kXcdCopies copies of element i, in copy order.
+__global__ void sum_ds_copies_kernel(const double* __restrict__ part, int64_t n, double* __restrict__ out)
+{
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+	{
+		double v = 0.0;
+#pragma unroll
+		for (int c = 0; c < kXcdCopies; c++) v += part[c * n + i];
+		out[i] = v;
+	}
+}
 
 // A operands of the dS contraction for one 64-point block: the upstream
 // gradient in the v_mfma_f64_16x16x4f64 A layout, A[i][k] from lane i + 16k
@@ -98,7 +117,15 @@ struct GradTiles
 __device__ __forceinline__ void contract_ds(const AdjointArgs& a, const StateDesc* sd, const double* Bc,
 	uint32_t mask, int chunk, int lane, const GradTiles& gt, int dofs)
 {
-	double* const ds = a.ds + ldc(&sd->ds_offset);
+	// dS is added to the copy of the XCD the wave runs on (atomics stay in
+	// that XCD's L2); the copies are summed in a fixed order afterwards
+	double* ds = a.ds + ldc(&sd->ds_offset);
+	if (a.ds_xcd > 0)
+	{
+		uint32_t xcc;
+		asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc));
+		ds += (int64_t)(xcc & (kXcdCopies - 1)) * a.ds_xcd;
+	}
 	const int32_t* order = ldc(&sd->order);
 	const int col = lane & 15, kq = lane >> 4;
 	const int node = chunk * kChunk + col;
@@ -457,7 +484,25 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 	if ((rc = hddm_lb_surplus_offset(ctx, -1, nullptr, &ds_total))) return rc;
 	// outputs are accumulated: clear them with a kernel (never a memset, so
 	// the sequence stays capturable)
-	if (dsurplus && ds_total) HIP_TRY(launch_fill_u32((uint32_t*)dsurplus, 2 * ds_total, 0u, st));
+	// dS through per-XCD copies when they fit the scratch limit
+	const bool xcd_copies = dsurplus && ds_total && ds_total * kXcdCopies <= kXcdCopiesMaxElems;
+	if (xcd_copies)
+	{
+		Scratch& scr = ctx->scratch;
+		const size_t need = (size_t)ds_total * kXcdCopies;
+		if (need > scr.cap_dspart)
+		{
+			dev_free(scr.dspart);
+			scr.dspart = nullptr;
+			scr.cap_dspart = 0;
+			int arc = 0;
+			scr.dspart = (double*)dev_alloc(need * sizeof(double), arc);
+			if (arc) return arc;
+			scr.cap_dspart = need;
+		}
+		HIP_TRY(launch_fill_u32((uint32_t*)scr.dspart, 2 * (int64_t)need, 0u, st));
+	}
+	else if (dsurplus && ds_total) HIP_TRY(launch_fill_u32((uint32_t*)dsurplus, 2 * ds_total, 0u, st));
 	if (dx && npoints) HIP_TRY(launch_fill_u32((uint32_t*)dx, 2 * (npoints - 1) * lddx + 2 * ctx->dim, 0u, st));
 	if (npoints == 0 || (!dx && !dsurplus)) return HDDM_LB_SUCCESS;
 
@@ -507,7 +552,8 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 	std::memset(&a, 0, sizeof(a));
 	a.states = ctx->d_states; a.plans = sc.plans; a.cost_order = ctx->d_cost_order; a.hdr = sc.hdr;
 	a.perm = sc.perm; a.x = x; a.ldx = ldx; a.g = gbar; a.ldg = ldg; a.dx = dx; a.lddx = lddx;
-	a.ds = dsurplus; a.nstates = ns; a.dim = ctx->dim;
+	a.ds = xcd_copies ? sc.dspart : dsurplus; a.ds_xcd = xcd_copies ? ds_total : 0;
+	a.nstates = ns; a.dim = ctx->dim;
 	bool first = true;
 	for (int m = 0; m < 2 && kerns[m]; m++)
 		for (int sl = 0; sl < ctx->dt_needed; sl++)
@@ -518,5 +564,12 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 			void* params[] = { (void*)&a };
 			HIP_TRY(hipLaunchKernel(kerns[m], dim3(grids[m]), dim3(groups[m] * kWave), params, ldss[m], st));
 		}
+	if (xcd_copies)
+	{
+		int blocks = (int)std::min<int64_t>(4096, (ds_total + 255) / 256);
+		hipLaunchKernelGGL(sum_ds_copies_kernel, dim3(std::max(blocks, 1)), dim3(256), 0, st,
+			(const double*)sc.dspart, ds_total, dsurplus);
+		HIP_TRY(hipGetLastError());
+	}
 	return HDDM_LB_SUCCESS;
 }

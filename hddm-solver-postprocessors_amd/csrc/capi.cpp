@@ -52,10 +52,12 @@ Scratch::~Scratch() { release(); }
 
 void Scratch::release()
 {
-	for (void* p : { (void*)bkey, (void*)perm, (void*)hist, (void*)cursor, (void*)partial, (void*)plans, (void*)hdr })
+	for (void* p : { (void*)bkey, (void*)perm, (void*)hist, (void*)cursor, (void*)partial, (void*)dspart,
+			(void*)plans, (void*)hdr })
 		dev_free(p);
 	bkey = perm = hist = cursor = nullptr;
-	partial = nullptr;
+	partial = dspart = nullptr;
+	cap_dspart = 0;
 	plans = nullptr;
 	hdr = nullptr;
 	cap_bkey = cap_perm = cap_hist = cap_cursor = cap_partial = cap_plans = cap_hdr = 0;

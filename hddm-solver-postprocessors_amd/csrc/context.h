@@ -46,6 +46,7 @@ struct Scratch
 	uint32_t* hist = nullptr; size_t cap_hist = 0;
 	uint32_t* cursor = nullptr; size_t cap_cursor = 0;
 	double* partial = nullptr; size_t cap_partial = 0;
+	double* dspart = nullptr; size_t cap_dspart = 0;   // per-XCD copies of the adjoint's dS
 	StatePlan* plans = nullptr; size_t cap_plans = 0;
 	PlanHeader* hdr = nullptr; size_t cap_hdr = 0;
 	~Scratch();
