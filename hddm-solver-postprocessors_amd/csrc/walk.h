@@ -233,6 +233,52 @@ __device__ __forceinline__ int walk_op_stream2(const char* ob, uint32_t& qb, uin
 	return node;
 }
 
+// Four coordinate banks (32 < D <= 64): x in v[64:191], temporaries v[192:199].
+template <int CB, bool BF>
+__device__ __forceinline__ int walk_op_stream4(const char* ob, uint32_t& qb, uint32_t qe, int chunk,
+	u4_t& opv, uint32_t& mask, uint32_t& nops, uint32_t& nprunes, uint32_t pl, uint32_t bl,
+	const d16_t& xv0, const d16_t& xv1, const d16_t& xv2, const d16_t& xv3, double& val)
+{
+	int node;
+#define HW_A "v[192:193]"
+#define HW_A0 "v192"
+#define HW_A1 "v193"
+#define HW_B "v[194:195]"
+#define HW_B0 "v194"
+#define HW_B1 "v195"
+#define HW_C "v[196:197]"
+#define HW_D "v198"
+#define HW_E "v199"
+	if constexpr (BF)
+	{
+#define HW_BSH "8"
+#define HW_BST "v_cvt_f32_f64 " HW_A0 ", " HW_A "\n" "ds_write_b32 " HW_E ", " HW_A0 "\n"
+		asm volatile(HDDM_WALK_ASM HDDM_WALK_OUTPUTS HDDM_WALK_INPUTS, "{v[64:95]}"(xv0), "{v[96:127]}"(xv1), "{v[128:159]}"(xv2), "{v[160:191]}"(xv3)
+			: HDDM_WALK_SCLOBBERS, "v192", "v193", "v194", "v195", "v196", "v197", "v198", "v199");
+#undef HW_BSH
+#undef HW_BST
+	}
+	else
+	{
+#define HW_BSH "9"
+#define HW_BST "ds_write_b64 " HW_E ", " HW_A "\n"
+		asm volatile(HDDM_WALK_ASM HDDM_WALK_OUTPUTS HDDM_WALK_INPUTS, "{v[64:95]}"(xv0), "{v[96:127]}"(xv1), "{v[128:159]}"(xv2), "{v[160:191]}"(xv3)
+			: HDDM_WALK_SCLOBBERS, "v192", "v193", "v194", "v195", "v196", "v197", "v198", "v199");
+#undef HW_BSH
+#undef HW_BST
+	}
+#undef HW_A
+#undef HW_A0
+#undef HW_A1
+#undef HW_B
+#undef HW_B0
+#undef HW_B1
+#undef HW_C
+#undef HW_D
+#undef HW_E
+	return node;
+}
+
 #undef HDDM_WALK_OUTPUTS
 #undef HDDM_WALK_INPUTS
 #undef HDDM_WALK_SCLOBBERS
@@ -317,15 +363,15 @@ struct Walker
 		const double (&xb0)[16], const double (&xb1)[16], const double (&xb2)[16], const double (&xb3)[16],
 		int& node, double& val)
 	{
-		if constexpr (WALK == kWalkSmem && XB <= 2)
+		if constexpr (WALK == kWalkSmem)
 		{
 			uint32_t qb = (uint32_t)q * (uint32_t)sizeof(Op);
 			const uint32_t qe = (uint32_t)tend * (uint32_t)sizeof(Op);
 			if (qb >= qe) return false;
 			u4_t opv = { op.dim, op.ctrl, op.ihi, op.njhi };
-			d16_t xv0, xv1;
+			d16_t xv0, xv1, xv2, xv3;
 #pragma unroll
-			for (int i = 0; i < 16; i++) { xv0[i] = xb0[i]; xv1[i] = xb1[i]; }
+			for (int i = 0; i < 16; i++) { xv0[i] = xb0[i]; xv1[i] = xb1[i]; xv2[i] = xb2[i]; xv3[i] = xb3[i]; }
 			double v = 0.0;
 			const uint32_t pl = lds_offset(P) + (uint32_t)lane * 8u;
 			const uint32_t bl = lds_offset(B) + (uint32_t)lane * (uint32_t)sizeof(BT);
@@ -333,70 +379,18 @@ struct Walker
 			if constexpr (XB == 1)
 				n = walk_op_stream<CB, sizeof(BT) == 4>((const char*)gops, qb, qe, chunk, opv, mask, nops, nprunes,
 					pl, bl, xv0, v);
-			else
+			else if constexpr (XB == 2)
 				n = walk_op_stream2<CB, sizeof(BT) == 4>((const char*)gops, qb, qe, chunk, opv, mask, nops, nprunes,
 					pl, bl, xv0, xv1, v);
+			else
+				n = walk_op_stream4<CB, sizeof(BT) == 4>((const char*)gops, qb, qe, chunk, opv, mask, nops, nprunes,
+					pl, bl, xv0, xv1, xv2, xv3, v);
 			op.dim = opv[0]; op.ctrl = opv[1]; op.ihi = opv[2]; op.njhi = opv[3];
 			q = (int)(qb / (uint32_t)sizeof(Op));
 			if (n < 0) return false;
 			node = n;
 			val = v;
 			return true;
-		}
-		else if constexpr (WALK == kWalkSmem)
-		{
-			// Scalar-load walk on the precomputed relation and children bits:
-			// no depth compares, the stack is read only on a move up the trie
-			// and written only by ops with children.
-			while (q < tend)
-			{
-				const uint32_t dimw = op.dim, ctrl = op.ctrl;
-				// the stream ends with kOpPad sentinels, so q + 1 is in bounds;
-				// the load is issued first and stays in flight over the FP64
-				// chain (the scheduling fence keeps it at the top)
-				const Op nop = ldc_op(gops + q + 1);
-				__builtin_amdgcn_sched_barrier(0);
-				const int t = (int)(ctrl & 15u);
-				const uint32_t rel = (ctrl >> kOpRelShift) & 3u;
-				double bt;
-				if (rel == 0u) bt = cur;
-				else if (rel == 1u) bt = base_prev;
-				else
-				{
-					bt = P[t * kWave + lane];
-					// wait here, on the stack-read path only, so the other
-					// paths do not wait for the load above
-					__builtin_amdgcn_s_waitcnt(0xc07f);
-				}
-				const double tt = fma(xget<XB, XSEL>(xb0, xb1, xb2, xb3, op_dim(dimw)),
-					f64_from_high(op.ihi), f64_from_high(op.njhi));
-				const double p = bt * fmax(1.0 - fabs(tt), 0.0);
-				base_prev = bt;
-				cur = p;
-				if (ctrl & kOpKids) P[(t + 1) * kWave + lane] = p;
-				nops++;
-				if (!wave_any(p != 0.0))
-				{
-					nprunes++;
-					// a pruned leaf skips to the next op, which is already
-					// loaded; using nop on both paths also keeps its load at
-					// the top of the iteration, ahead of the FP64 chain
-					const int rs = (int)(dimw >> kOpSkipShift);
-					q += rs;
-					op = rs == 1 ? nop : ldc_op(gops + q);
-					continue;
-				}
-				q++;
-				op = nop;
-				if (ctrl & kOpLast)
-				{
-					const int n = (int)(ctrl >> kOpNodeShift);
-					if ((n >> CB) != chunk) { node = n; val = p; return true; }
-					B[(n & kCM) * kWave + lane] = (BT)p;
-					mask |= 1u << (n & kCM);
-				}
-			}
-			return false;
 		}
 		else
 		{
