@@ -58,6 +58,58 @@ def time_path(ctx, x, y, prec, steps, warmup):
     return (t1 - t0) * 1e3 / steps, tot / max(n, 1), mn, mx
 
 
+def time_multistate(args, prec, which):
+    """Multistate forwards in one launch, timed like the headline (events
+    around whole calls, inputs resident):
+    d20x12: 12 states, D=20, adaptive grids (max level 6) with node counts
+            log-uniform in [2k, 200k], dofs per state from {16, 32, 64},
+            262,144 points with seeded state ids;
+    d59x4:  4 states, D=59, regular grids sum(l-1) <= 3 (281,077 nodes each),
+            dofs=128, 2^20 points with seeded state ids.
+    The unit counts every output element: sum over points of the dofs of the
+    point's state."""
+    import numpy as np
+    import torch
+    import hddm_lb
+    ctx = hddm_lb.Context(0)
+    rng = np.random.default_rng(2024)
+    if which == "d20x12":
+        nnos = np.exp(rng.uniform(np.log(2000), np.log(200000), size=12)).astype(int)
+        dofs = rng.choice([16, 32, 64], size=12)
+        specs = [(20, 1, 6, 0.3, 300 + s, int(nnos[s]), int(dofs[s])) for s in range(12)]
+        n, dim, steps = 262144, 20, max(2, args.steps // 2)
+    else:
+        specs = [(59, 3, 4, 0.0, 500 + s, 0, 128) for s in range(4)]
+        n, dim, steps = 1 << 20, 59, 2
+    nno_tot, dofs_of = 0, []
+    for s, (d, start, maxl, p, seed, target, nd) in enumerate(specs):
+        ii, jj, lv, _ = hddm_lb.generate_grid(d, start, maxl, p, seed, target)
+        S = hddm_lb.synthetic_surplus(lv, nd, seed + 100)
+        ctx.upload_state(s, ii, jj, S)
+        nno_tot += ii.shape[0]
+        dofs_of.append(nd)
+    X = hddm_lb.synthetic_points(n, dim, 77)
+    w = rng.uniform(0.5, 1.5, size=len(specs))
+    sid = rng.choice(len(specs), size=n, p=w / w.sum()).astype(np.int32)
+    work = float(np.asarray(dofs_of)[sid].sum())
+    x = torch.from_numpy(X).cuda()
+    sd = torch.from_numpy(sid).cuda()
+    y = torch.empty((n, max(dofs_of)), dtype=torch.float64, device="cuda")
+    res = {"workload": "%d states, D=%d, %d nodes in all, %d points" % (len(specs), dim, nno_tot, n)}
+    for name, pr in (("mixed", prec), ("fp64", hddm_lb.FP64)):
+        ctx.forward(x, y, sd, pr)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.forward(x, y, sd, pr)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        res[name] = {"ms_per_step": ms, "value": work / (ms * 1e-3)}
+    res["unit"] = "point·dof/s"
+    ctx.close()
+    return res
+
+
 def time_adjoint(args, prec):
     """Forward + adjoint on the differentiated workload: D=20 adaptive grid of
     ~100k nodes (max level 6), dofs=32, 65,536 points in [0.01, 0.99]^20,
@@ -152,6 +204,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and error check")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the forward+adjoint measurement on the differentiated workload")
+    ap.add_argument("--no-multistate", action="store_true",
+                    help="skip the two multistate measurements (D=20 x 12 states, D=59 x 4 states)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device 3 times (calibrates profiler byte counters)")
     args = ap.parse_args()
@@ -253,6 +307,8 @@ def main():
                            "note": "x and y in pageable host memory, copies included"}
     if not args.no_adjoint:
         out["forward_adjoint"] = time_adjoint(args, precs[args.precision])
+    if not args.no_multistate:
+        out["multistate"] = {k: time_multistate(args, precs[args.precision], k) for k in ("d20x12", "d59x4")}
     if not args.no_cpu:
         cb, errs, port_err = cpu_baseline(ii, jj, S, X, ygpu, args.cpu_seconds)
         out["cpu_baseline"] = cb
