@@ -62,23 +62,37 @@ __global__ void __launch_bounds__(1024) scan_plan_kernel(const uint32_t* __restr
 	const StateDesc* __restrict__ states, const int32_t* __restrict__ cost_order,
 	StatePlan* __restrict__ plans, PlanHeader* __restrict__ hdr, int group)
 {
-	__shared__ uint32_t part[1024];
+	// exclusive scan of the histogram into the cursors, 1024 buckets per
+	// step with coalesced reads: a wave-level scan, then the wave totals
+	__shared__ uint32_t wsum[16];
 	const int t = threadIdx.x;
-	const int64_t per = (nbuckets + 1023) / 1024;
-	const int64_t b0 = (int64_t)t * per, b1 = b0 + per < nbuckets ? b0 + per : nbuckets;
-	uint32_t sum = 0;
-	for (int64_t b = b0; b < b1; b++) sum += hist[b];
-	part[t] = sum;
-	__syncthreads();
-	for (int off = 1; off < 1024; off <<= 1)
+	const int lane = t & 63, w = t >> 6;
+	uint32_t carry = 0;
+	for (int64_t base = 0; base < nbuckets; base += 1024)
 	{
-		uint32_t v = t >= off ? part[t - off] : 0;
+		const int64_t b = base + t;
+		const uint32_t v = b < nbuckets ? hist[b] : 0u;
+		uint32_t x = v;
+#pragma unroll
+		for (int off = 1; off < 64; off <<= 1)
+		{
+			const uint32_t y = __shfl_up(x, off, 64);
+			if (lane >= off) x += y;
+		}
+		if (lane == 63) wsum[w] = x;
 		__syncthreads();
-		part[t] += v;
+		uint32_t wpre = 0, total = 0;
+#pragma unroll
+		for (int i = 0; i < 16; i++)
+		{
+			const uint32_t ws = wsum[i];
+			wpre += i < w ? ws : 0u;
+			total += ws;
+		}
+		if (b < nbuckets) cursor[b] = carry + wpre + x - v;
+		carry += total;
 		__syncthreads();
 	}
-	uint32_t run = part[t] - sum;  // exclusive prefix of this thread's range
-	for (int64_t b = b0; b < b1; b++) { uint32_t c = hist[b]; cursor[b] = run; run += c; }
 	__threadfence_block();
 	__syncthreads();
 
