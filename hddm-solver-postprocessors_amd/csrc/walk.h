@@ -1,6 +1,7 @@
-// The per-wave prefix-trie walk shared by the forward sweep and the adjoint
-// (gfx950 only): the hand-scheduled scalar-load walk for one or two
-// coordinate banks and the walk state (path stack, chunk buffer, op position).
+// The per-wave prefix-trie walk shared by the forward sweep and the adjoint's
+// dS kernel (gfx950 only): the hand-scheduled scalar-load walk for one, two or
+// four coordinate banks and the walk state (path stack, chunk buffer, op
+// position).
 #pragma once
 
 #include "sweep_common.h"
@@ -15,10 +16,16 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 	return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
 }
 
-// Hand-scheduled scalar-load walk for points whose coordinates fit one or two
-// banks (D <= 32). Walks ops from byte offset qb until the first surviving node of
-// a chunk other than `chunk` (returned in node and val, not stored; qb is
-// then past its op) or until qb reaches qe (node = -1).
+// Hand-scheduled scalar-load walk (D <= 64). Walks ops from byte offset qb
+// until the first surviving node of a chunk other than `chunk` (returned in
+// node and val, not stored; qb is then past its op) or until qb reaches qe
+// (node = -1).
+// Bounds, shown on the host side: every relative skip is >= 1, so qb grows
+// each step and the loop ends; a successor, the op two ahead and a skip
+// target lie at most two ops past the last op of the stream, inside its
+// kOpPad sentinels; the depth t <= 14 and P has a row for depth t + 1 (the
+// stage holds 8 rows when every node has at most 7 factors, else 16); the
+// register index 2 * dim stays inside the pinned coordinate banks.
 // Per op: the parent product comes from the LDS path stack (P[t],
 // lane-strided rows of 512 bytes at pl) and the op's product is stored as
 // P[t+1], so P[k] is the product of the first k factors of the current trie
@@ -34,9 +41,11 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 // code: the common path takes one branch per three ops, prunes and the
 // emission of a node of a new chunk branch out. The op handed back is in
 // s[76:79].
-// Fixed registers: the FP64 i as s[94:95] (low word 0), x from v64 (one or
-// two banks), temporaries s88-s90 and eight VGPRs after x (HW_*; the low word
-// of -j is kept 0 in HW_B0); the indexed moves set m0.
+// Fixed registers: the FP64 i as s[94:95] (low word 0), x from v64 (one,
+// two or four banks of 32 VGPRs), temporaries s88-s90 and eight VGPRs after
+// x (HW_*; the low word of -j is kept 0 in HW_B0); the indexed moves set m0.
+// The chunk buffer holds doubles (rows of 512 bytes) or, for the F32 path,
+// floats (rows of 256 bytes): HW_BSH and HW_BST.
 #define HDDM_WALK_MAIN(X, X0, X1, X2, X3, ZQ, NEXT)                                    \
 	"L_top" X "_%=:\n"                                                                 \
 	"s_lshl_b32 s88, " X1 ", 9\n"                                                      \
