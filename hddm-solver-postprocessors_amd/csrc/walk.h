@@ -310,7 +310,7 @@ struct Walker
 	static constexpr int kC = 1 << CB;        // nodes per chunk
 	static constexpr int kCM = kC - 1;
 	double* P;          // [PM][64] in LDS
-	BT* B;              // [16][64] in LDS: basis values of the current chunk
+	BT* B;              // [chunk nodes][64] in LDS: basis values of the current chunk
 	double base_prev;   // P[last_t]
 	double cur;         // P[last_t + 1]
 	int last_t;
