@@ -8,7 +8,7 @@ p=0.3 from level sum <= 2, max level 7, stopped at 60,018 nodes), dofs=32,
 A step is one forward over the whole batch. The headline path is the
 mixed-precision MFMA sweep (basis formed in FP64, contraction on
 v_mfma_f32_32x32x2f32 with FP32 accumulation folded into FP64 after every
-16-node chunk); the plain FP64 GPU sweep (same walk, FP64 contraction) is
+32-node chunk); the plain FP64 GPU sweep (same walk, FP64 contraction) is
 timed beside it. Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--precision f32]
@@ -24,8 +24,8 @@ sys.path.insert(0, os.path.join(ROOT, "hddm-solver-postprocessors_amd"))
 
 METRIC = "point·dof evals/sec on 1 MI355X, mixed/MFMA vs FP64 plain at ≤1e-6 rel err"
 PEAK_TFLOPS = {"f16x2": 2500.0, "bf16x3": 2500.0, "f32": 157.3, "fp64": 78.6}
-PRODUCTS = {"f16x2": 3, "bf16x3": 6, "f32": 1}   # MFMA products per chunk and dof tile
-kChunkNodes = 16
+# FLOP of one MFMA instruction of each path's contraction
+MFMA_FLOPS = {"f32": 2 * 32 * 32 * 2, "f16x2": 2 * 32 * 32 * 16, "bf16x3": 2 * 32 * 32 * 16}
 DTYPE = {"f16x2": "f16", "bf16x3": "bf16", "f32": "f32", "fp64": "f64"}
 
 
@@ -242,18 +242,23 @@ def main():
         ygpu[name] = y.cpu().numpy()
     ms, sweep_ms, _, _ = res[args.precision]
     # executed work of one sweep (outside the timed steps): trie ops evaluated
-    # per 64-point block and 16-node chunks contracted
+    # per 64-point block, chunks contracted and MFMA instructions issued
     ctx.set_option(hddm_lb.OPT_STATS, 1)
     ctx.forward(x, y, None, precs[args.precision])
-    nops_ex, nprune, nchunk, nitem = ctx.last_stats()
+    nops_ex, nprune, nchunk, nitem, nmfma = ctx.last_stats(5)
     ctx.set_option(hddm_lb.OPT_STATS, 0)
 
     # algorithmic work of one launch: dense contraction 2*nno*dofs plus
     # 4 flops per active factor, per point
     flops = N * (2.0 * nno * dofs + 4.0 * factors)
-    # nodes per contracted chunk: FP64 8 (VALU), F32 32, the split MFMA paths 16
-    cnodes = {"fp64": 8, "f32": 32}.get(args.precision, kChunkNodes)
-    chunk_tflops = nchunk * 2.0 * cnodes * 64 * 32 * PRODUCTS.get(args.precision, 1) / (sweep_ms * 1e-3) / 1e12
+    # contraction work executed: the MFMA instructions issued (empty K-steps
+    # and all-zero 32-point halves are skipped); on the FP64 path (VALU) the
+    # 8-node chunks counted whole
+    if args.precision in MFMA_FLOPS:
+        exec_flops = nmfma * float(MFMA_FLOPS[args.precision])
+    else:
+        exec_flops = nchunk * 2.0 * 8 * 64 * 32
+    chunk_tflops = exec_flops / (sweep_ms * 1e-3) / 1e12
     achieved = flops / (sweep_ms * 1e-3) / 1e12
     peak = PEAK_TFLOPS[args.precision]
     traffic = None
@@ -285,8 +290,8 @@ def main():
                      "frac_executed": chunk_tflops / peak,
                      "note": "achieved counts the dense contraction over all nodes; the trie walk "
                              "skips nodes whose basis is zero on a whole 64-point block, so frac "
-                             "can exceed 1. frac_executed: MFMA work of the contracted chunks "
-                             "(whole chunks counted) over the same peak"},
+                             "can exceed 1. frac_executed: FLOP of the MFMA instructions the "
+                             "contraction issued over the same peak"},
         "fp64_plain": {"value": N * dofs / (res["fp64"][0] * 1e-3), "ms_per_step": res["fp64"][0],
                        "kernel_ms": res["fp64"][1],
                        "roofline_frac": flops / (res["fp64"][1] * 1e-3) / 1e12 / PEAK_TFLOPS["fp64"]},
@@ -294,7 +299,7 @@ def main():
         "executed": {"walk_ops_per_block": nops_ex / max(nitem, 1),
                      "pruned_runs_per_block": nprune / max(nitem, 1),
                      "chunks_per_block": nchunk / max(nitem, 1), "blocks": nitem,
-                     "mfma_tflops": chunk_tflops},
+                     "mfma_per_block": nmfma / max(nitem, 1), "mfma_tflops": chunk_tflops},
     }
     # the same forward through host buffers (what the plugin's interpolate()
     # uses): x in and y out over PCIe, synchronous

@@ -454,10 +454,10 @@ int enqueue_forward(hddm_lb_ctx* ctx, Scratch& sc, int64_t npoints, const double
 		if (!ctx->d_stats)
 		{
 			int rc = 0;
-			ctx->d_stats = (unsigned long long*)dev_alloc(4 * sizeof(unsigned long long), rc);
+			ctx->d_stats = (unsigned long long*)dev_alloc(hddm_lb_ctx::kStatCount * sizeof(unsigned long long), rc);
 			if (rc) return rc;
 		}
-		HIP_TRY(launch_fill_u32((uint32_t*)ctx->d_stats, 8, 0u, st));
+		HIP_TRY(launch_fill_u32((uint32_t*)ctx->d_stats, 2 * hddm_lb_ctx::kStatCount, 0u, st));
 		a.stats = ctx->d_stats;
 	}
 	const int nslab = (ctx->dt_needed + cfg.slab - 1) / cfg.slab;
@@ -514,14 +514,21 @@ extern "C" int hddm_lb_forward(hddm_lb_ctx* ctx, int64_t npoints,
 	return enqueue_forward(ctx, ctx->scratch, npoints, x, ldx, state_ids, y, ldy, precision, (hipStream_t)stream, cfg, ctx->opt_timing);
 }
 
-extern "C" int hddm_lb_last_stats(hddm_lb_ctx* ctx, uint64_t* stats)
+extern "C" int hddm_lb_last_stats_n(hddm_lb_ctx* ctx, uint64_t* stats, int n)
 {
 	if (!ctx || !stats) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "last_stats: NULL argument");
+	if (n < 1 || n > hddm_lb_ctx::kStatCount)
+		return set_error(HDDM_LB_ERROR_INVALID_VALUE, "last_stats: %d counters requested, 1 to %d available", n, hddm_lb_ctx::kStatCount);
 	if (!ctx->d_stats) return set_error(HDDM_LB_ERROR_STATE, "last_stats: HDDM_LB_OPT_STATS was not set for a forward");
 	DeviceGuard g(ctx->device);
 	HIP_TRY(hipDeviceSynchronize());
-	HIP_TRY(hipMemcpy(stats, ctx->d_stats, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost));
+	HIP_TRY(hipMemcpy(stats, ctx->d_stats, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToHost));
 	return HDDM_LB_SUCCESS;
+}
+
+extern "C" int hddm_lb_last_stats(hddm_lb_ctx* ctx, uint64_t* stats)
+{
+	return hddm_lb_last_stats_n(ctx, stats, 4);
 }
 
 extern "C" double hddm_lb_last_sweep_ms(hddm_lb_ctx* ctx)

@@ -103,6 +103,8 @@ struct hddm_lb_ctx
 
 	// measurement
 	bool opt_stats = false;
+	// walk ops, prunes, chunks contracted, work items, MFMA instructions issued
+	static constexpr int kStatCount = 5;
 	unsigned long long* d_stats = nullptr;
 	// ring of event pairs bracketing the sweep kernels of timed forwards
 	static constexpr int kTimingRing = 256;

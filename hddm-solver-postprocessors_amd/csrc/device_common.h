@@ -45,18 +45,26 @@ __device__ __forceinline__ bool wave_any(bool pred)
 // (level 1, dim 0), (level 1, dim 1), ..., (level 2, dim 0), ...
 // Bit (l, d) is the l-th binary digit of x_d, i.e. which half, quarter, ...
 // of [0, 1] holds x_d. Points adjacent in this order share supports.
+// Each coordinate is read once: with L the deepest level used, the digit of
+// level l is bit L - l of floor(x_d 2^L) (scaling by 2^L is exact, and
+// floor(floor(y) / 2^k) = floor(y / 2^k) for y >= 0).
 __device__ __forceinline__ uint64_t morton_key(const double* x, int dim, int bits)
 {
+	if (bits <= 0 || dim <= 0) return 0;
+	int L = (bits + dim - 1) / dim;
+	if (L > 30) L = 30;
+	const int nd = dim < bits ? dim : bits;
+	const double scale = (double)(1u << L);
 	uint64_t key = 0;
-	int got = 0;
-	for (int lev = 1; got < bits && lev <= 30; lev++)
-		for (int d = 0; d < dim && got < bits; d++, got++)
-		{
-			double v = x[d];
-			v = v < 0.0 ? 0.0 : (v >= 1.0 ? 0.9999999999999999 : v);
-			uint64_t digit = (uint64_t)(int64_t)(v * (double)(1u << lev)) & 1u;
-			key = (key << 1) | digit;
-		}
+	for (int d = 0; d < nd; d++)
+	{
+		double v = x[d];
+		v = v < 0.0 ? 0.0 : (v >= 1.0 ? 0.9999999999999999 : v);
+		const uint32_t q = (uint32_t)(v * scale);
+		// bit (lev, d) sits at position bits - 1 - ((lev - 1) dim + d)
+		for (int lev = 1, pos = d; lev <= L && pos < bits; lev++, pos += dim)
+			key |= (uint64_t)((q >> (L - lev)) & 1u) << (bits - 1 - pos);
+	}
 	return key;
 }
 

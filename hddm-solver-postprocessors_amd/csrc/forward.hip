@@ -15,12 +15,13 @@
 //                    cost order)
 //   4. scatter       counting-sort scatter: perm[sorted position] = point
 //   5. sweep         persistent workgroups of G waves pop items; one point per
-//                    lane, one block per wave. The workgroup streams the
-//                    segment's op stream through double-buffered LDS tiles;
-//                    each wave walks the prefix trie of nodes, pruning every
-//                    subtree whose prefix product is zero on all 64 lanes, and
-//                    contracts the surviving nodes against the surplus
-//                    (FP64 VALU, or MFMA on 16-node chunks with split operands)
+//                    lane, one block per wave. Each wave reads the segment's
+//                    op stream through scalar loads (or the workgroup streams
+//                    it through double-buffered LDS tiles), walks the prefix
+//                    trie of nodes, pruning every subtree whose prefix product
+//                    is zero on all 64 lanes, and contracts the surviving
+//                    nodes against the surplus (FP64 VALU on 8-node chunks,
+//                    MFMA on 32-node F32 or 16-node split-operand chunks)
 //   6. reduce        sums node-segment partials of split states (fixed order)
 #include "sweep_common.h"
 #include <type_traits>
@@ -41,18 +42,42 @@ __global__ void fill_u32_kernel(uint32_t* p, int64_t n, uint32_t v)
 		p[e] = v;
 }
 
-__global__ void bucket_key_kernel(int64_t n, const double* __restrict__ x, int64_t ldx,
+// Bucket bits never exceed 16, so a key reads at most the first 16
+// coordinates of a point.
+constexpr int kKeyDims = 16;
+constexpr int kKeyBlock = 256;
+
+// One point per thread. The workgroup first stages the coordinates its keys
+// read (the first min(dim, bbits) of each of its 256 points) through LDS with
+// coalesced loads: read lane by lane, rows of 128 bytes are re-fetched from
+// HBM once per coordinate when the many resident waves evict them from L2.
+__global__ void __launch_bounds__(kKeyBlock) bucket_key_kernel(int64_t n, const double* __restrict__ x, int64_t ldx,
 	const int32_t* __restrict__ sid, const uint8_t* __restrict__ state_ok, int nstates,
 	int dim, int bbits, uint32_t* __restrict__ bkey, uint32_t* __restrict__ hist)
 {
-	for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x)
+	__shared__ double rows[kKeyBlock * (kKeyDims + 1)];
+	const int tid = threadIdx.x;
+	const int nd = bbits < dim ? bbits : dim;
+	for (int64_t base = (int64_t)blockIdx.x * kKeyBlock; base < n; base += (int64_t)gridDim.x * kKeyBlock)
 	{
-		int s = sid ? sid[p] : 0;
-		if (s < 0 || s >= nstates || !state_ok[s]) s = nstates;  // never planned: row left untouched
-		uint32_t key = ((uint32_t)s << bbits);
-		if (bbits) key |= (uint32_t)morton_key(x + p * ldx, dim, bbits);
-		bkey[p] = key;
-		atomicAdd(&hist[key], 1u);
+		const int npt = n - base < kKeyBlock ? (int)(n - base) : kKeyBlock;
+		for (int k = tid; k < npt * nd; k += kKeyBlock)
+		{
+			const int pt = k / nd, d = k - pt * nd;
+			rows[pt * (kKeyDims + 1) + d] = x[(base + pt) * ldx + d];
+		}
+		__syncthreads();
+		if (tid < npt)
+		{
+			const int64_t p = base + tid;
+			int s = sid ? sid[p] : 0;
+			if (s < 0 || s >= nstates || !state_ok[s]) s = nstates;  // never planned: row left untouched
+			uint32_t key = ((uint32_t)s << bbits);
+			if (bbits) key |= (uint32_t)morton_key(rows + tid * (kKeyDims + 1), dim, bbits);
+			bkey[p] = key;
+			atomicAdd(&hist[key], 1u);
+		}
+		__syncthreads();
 	}
 }
 
@@ -410,7 +435,7 @@ __global__ void __launch_bounds__(256)
 
 		Walker<XB, BT, WALK, PM, PREC == HDDM_LB_FP64, CB> w;
 		w.init(q0, np > 0 ? q1 : q0, Pl, Bl, lane, gops, gskip);
-		uint32_t stat_chunks = 0;
+		uint32_t stat_chunks = 0, stat_mfma = 0;
 		const bool split = nseg > 1;
 		double* const part = a.partial + ldc(&a.plans[it.s].partial_base);
 		// one pass over [q0, q1) for scalar-load walks; tiles of kTileOps otherwise
@@ -483,7 +508,28 @@ __global__ void __launch_bounds__(256)
 					for (int r = 0; r < 16; r++) acc64[t][dt][r] = 0.0;
 			SOperands<PREC, DT> so;
 			auto contract = [&]() {
-				if (a.stats) stat_chunks++;
+				if (a.stats)
+				{
+					// counted in a pass of its own (statistics mode only) so
+					// that the contraction below stays as it runs otherwise
+					stat_chunks++;
+					if constexpr (PREC == HDDM_LB_F32)
+					{
+#pragma unroll 1
+						for (int q = 0; q < 16; q++)
+							if ((w.mask >> (2 * q)) & 3u)
+							{
+								auto r = __builtin_amdgcn_permlane32_swap(
+									__builtin_bit_cast(uint32_t, Bl[(2 * q) * kWave + lane]),
+									__builtin_bit_cast(uint32_t, Bl[(2 * q + 1) * kWave + lane]), false, false);
+								const uint32_t n0 = __builtin_amdgcn_ballot_w64(__builtin_bit_cast(float, (uint32_t)r[0]) != 0.0f) != 0;
+								const uint32_t n1 = __builtin_amdgcn_ballot_w64(__builtin_bit_cast(float, (uint32_t)r[1]) != 0.0f) != 0;
+								stat_mfma += (n0 + n1) * (uint32_t)ndt;
+							}
+					}
+					else
+						stat_mfma += (PREC == HDDM_LB_F16X2 ? 6u : 12u) * (uint32_t)ndt;
+				}
 				f16_t acc[2][DT];
 #pragma unroll
 				for (int t = 0; t < 2; t++)
@@ -507,12 +553,15 @@ __global__ void __launch_bounds__(256)
 								__builtin_bit_cast(uint32_t, b1), false, false);
 							const float B0 = __builtin_bit_cast(float, (uint32_t)r[0]);
 							const float B1 = __builtin_bit_cast(float, (uint32_t)r[1]);
+							// a 32-point half where both nodes are zero adds nothing
+							const bool h0 = __builtin_amdgcn_ballot_w64(B0 != 0.0f) != 0;
+							const bool h1 = __builtin_amdgcn_ballot_w64(B1 != 0.0f) != 0;
 #pragma unroll
 							for (int dt = 0; dt < DT; dt++)
 								if (dt < ndt)
 								{
-									acc[0][dt] = __builtin_amdgcn_mfma_f32_32x32x2f32(so.A[dt][q], B0, acc[0][dt], 0, 0, 0);
-									acc[1][dt] = __builtin_amdgcn_mfma_f32_32x32x2f32(so.A[dt][q], B1, acc[1][dt], 0, 0, 0);
+									if (h0) acc[0][dt] = __builtin_amdgcn_mfma_f32_32x32x2f32(so.A[dt][q], B0, acc[0][dt], 0, 0, 0);
+									if (h1) acc[1][dt] = __builtin_amdgcn_mfma_f32_32x32x2f32(so.A[dt][q], B1, acc[1][dt], 0, 0, 0);
 								}
 						}
 				}
@@ -591,6 +640,7 @@ __global__ void __launch_bounds__(256)
 			atomicAdd(&a.stats[1], (unsigned long long)w.nprunes);
 			atomicAdd(&a.stats[2], (unsigned long long)stat_chunks);
 			atomicAdd(&a.stats[3], 1ull);
+			atomicAdd(&a.stats[4], (unsigned long long)stat_mfma);
 		}
 	}
 }
@@ -698,10 +748,11 @@ hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st)
 hipError_t launch_bucket_key(int64_t n, const double* x, int64_t ldx, const int32_t* sid,
 	const uint8_t* state_ok, int nstates, int dim, int bbits, uint32_t* bkey, uint32_t* hist, hipStream_t st)
 {
-	int blocks = (int)((n + 255) / 256);
+	if (bbits > kKeyDims) return hipErrorInvalidValue;
+	int blocks = (int)((n + kKeyBlock - 1) / kKeyBlock);
 	if (blocks > 8192) blocks = 8192;
 	if (blocks < 1) blocks = 1;
-	hipLaunchKernelGGL(bucket_key_kernel, dim3(blocks), dim3(256), 0, st, n, x, ldx, sid, state_ok, nstates, dim, bbits, bkey, hist);
+	hipLaunchKernelGGL(bucket_key_kernel, dim3(blocks), dim3(kKeyBlock), 0, st, n, x, ldx, sid, state_ok, nstates, dim, bbits, bkey, hist);
 	return hipGetLastError();
 }
 

@@ -82,6 +82,7 @@ _SIGS = {
     "hddm_lb_plan_destroy": (_i32, [_vp]),
     "hddm_lb_last_sweep_ms": (_dbl, [_vp]),
     "hddm_lb_last_stats": (_i32, [_vp, _vp]),
+    "hddm_lb_last_stats_n": (_i32, [_vp, _vp, _i32]),
     "hddm_lb_sweep_time_summary": (_i32, [_vp, _vp, _vp, _vp, _vp]),
     "hddm_lb_plt_read": (_i32, [_c.c_char_p, _i32, _c.POINTER(_i32), _c.POINTER(_i32), _c.POINTER(_i32),
                                 _c.POINTER(_i32), _c.POINTER(_I32P), _c.POINTER(_I32P), _c.POINTER(_DP)]),
@@ -333,10 +334,11 @@ class Context:
         _check(lib.hddm_lb_sweep_time_summary(self._h, ctypes.byref(n), ctypes.byref(tot), ctypes.byref(mn), ctypes.byref(mx)))
         return n.value, tot.value, mn.value, mx.value
 
-    def last_stats(self):
-        """(ops evaluated, subtrees pruned, chunks contracted, work items) of the last forward."""
-        st = np.zeros(4, np.uint64)
-        _check(lib.hddm_lb_last_stats(self._h, _ptr(st)))
+    def last_stats(self, n=4):
+        """(ops evaluated, subtrees pruned, chunks contracted, work items[, MFMA
+        instructions issued]) of the last forward: the first n (at most 5)."""
+        st = np.zeros(n, np.uint64)
+        _check(lib.hddm_lb_last_stats_n(self._h, _ptr(st), n))
         return tuple(int(v) for v in st)
 
     def plan(self, x, y, state_ids=None, precision=FP64, stream=None):

@@ -144,9 +144,14 @@ double hddm_lb_last_sweep_ms(hddm_lb_ctx* ctx);
 int hddm_lb_sweep_time_summary(hddm_lb_ctx* ctx, int64_t* ncalls, double* total_ms,
 	double* min_ms, double* max_ms);
 /* With HDDM_LB_OPT_STATS set: counts of the last forward, summed over all
- * work items: stats[0] ops evaluated, [1] subtrees pruned, [2] 16-node chunks
- * contracted, [3] work items. Waits for the forward to finish. */
+ * work items: stats[0] ops evaluated, [1] subtrees pruned, [2] chunks
+ * contracted (8 nodes on the FP64 path, 32 on F32, 16 on F16X2 / BF16X3),
+ * [3] work items. Waits for the forward to finish. */
 int hddm_lb_last_stats(hddm_lb_ctx* ctx, uint64_t* stats);
+/* The first n (1 to 5) of the same counters; stats[4] is the number of MFMA
+ * instructions the contraction issued (v_mfma_f32_32x32x2f32 on F32,
+ * v_mfma_f32_32x32x16_{f16,bf16} on the split paths, 0 on FP64). */
+int hddm_lb_last_stats_n(hddm_lb_ctx* ctx, uint64_t* stats, int n);
 
 /* ---- host-side node layout, for inspection without a device ---- */
 typedef struct hddm_lb_layout hddm_lb_layout;

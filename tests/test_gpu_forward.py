@@ -273,3 +273,32 @@ def test_walk_modes_and_group_sizes(ctx, walk, waves):
         ctx.set_option(hddm_lb.OPT_WALK, 0)
         ctx.set_option(hddm_lb.OPT_WAVES_PER_WG, 0)
         ctx.set_option(hddm_lb.OPT_SEGMENT_NODES, 0)
+
+
+def test_stats_counters(ctx):
+    """Statistics mode leaves the results bit for bit unchanged and reports
+    consistent counts: the F32 path issues at most 2 MFMAs (two 32-point
+    halves) per K-step of each 32-node chunk, the FP64 path none."""
+    ii, jj, lv, ix = hddm_lb.generate_grid(12, 2, 6, 0.3, 31, 6000)
+    S = hddm_lb.synthetic_surplus(lv, 32, 32)
+    ctx.upload_state(0, ii, jj, S)
+    X = hddm_lb.synthetic_points(3000, 12, 33)
+    try:
+        for prec in (hddm_lb.F32, hddm_lb.FP64):
+            say("stats precision", prec)
+            a = run_forward(ctx, X, precision=prec)
+            ctx.set_option(hddm_lb.OPT_STATS, 1)
+            b = run_forward(ctx, X, precision=prec)
+            ctx.set_option(hddm_lb.OPT_STATS, 0)
+            assert np.array_equal(a, b)
+            ops, prunes, chunks, items, mfma = ctx.last_stats(5)
+            assert ctx.last_stats() == (ops, prunes, chunks, items)
+            assert items >= (3000 + 63) // 64 and ops > 0 and chunks > 0 and prunes < ops
+            if prec == hddm_lb.F32:
+                assert 0 < mfma <= 2 * 16 * chunks
+            else:
+                assert mfma == 0
+        with pytest.raises(hddm_lb.HddmLbError):
+            ctx.last_stats(6)
+    finally:
+        ctx.set_option(hddm_lb.OPT_STATS, 0)
