@@ -29,8 +29,9 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 // Per op: the parent product comes from the LDS path stack (P[t],
 // lane-strided rows of 512 bytes at pl) and the op's product is stored as
 // P[t+1], so P[k] is the product of the first k factors of the current trie
-// path; the coordinate is read from the bank with an indexed move whose
-// index is the op's dim field (twice the dimension). A wave-wide zero
+// path; the hat's fma reads the coordinate straight from the bank as its
+// first source under GPR indexing (s_set_gpr_idx_on, index = the op's dim
+// field, twice the dimension). A wave-wide zero
 // product skips the op's run (the skip target and its successor are loaded
 // only when the target is not the successor); a live last factor emits its
 // node into the chunk buffer at bl.
@@ -43,7 +44,7 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 // s[76:79].
 // Fixed registers: the FP64 i as s[94:95] (low word 0), x from v64 (one,
 // two or four banks of 32 VGPRs), temporaries s88-s90 and eight VGPRs after
-// x (HW_*; the low word of -j is kept 0 in HW_B0); the indexed moves set m0.
+// x (HW_*; the low word of -j is kept 0 in HW_B0); indexing sets m0.
 // The chunk buffer holds doubles (rows of 512 bytes) or, for the F32 path,
 // floats (rows of 256 bytes): HW_BSH and HW_BST.
 #define HDDM_WALK_MAIN(X, X0, X1, X2, X3, ZQ, NEXT)                                    \
@@ -52,13 +53,11 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 	"s_and_b32 s88, s88, 0x1e00\n"                                                     \
 	"v_add_u32 " HW_D ", s88, %[pl]\n"                                                     \
 	"ds_read_b64 " HW_C ", " HW_D "\n"                                                   \
-	"s_set_gpr_idx_on " X0 ", gpr_idx(SRC0)\n"                                         \
-	"v_mov_b32 " HW_A0 ", v64\n"                                                             \
-	"v_mov_b32 " HW_A1 ", v65\n"                                                             \
-	"s_set_gpr_idx_off\n"                                                              \
 	"s_mov_b32 s95, " X2 "\n"                                                          \
 	"v_mov_b32 " HW_B1 ", " X3 "\n"                                                          \
-	"v_fma_f64 " HW_A ", " HW_A ", s[94:95], " HW_B "\n"                               \
+	"s_set_gpr_idx_on " X0 ", gpr_idx(SRC0)\n"                                         \
+	"v_fma_f64 " HW_A ", v[64:65], s[94:95], " HW_B "\n"                               \
+	"s_set_gpr_idx_off\n"                                                              \
 	"v_add_f64 " HW_A ", -|" HW_A "|, 1.0\n"                                           \
 	"v_max_f64 " HW_A ", " HW_A ", 0\n"                                                \
 	"s_waitcnt lgkmcnt(0)\n"                                                           \
@@ -195,7 +194,7 @@ __device__ __forceinline__ int walk_op_stream(const char* ob, uint32_t& qb, uint
 	return node;
 }
 
-// Two coordinate banks (16 < D <= 32): x in v[64:127] (the indexed move
+// Two coordinate banks (16 < D <= 32): x in v[64:127] (the indexed source
 // reaches dims 16-31 through the same index), temporaries v[128:135].
 template <int CB, bool BF>
 __device__ __forceinline__ int walk_op_stream2(const char* ob, uint32_t& qb, uint32_t qe, int chunk,
