@@ -24,12 +24,17 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 // each step and the loop ends; a successor, the op two ahead and a skip
 // target lie at most two ops past the last op of the stream, inside its
 // kOpPad sentinels; the depth t <= 14 and P has a row for depth t + 1 (the
-// stage holds 8 rows when every node has at most 7 factors, else 16); the
+// stage holds 8 rows when every node has at most 7 factors, else 16), also
+// for the read issued ahead for the op after a segment's last (an op of any
+// state, or a sentinel of depth 0); the
 // register index 2 * dim stays inside the pinned coordinate banks.
 // Per op: the parent product comes from the LDS path stack (P[t],
 // lane-strided rows of 512 bytes at pl) and the op's product is stored as
 // P[t+1], so P[k] is the product of the first k factors of the current trie
-// path; the hat's fma reads the coordinate straight from the bank as its
+// path. The read of P[t] is issued at the end of the previous step (or after
+// a skip target is loaded, or on entry), as soon as the op's depth is known;
+// a wave's LDS operations complete in order, so it sees the P[t+1] its parent
+// step has just stored; the hat's fma reads the coordinate straight from the bank as its
 // first source under GPR indexing (s_set_gpr_idx_on, index = the op's dim
 // field, twice the dimension). A wave-wide zero
 // product skips the op's run (the skip target and its successor are loaded
@@ -47,12 +52,16 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 // x (HW_*; the low word of -j is kept 0 in HW_B0); indexing sets m0.
 // The chunk buffer holds doubles (rows of 512 bytes) or, for the F32 path,
 // floats (rows of 256 bytes): HW_BSH and HW_BST.
-#define HDDM_WALK_MAIN(X, X0, X1, X2, X3, ZQ, NEXT)                                    \
-	"L_top" X "_%=:\n"                                                                 \
-	"s_lshl_b32 s88, " X1 ", 9\n"                                                      \
+// Issues the read of P[t] for the op whose ctrl word is C1 (into HW_C, its
+// address in HW_D).
+#define HDDM_WALK_PREP(C1)                                                             \
+	"s_lshl_b32 s88, " C1 ", 9\n"                                                      \
 	"s_and_b32 s88, s88, 0x1e00\n"                                                     \
 	"v_add_u32 " HW_D ", s88, %[pl]\n"                                                     \
-	"ds_read_b64 " HW_C ", " HW_D "\n"                                                   \
+	"ds_read_b64 " HW_C ", " HW_D "\n"
+
+#define HDDM_WALK_MAIN(X, X0, X1, X2, X3, Y1, ZQ, NEXT)                                \
+	"L_top" X "_%=:\n"                                                                 \
 	"s_mov_b32 s95, " X2 "\n"                                                          \
 	"v_mov_b32 " HW_B1 ", " X3 "\n"                                                          \
 	"s_set_gpr_idx_on " X0 ", gpr_idx(SRC0)\n"                                         \
@@ -78,6 +87,7 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 	"v_add_u32 " HW_E ", s89, %[bl]\n"                                                 \
 	HW_BST                                                                             \
 	"L_adv" X "_%=:\n"                                                                 \
+	HDDM_WALK_PREP(Y1)                                                                 \
 	"s_add_u32 %[qb], %[qb], 16\n"                                                     \
 	"s_cmp_lt_u32 %[qb], %[qe]\n"                                                      \
 	NEXT
@@ -86,7 +96,7 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 #define HDDM_WALK_FALL(Y) "s_cbranch_scc0 L_end" Y "_%=\n"
 #define HDDM_WALK_LOOP(Y) "s_cbranch_scc1 L_top" Y "_%=\n" "s_branch L_end" Y "_%=\n"
 
-#define HDDM_WALK_SIDE(X, X0, X1, Y, YQ, ZQ)                                           \
+#define HDDM_WALK_SIDE(X, X0, X1, Y, Y1, YQ, ZQ)                                       \
 	"L_prune" X "_%=:\n"                                                               \
 	"s_add_u32 %[nprunes], %[nprunes], 1\n"                                            \
 	"s_lshr_b32 s90, " X0 ", 8\n"                                                      \
@@ -98,6 +108,7 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 	"s_load_dwordx4 " YQ ", %[ob], %[qb]\n"                                            \
 	"s_load_dwordx4 " ZQ ", %[ob], %[qb] offset:0x10\n"                                \
 	"s_waitcnt lgkmcnt(0)\n"                                                           \
+	HDDM_WALK_PREP(Y1)                                                                 \
 	"s_cmp_lt_u32 %[qb], %[qe]\n"                                                      \
 	"s_cbranch_scc1 L_top" Y "_%=\n"                                                   \
 	"s_branch L_end" Y "_%=\n"                                                         \
@@ -111,12 +122,13 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 	"v_mov_b32 " HW_B0 ", 0\n"                                                               \
 	"s_mov_b32 s94, 0\n"                                                               \
 	"s_load_dwordx4 s[80:83], %[ob], %[qb] offset:0x10\n"                              \
-	HDDM_WALK_MAIN("0", "s76", "s77", "s78", "s79", "s[84:87]", HDDM_WALK_FALL("1"))   \
-	HDDM_WALK_MAIN("1", "s80", "s81", "s82", "s83", "s[76:79]", HDDM_WALK_FALL("2"))   \
-	HDDM_WALK_MAIN("2", "s84", "s85", "s86", "s87", "s[80:83]", HDDM_WALK_LOOP("0"))   \
-	HDDM_WALK_SIDE("0", "s76", "s77", "1", "s[80:83]", "s[84:87]")                     \
-	HDDM_WALK_SIDE("1", "s80", "s81", "2", "s[84:87]", "s[76:79]")                     \
-	HDDM_WALK_SIDE("2", "s84", "s85", "0", "s[76:79]", "s[80:83]")                     \
+	HDDM_WALK_PREP("s77")                                                              \
+	HDDM_WALK_MAIN("0", "s76", "s77", "s78", "s79", "s81", "s[84:87]", HDDM_WALK_FALL("1")) \
+	HDDM_WALK_MAIN("1", "s80", "s81", "s82", "s83", "s85", "s[76:79]", HDDM_WALK_FALL("2")) \
+	HDDM_WALK_MAIN("2", "s84", "s85", "s86", "s87", "s77", "s[80:83]", HDDM_WALK_LOOP("0")) \
+	HDDM_WALK_SIDE("0", "s76", "s77", "1", "s81", "s[80:83]", "s[84:87]")              \
+	HDDM_WALK_SIDE("1", "s80", "s81", "2", "s85", "s[84:87]", "s[76:79]")              \
+	HDDM_WALK_SIDE("2", "s84", "s85", "0", "s77", "s[76:79]", "s[80:83]")              \
 	"L_end0_%=:\n"                                                                     \
 	"s_mov_b32 %[node], -1\n"                                                          \
 	"L_exit0_%=:\n"                                                                    \
