@@ -60,13 +60,13 @@ def test_kat_on_device(kat):
 
 
 @pytest.mark.parametrize("prec", ALL_PRECISIONS)
-def test_config1_regular_d4(ctx, prec):
-    """Plain FP64 configuration: regular D=4 grid sum(l-1) <= 4 (401 nodes), dofs=8, 4096 points."""
+def test_regular_grid_d4_401_nodes(ctx, prec):
+    """Regular D=4 grid sum(l-1) <= 4 (401 nodes), dofs=8, 4096 points."""
     ii, jj, lv, ix = hddm_lb.generate_grid(4, 4, 5, 1.0, 1, 0)
     S = hddm_lb.synthetic_surplus(lv, 8, 11)
     ctx.upload_state(0, ii, jj, S)
     X = hddm_lb.synthetic_points(4096, 4, 12)
-    say("config1 precision", prec)
+    say("regular d4 precision", prec)
     y = run_forward(ctx, X, precision=prec)
     yo = oracle.forward(ii, jj, S, X)
     err = normwise_rel_err(y, yo)
@@ -75,13 +75,13 @@ def test_config1_regular_d4(ctx, prec):
 
 
 @pytest.mark.parametrize("prec", ALL_PRECISIONS)
-def test_config2_grid_d16_subset(ctx, prec):
+def test_adaptive_grid_d16_60k_subset(ctx, prec):
     """D=16 adaptive grid of ~60k nodes, dofs=32, on a 1536-point subset."""
     ii, jj, lv, ix = hddm_lb.generate_grid(16, 2, 7, 0.3, 1, 60000)
     S = hddm_lb.synthetic_surplus(lv, 32, 2)
     ctx.upload_state(0, ii, jj, S)
     X = hddm_lb.synthetic_points(1536, 16, 3)
-    say("config2 subset precision", prec)
+    say("d16 60k subset precision", prec)
     y = run_forward(ctx, X, precision=prec)
     yo = oracle.forward(ii, jj, S, X)
     err = normwise_rel_err(y, yo)

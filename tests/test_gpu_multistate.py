@@ -55,7 +55,7 @@ def check_subset(states, X, sid, Y, prec, per_state, seed):
 
 
 @pytest.fixture(scope="module")
-def cfg3(torch_mod):
+def d20_twelve_states(torch_mod):
     rng = np.random.default_rng(2024)
     nnos = np.exp(rng.uniform(np.log(2000), np.log(200000), size=12)).astype(int)
     dofs = rng.choice([16, 32, 64], size=12)
@@ -71,9 +71,9 @@ def cfg3(torch_mod):
 
 
 @pytest.mark.parametrize("prec", [hddm_lb.FP64, hddm_lb.F32, hddm_lb.F16X2])
-def test_multistate_d20_twelve_states(torch_mod, cfg3, prec):
+def test_multistate_d20_twelve_states(torch_mod, d20_twelve_states, prec):
     torch = torch_mod
-    ctx, states, X, sid = cfg3
+    ctx, states, X, sid = d20_twelve_states
     x = torch.from_numpy(X).cuda()
     s = torch.from_numpy(sid).cuda()
     y = torch.full((X.shape[0], 64), np.nan, dtype=torch.float64, device="cuda")
@@ -89,7 +89,7 @@ def test_multistate_d20_twelve_states(torch_mod, cfg3, prec):
 
 
 @pytest.fixture(scope="module")
-def cfg5(torch_mod):
+def d59_four_states(torch_mod):
     ctx = hddm_lb.Context(0)
     states = build_states(ctx, [(59, 3, 4, 0.0, 500 + s, 0, 128) for s in range(4)])
     n = 65536
@@ -100,9 +100,9 @@ def cfg5(torch_mod):
 
 
 @pytest.mark.parametrize("prec", [hddm_lb.FP64, hddm_lb.F16X2, hddm_lb.BF16X3])
-def test_multistate_d59_dofs128(torch_mod, cfg5, prec):
+def test_multistate_d59_dofs128(torch_mod, d59_four_states, prec):
     torch = torch_mod
-    ctx, states, X, sid = cfg5
+    ctx, states, X, sid = d59_four_states
     x = torch.from_numpy(X).cuda()
     s = torch.from_numpy(sid).cuda()
     y = torch.full((X.shape[0], 128), np.nan, dtype=torch.float64, device="cuda")
