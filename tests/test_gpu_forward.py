@@ -302,3 +302,27 @@ def test_stats_counters(ctx):
             ctx.last_stats(6)
     finally:
         ctx.set_option(hddm_lb.OPT_STATS, 0)
+
+
+def test_padded_rows(ctx):
+    """Row strides larger than dim and dofs: padding columns of x hold NaN
+    (any read of them would poison the result), padding columns of y are left
+    untouched, and results equal those of contiguous rows bit for bit."""
+    import torch
+    ii, jj, lv, ix = hddm_lb.generate_grid(12, 2, 6, 0.3, 41, 3000)
+    S = hddm_lb.synthetic_surplus(lv, 24, 42)
+    ctx.upload_state(0, ii, jj, S)
+    n, dim, dofs = 777, 12, 24
+    X = hddm_lb.synthetic_points(n, dim, 43)
+    xp = np.full((n, dim + 5), np.nan)
+    xp[:, :dim] = X
+    x = torch.from_numpy(xp).cuda()
+    for prec in (hddm_lb.FP64, hddm_lb.F32):
+        say("padded rows precision", prec)
+        ref = run_forward(ctx, X, precision=prec)
+        y = torch.full((n, dofs + 3), np.nan, dtype=torch.float64, device="cuda")
+        ctx.forward(x, y, None, prec)
+        torch.cuda.synchronize()
+        yh = y.cpu().numpy()
+        assert np.array_equal(yh[:, :dofs], ref)
+        assert np.isnan(yh[:, dofs:]).all()
