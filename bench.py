@@ -210,7 +210,7 @@ def main():
                     help="first copy MIB MiB device-to-device 3 times (calibrates profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (a forward shards by points; "
+        print("bench.py measures one GPU: the library has no multi-GPU path (a forward shards by points; "
               "run one process per GPU on its own share). Use --gpus 1.", file=sys.stderr)
         return 2
 
