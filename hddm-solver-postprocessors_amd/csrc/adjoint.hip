@@ -44,6 +44,7 @@ struct AdjointArgs
 	int64_t ldg;
 	double* dx;        // may be null
 	int64_t lddx;
+	double* dxpart;    // node-segment partials of dx for split states ([nseg][count][dim] per state)
 	double* ds;        // may be null; state s starts at states[s].ds_offset
 	int64_t ds_xcd;    // > 0: ds holds one copy of this many elements per XCD
 	int nstates;
@@ -343,16 +344,48 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 			const double r = R[tt * kWave + lane];
 			if (wave_any(r != 0.0)) leave_edge<XB>(edges[tt], r, xb0, xb1, xb2, xb3, DX, R, tt, lane);
 		}
-		if (a.dx && lane < np)
+		// Each (segment, block) item belongs to one wave per launch and the
+		// dof-slab launches are stream-ordered, so the first slab stores and
+		// later slabs add without atomics; split states go through
+		// per-segment partials summed in segment order afterwards.
+		if (lane < np)
 		{
-			double* out = a.dx + (int64_t)orig * a.lddx;
+			double* out = ldc(&sd->nseg) > 1
+				? a.dxpart + ldc(&a.plans[it.s].dx_base) + ((int64_t)it.seg * it.count + blk * kWave + lane) * a.dim
+				: a.dx + (int64_t)orig * a.lddx;
 #pragma unroll
 			for (int d = 0; d < DXR; d++)
 				if (d < a.dim)
 				{
 					const double v = DX[d * kWave + lane];
-					if (v != 0.0) unsafeAtomicAdd(out + d, v);
+					out[d] = dt0 == 0 ? v : out[d] + v;
 				}
+		}
+	}
+}
+
+// dx of split states: sums the node-segment partials of every point in
+// segment order. Block (64, 4): threadIdx.x walks the dims, threadIdx.y picks
+// one of 4 points; blocks stride over the state's points.
+__global__ void __launch_bounds__(256) reduce_dx_partials_kernel(const StateDesc* __restrict__ states,
+	const StatePlan* __restrict__ plans, const uint32_t* __restrict__ perm, const double* __restrict__ part,
+	int dim, double* __restrict__ dx, int64_t lddx)
+{
+	const int s = blockIdx.y;
+	const int nseg = states[s].nseg;
+	if (nseg <= 1) return;
+	const int64_t count = plans[s].count, start = plans[s].start;
+	const double* base = part + plans[s].dx_base;
+	const int64_t gstride = count * dim;
+	for (int64_t lp = (int64_t)blockIdx.x * 4 + threadIdx.y; lp < count; lp += (int64_t)gridDim.x * 4)
+	{
+		const uint32_t orow = perm[start + lp];
+		const double* src = base + lp * dim;
+		for (int d = threadIdx.x; d < dim; d += 64)
+		{
+			double sum = 0.0;
+			for (int g = 0; g < nseg; g++) sum += src[g * gstride + d];
+			dx[(int64_t)orow * lddx + d] = sum;
 		}
 	}
 }
@@ -480,18 +513,20 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 	hipStream_t st = (hipStream_t)stream;
 	int rc = prepare_scratch(ctx, ctx->scratch, std::max<int64_t>(npoints, 1));
 	if (rc) return rc;
+	if ((rc = scratch_wait(ctx, st))) return rc;
 	int64_t ds_total = 0;
 	if ((rc = hddm_lb_surplus_offset(ctx, -1, nullptr, &ds_total))) return rc;
-	// outputs are accumulated: clear them with a kernel (never a memset, so
-	// the sequence stays capturable)
-	// dS through per-XCD copies when they fit the scratch limit
-	const bool xcd_copies = dsurplus && ds_total && ds_total * kXcdCopies <= kXcdCopiesMaxElems;
+	// dS is accumulated: clear it with a kernel (never a memset, so the
+	// sequence stays capturable), through per-XCD copies when they fit the
+	// scratch limit
+	const bool xcd_copies = npoints > 0 && dsurplus && ds_total && ds_total * kXcdCopies <= kXcdCopiesMaxElems;
 	if (xcd_copies)
 	{
 		Scratch& scr = ctx->scratch;
 		const size_t need = (size_t)ds_total * kXcdCopies;
 		if (need > scr.cap_dspart)
 		{
+			if (scr.dspart) HIP_TRY(hipDeviceSynchronize());
 			dev_free(scr.dspart);
 			scr.dspart = nullptr;
 			scr.cap_dspart = 0;
@@ -503,8 +538,25 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 		HIP_TRY(launch_fill_u32((uint32_t*)scr.dspart, 2 * (int64_t)need, 0u, st));
 	}
 	else if (dsurplus && ds_total) HIP_TRY(launch_fill_u32((uint32_t*)dsurplus, 2 * ds_total, 0u, st));
-	if (dx && npoints) HIP_TRY(launch_fill_u32((uint32_t*)dx, 2 * (npoints - 1) * lddx + 2 * ctx->dim, 0u, st));
-	if (npoints == 0 || (!dx && !dsurplus)) return HDDM_LB_SUCCESS;
+	if (npoints == 0 || (!dx && !dsurplus)) return scratch_mark(ctx, st);
+	// dx is stored, not accumulated: no clear, and columns past dim of a
+	// padded dx stay as the caller left them
+	if (dx && ctx->max_nseg > 1)
+	{
+		Scratch& scr = ctx->scratch;
+		const size_t need = (size_t)npoints * (size_t)ctx->max_nseg * (size_t)ctx->dim;
+		if (need > scr.cap_dxpart)
+		{
+			if (scr.dxpart) HIP_TRY(hipDeviceSynchronize());
+			dev_free(scr.dxpart);
+			scr.dxpart = nullptr;
+			scr.cap_dxpart = 0;
+			int arc = 0;
+			scr.dxpart = (double*)dev_alloc(need * sizeof(double), arc);
+			if (arc) return arc;
+			scr.cap_dxpart = need;
+		}
+	}
 
 	// dx and dS together: the edge-sum walk for dx, then the forward's walk
 	// for dS (two lighter kernels run faster than one that carries both)
@@ -545,13 +597,15 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 	const int64_t nb = (int64_t)(ns + 1) << bb;
 	HIP_TRY(launch_fill_u32(sc.hist, nb, 0u, st));
 	HIP_TRY(launch_bucket_key(npoints, x, ldx, state_ids, ctx->d_state_ok, ns, ctx->dim, bb, sc.bkey, sc.hist, st));
-	HIP_TRY(launch_scan_plan(sc.hist, sc.cursor, nb, bb, ns, ctx->d_states, ctx->d_cost_order, sc.plans, sc.hdr, 1, st));
+	HIP_TRY(launch_scan_plan(sc.hist, sc.cursor, nb, bb, ns, ctx->d_states, ctx->d_cost_order, sc.plans, sc.hdr, 1,
+		ctx->dim, st));
 	HIP_TRY(launch_scatter(npoints, sc.bkey, sc.cursor, sc.perm, st));
 
 	AdjointArgs a;
 	std::memset(&a, 0, sizeof(a));
 	a.states = ctx->d_states; a.plans = sc.plans; a.cost_order = ctx->d_cost_order; a.hdr = sc.hdr;
 	a.perm = sc.perm; a.x = x; a.ldx = ldx; a.g = gbar; a.ldg = ldg; a.dx = dx; a.lddx = lddx;
+	a.dxpart = sc.dxpart;
 	a.ds = xcd_copies ? sc.dspart : dsurplus; a.ds_xcd = xcd_copies ? ds_total : 0;
 	a.nstates = ns; a.dim = ctx->dim;
 	bool first = true;
@@ -564,6 +618,13 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 			void* params[] = { (void*)&a };
 			HIP_TRY(hipLaunchKernel(kerns[m], dim3(grids[m]), dim3(groups[m] * kWave), params, ldss[m], st));
 		}
+	if (dx && ctx->max_nseg > 1)
+	{
+		hipLaunchKernelGGL(reduce_dx_partials_kernel, dim3(1024, ns), dim3(64, 4), 0, st,
+			(const StateDesc*)ctx->d_states, (const StatePlan*)sc.plans, (const uint32_t*)sc.perm,
+			(const double*)sc.dxpart, ctx->dim, dx, lddx);
+		HIP_TRY(hipGetLastError());
+	}
 	if (xcd_copies)
 	{
 		int blocks = (int)std::min<int64_t>(4096, (ds_total + 255) / 256);
@@ -571,5 +632,5 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 			(const double*)sc.dspart, ds_total, dsurplus);
 		HIP_TRY(hipGetLastError());
 	}
-	return HDDM_LB_SUCCESS;
+	return scratch_mark(ctx, st);
 }

@@ -53,11 +53,11 @@ Scratch::~Scratch() { release(); }
 void Scratch::release()
 {
 	for (void* p : { (void*)bkey, (void*)perm, (void*)hist, (void*)cursor, (void*)partial, (void*)dspart,
-			(void*)plans, (void*)hdr })
+			(void*)dxpart, (void*)plans, (void*)hdr })
 		dev_free(p);
 	bkey = perm = hist = cursor = nullptr;
-	partial = dspart = nullptr;
-	cap_dspart = 0;
+	partial = dspart = dxpart = nullptr;
+	cap_dspart = cap_dxpart = 0;
 	plans = nullptr;
 	hdr = nullptr;
 	cap_bkey = cap_perm = cap_hist = cap_cursor = cap_partial = cap_plans = cap_hdr = 0;
@@ -96,6 +96,7 @@ hddm_lb_ctx::~hddm_lb_ctx()
 	for (void* p : { (void*)d_states, (void*)d_cost_order, (void*)d_state_ok,
 		(void*)d_xstage, (void*)d_ystage, (void*)d_sidstage, (void*)d_wbuf, (void*)d_stats })
 		dev_free(p);
+	if (scratch_event) (void)hipEventDestroy(scratch_event);
 	for (hipEvent_t e : tev0) (void)hipEventDestroy(e);
 	for (hipEvent_t e : tev1) (void)hipEventDestroy(e);
 	if (own_stream) (void)hipStreamDestroy(own_stream);
@@ -130,6 +131,7 @@ extern "C" int hddm_lb_create(int device, hddm_lb_ctx** out)
 	HIP_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device));
 	ctx->num_cus = cus;
 	ctx->max_lds = (size_t)lds;
+	HIP_TRY(hipEventCreateWithFlags(&ctx->scratch_event, hipEventDisableTiming));
 	for (int i = 0; i < hddm_lb_ctx::kTimingRing; i++)
 	{
 		hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -279,12 +281,31 @@ extern "C" int hddm_lb_surplus_offset(hddm_lb_ctx* ctx, int istate, int64_t* off
 
 namespace hddm_lb {
 
+int scratch_wait(hddm_lb_ctx* ctx, hipStream_t st)
+{
+	if (ctx->scratch_pending && ctx->scratch_stream != st)
+		HIP_TRY(hipStreamWaitEvent(st, ctx->scratch_event, 0));
+	return HDDM_LB_SUCCESS;
+}
+
+int scratch_mark(hddm_lb_ctx* ctx, hipStream_t st)
+{
+	HIP_TRY(hipEventRecord(ctx->scratch_event, st));
+	ctx->scratch_stream = st;
+	ctx->scratch_pending = true;
+	return HDDM_LB_SUCCESS;
+}
+
 // Uploads the per-state descriptor table when states or segmenting changed.
 int prepare_tables(hddm_lb_ctx* ctx)
 {
 	const int ns = (int)ctx->states.size();
 	if (ns == 0) return set_error(HDDM_LB_ERROR_STATE, "no state is loaded");
 	if (!ctx->tables_dirty && ctx->tables_n == ns) return HDDM_LB_SUCCESS;
+	// the tables are rewritten with synchronous copies: work still in flight
+	// on any stream may read them (this runs only after an upload or an
+	// option change)
+	HIP_TRY(hipDeviceSynchronize());
 	int nloaded = 0;
 	for (auto& s : ctx->states) if (s && s->loaded) nloaded++;
 	if (!nloaded) return set_error(HDDM_LB_ERROR_STATE, "no state is loaded");
@@ -295,6 +316,7 @@ int prepare_tables(hddm_lb_ctx* ctx)
 	ctx->dt_needed = 1;
 	ctx->fmax_all = 1;
 	ctx->max_nseg_dp = 0;
+	ctx->max_nseg = 1;
 	int64_t ds_offset = 0;
 	for (int s = 0; s < ns; s++)
 	{
@@ -333,6 +355,7 @@ int prepare_tables(hddm_lb_ctx* ctx)
 		ctx->dt_needed = std::max(ctx->dt_needed, d.dofs_pad / 32);
 		ctx->fmax_all = std::max(ctx->fmax_all, d.max_depth);
 		if (d.nseg > 1) ctx->max_nseg_dp = std::max<int64_t>(ctx->max_nseg_dp, (int64_t)d.nseg * d.dofs_pad);
+		ctx->max_nseg = std::max(ctx->max_nseg, d.nseg);
 		cost.push_back(std::make_pair(-(double)d.nops / d.nseg * d.dofs_pad, s));
 	}
 	std::stable_sort(cost.begin(), cost.end());
@@ -362,6 +385,8 @@ template <typename T>
 static int grow(T*& p, size_t& cap, size_t need)
 {
 	if (need <= cap && p) return HDDM_LB_SUCCESS;
+	// a buffer being replaced may still be read by work in flight on any stream
+	if (p) HIP_TRY(hipDeviceSynchronize());
 	dev_free(p);
 	p = nullptr;
 	cap = 0;
@@ -431,7 +456,7 @@ int enqueue_forward(hddm_lb_ctx* ctx, Scratch& sc, int64_t npoints, const double
 	HIP_TRY(launch_fill_u32(sc.hist, nb, 0u, st));
 	HIP_TRY(launch_bucket_key(npoints, x, ldx, sid, ctx->d_state_ok, ns, ctx->dim, bb, sc.bkey, sc.hist, st));
 	HIP_TRY(launch_scan_plan(sc.hist, sc.cursor, nb, bb, ns, ctx->d_states, ctx->d_cost_order,
-		sc.plans, sc.hdr, cfg.walk == kWalkLds ? cfg.waves : 1, st));
+		sc.plans, sc.hdr, cfg.walk == kWalkLds ? cfg.waves : 1, 0, st));
 	HIP_TRY(launch_scatter(npoints, sc.bkey, sc.cursor, sc.perm, st));
 
 	const void* kern = sweep_kernel_ptr(prec, cfg.slab, xbanks_for(ctx->dim), ctx->fmax_all, cfg.walk);
@@ -506,12 +531,16 @@ extern "C" int hddm_lb_forward(hddm_lb_ctx* ctx, int64_t npoints,
 	if (rc) return rc;
 	if (npoints == 0) return HDDM_LB_SUCCESS;
 	DeviceGuard g(ctx->device);
+	hipStream_t st = (hipStream_t)stream;
 	if ((rc = prepare_scratch(ctx, ctx->scratch, npoints))) return rc;
+	if ((rc = scratch_wait(ctx, st))) return rc;
 	TuneConfig cfg = default_config(ctx, precision);
-	if (ctx->opt_autotune && (rc = autotune(ctx, npoints, x, ldx, state_ids, y, ldy, precision, (hipStream_t)stream, cfg)))
+	if (ctx->opt_autotune && (rc = autotune(ctx, npoints, x, ldx, state_ids, y, ldy, precision, st, cfg)))
 		return rc;
 	ctx->last_timed = ctx->opt_timing;
-	return enqueue_forward(ctx, ctx->scratch, npoints, x, ldx, state_ids, y, ldy, precision, (hipStream_t)stream, cfg, ctx->opt_timing);
+	rc = enqueue_forward(ctx, ctx->scratch, npoints, x, ldx, state_ids, y, ldy, precision, st, cfg, ctx->opt_timing);
+	if (rc) return rc;
+	return scratch_mark(ctx, st);
 }
 
 extern "C" int hddm_lb_last_stats_n(hddm_lb_ctx* ctx, uint64_t* stats, int n)
@@ -620,8 +649,12 @@ extern "C" int hddm_lb_plan_create(hddm_lb_ctx* ctx, int64_t npoints,
 	DeviceGuard g(ctx->device);
 	if ((rc = prepare_scratch(ctx, ctx->scratch, npoints))) return rc;
 	TuneConfig cfg = default_config(ctx, precision);
-	if (ctx->opt_autotune && (rc = autotune(ctx, npoints, x, ldx, state_ids, y, ldy, precision, (hipStream_t)stream, cfg)))
-		return rc;
+	if (ctx->opt_autotune)
+	{
+		if ((rc = scratch_wait(ctx, (hipStream_t)stream))) return rc;
+		if ((rc = autotune(ctx, npoints, x, ldx, state_ids, y, ldy, precision, (hipStream_t)stream, cfg))) return rc;
+		if ((rc = scratch_mark(ctx, (hipStream_t)stream))) return rc;
+	}
 	std::unique_ptr<hddm_lb_plan> plan(new hddm_lb_plan);
 	plan->ctx = ctx;
 	plan->generation = ctx->generation;

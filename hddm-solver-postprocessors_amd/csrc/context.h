@@ -47,6 +47,7 @@ struct Scratch
 	uint32_t* cursor = nullptr; size_t cap_cursor = 0;
 	double* partial = nullptr; size_t cap_partial = 0;
 	double* dspart = nullptr; size_t cap_dspart = 0;   // per-XCD copies of the adjoint's dS
+	double* dxpart = nullptr; size_t cap_dxpart = 0;   // node-segment partials of the adjoint's dx
 	StatePlan* plans = nullptr; size_t cap_plans = 0;
 	PlanHeader* hdr = nullptr; size_t cap_hdr = 0;
 	~Scratch();
@@ -86,6 +87,7 @@ struct hddm_lb_ctx
 	int tables_n = 0, tables_cap = 0;
 	int dt_needed = 1, fmax_all = 1;
 	int64_t max_nseg_dp = 0;
+	int max_nseg = 1;            // most node segments of a split state (1: none is split)
 	hddm_lb::StateDesc* d_states = nullptr;
 	int32_t* d_cost_order = nullptr;
 	uint8_t* d_state_ok = nullptr;
@@ -97,6 +99,11 @@ struct hddm_lb_ctx
 	double* d_ystage = nullptr; size_t cap_ystage = 0;
 	int32_t* d_sidstage = nullptr; size_t cap_sidstage = 0;
 	hipStream_t own_stream = nullptr;
+	// Eager calls share `scratch`: each one records scratch_event on its
+	// stream, and the next call on another stream waits for it first.
+	hipEvent_t scratch_event = nullptr;
+	hipStream_t scratch_stream = nullptr;
+	bool scratch_pending = false;
 
 	// adjoint scratch
 	float* d_wbuf = nullptr; size_t cap_wbuf = 0;
@@ -122,6 +129,10 @@ struct hddm_lb_ctx
 namespace hddm_lb {
 
 int prepare_tables(hddm_lb_ctx* ctx);
+// Orders a call on st after the last eager call that used ctx->scratch on another stream.
+int scratch_wait(hddm_lb_ctx* ctx, hipStream_t st);
+// Marks the end of an eager call's use of ctx->scratch on st.
+int scratch_mark(hddm_lb_ctx* ctx, hipStream_t st);
 int prepare_scratch(hddm_lb_ctx* ctx, Scratch& sc, int64_t npoints);
 TuneConfig default_config(hddm_lb_ctx* ctx, int prec);
 int enqueue_forward(hddm_lb_ctx* ctx, Scratch& sc, int64_t npoints, const double* x, int64_t ldx,

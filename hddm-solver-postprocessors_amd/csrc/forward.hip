@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(kKeyBlock) bucket_key_kernel(int64_t n, const 
 __global__ void __launch_bounds__(1024) scan_plan_kernel(const uint32_t* __restrict__ hist,
 	uint32_t* __restrict__ cursor, int64_t nbuckets, int bbits, int nstates,
 	const StateDesc* __restrict__ states, const int32_t* __restrict__ cost_order,
-	StatePlan* __restrict__ plans, PlanHeader* __restrict__ hdr, int group)
+	StatePlan* __restrict__ plans, PlanHeader* __restrict__ hdr, int group, int dxw)
 {
 	// exclusive scan of the histogram into the cursors, 1024 buckets per
 	// step with coalesced reads: a wave-level scan, then the wave totals
@@ -132,7 +132,7 @@ __global__ void __launch_bounds__(1024) scan_plan_kernel(const uint32_t* __restr
 	__syncthreads();
 	if (t == 0)
 	{
-		int64_t item = 0, pbase = 0;
+		int64_t item = 0, pbase = 0, xbase = 0;
 		for (int i = 0; i < nstates; i++)
 		{
 			int s = cost_order[i];
@@ -140,7 +140,12 @@ __global__ void __launch_bounds__(1024) scan_plan_kernel(const uint32_t* __restr
 			plans[s].item_base = item;
 			item += (plans[s].nblk + group - 1) / group * nseg;
 			plans[s].partial_base = pbase;
-			if (nseg > 1) pbase += (int64_t)nseg * plans[s].count * states[s].dofs_pad;
+			plans[s].dx_base = xbase;
+			if (nseg > 1)
+			{
+				pbase += (int64_t)nseg * plans[s].count * states[s].dofs_pad;
+				xbase += (int64_t)nseg * plans[s].count * dxw;
+			}
 		}
 		hdr->nitems = item;
 		hdr->counter = 0;
@@ -757,10 +762,11 @@ hipError_t launch_bucket_key(int64_t n, const double* x, int64_t ldx, const int3
 }
 
 hipError_t launch_scan_plan(const uint32_t* hist, uint32_t* cursor, int64_t nbuckets, int bbits, int nstates,
-	const StateDesc* states, const int32_t* cost_order, StatePlan* plans, PlanHeader* hdr, int group, hipStream_t st)
+	const StateDesc* states, const int32_t* cost_order, StatePlan* plans, PlanHeader* hdr, int group, int dxw,
+	hipStream_t st)
 {
 	hipLaunchKernelGGL(scan_plan_kernel, dim3(1), dim3(1024), 0, st, hist, cursor, nbuckets, bbits, nstates,
-		states, cost_order, plans, hdr, group);
+		states, cost_order, plans, hdr, group, dxw);
 	return hipGetLastError();
 }
 

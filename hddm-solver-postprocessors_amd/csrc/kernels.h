@@ -44,7 +44,8 @@ hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st);
 hipError_t launch_bucket_key(int64_t n, const double* x, int64_t ldx, const int32_t* sid,
 	const uint8_t* state_ok, int nstates, int dim, int bbits, uint32_t* bkey, uint32_t* hist, hipStream_t st);
 hipError_t launch_scan_plan(const uint32_t* hist, uint32_t* cursor, int64_t nbuckets, int bbits, int nstates,
-	const StateDesc* states, const int32_t* cost_order, StatePlan* plans, PlanHeader* hdr, int group, hipStream_t st);
+	const StateDesc* states, const int32_t* cost_order, StatePlan* plans, PlanHeader* hdr, int group, int dxw,
+	hipStream_t st);
 hipError_t launch_scatter(int64_t n, const uint32_t* bkey, uint32_t* cursor, uint32_t* perm, hipStream_t st);
 hipError_t launch_reset_counter(PlanHeader* hdr, hipStream_t st);
 hipError_t launch_sweep(const void* kern, int grid, int waves, size_t lds, const SweepArgs& args, hipStream_t st);

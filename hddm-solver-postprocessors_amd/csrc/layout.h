@@ -79,6 +79,7 @@ struct StatePlan
 	int64_t nblk;          // 64-point blocks
 	int64_t item_base;     // first work item (items ordered by state cost)
 	int64_t partial_base;  // element offset in the partial buffer (split states)
+	int64_t dx_base;       // element offset in the adjoint's dx partial buffer (split states)
 };
 
 struct PlanHeader

@@ -49,7 +49,10 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 // s[76:79].
 // Fixed registers: the FP64 i as s[94:95] (low word 0), x from v64 (one,
 // two or four banks of 32 VGPRs), temporaries s88-s90 and eight VGPRs after
-// x (HW_*; the low word of -j is kept 0 in HW_B0); indexing sets m0.
+// x (HW_*; the low word of -j is kept 0 in HW_B0). GPR indexing writes m0,
+// which the compiler reserves and may hold live across the statement: it is
+// copied to s91 on entry and restored at the one exit (L_done), so the
+// statement leaves m0 as it found it.
 // The chunk buffer holds doubles (rows of 512 bytes) or, for the F32 path,
 // floats (rows of 256 bytes): HW_BSH and HW_BST.
 // Issues the read of P[t] for the op whose ctrl word is C1 (into HW_C, its
@@ -119,6 +122,7 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 	"s_branch L_exit" Y "_%=\n"
 
 #define HDDM_WALK_ASM                                                                  \
+	"s_mov_b32 s91, m0\n"                                                              \
 	"v_mov_b32 " HW_B0 ", 0\n"                                                               \
 	"s_mov_b32 s94, 0\n"                                                               \
 	"s_load_dwordx4 s[80:83], %[ob], %[qb] offset:0x10\n"                              \
@@ -147,7 +151,8 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 	"s_waitcnt lgkmcnt(0)\n"                                                           \
 	"s_mov_b64 s[76:77], s[84:85]\n"                                                   \
 	"s_mov_b64 s[78:79], s[86:87]\n"                                                   \
-	"L_done_%=:\n"
+	"L_done_%=:\n"                                                                     \
+	"s_mov_b32 m0, s91\n"
 
 #define HDDM_WALK_OUTPUTS                                                              \
 	: [qb] "+s"(qb), [mask] "+s"(mask), [nops] "+s"(nops), [nprunes] "+s"(nprunes),    \
@@ -157,7 +162,7 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 	  [cs] "i"(8 + CB), [cf] "i"((CB << 16) | 8)
 #define HDDM_WALK_SCLOBBERS                                                            \
 	"s80", "s81", "s82", "s83", "s84", "s85", "s86", "s87", "s88", "s89", "s90",       \
-	"s94", "s95", "vcc", "scc", "m0", "memory"
+	"s91", "s94", "s95", "vcc", "scc", "memory"
 
 // One coordinate bank (D <= 16): x in v[64:95], temporaries v[96:103].
 // BF: the chunk buffer holds floats (rows of 256 bytes), else doubles (512).

@@ -114,8 +114,10 @@ int hddm_lb_forward_host(hddm_lb_ctx* ctx, int64_t npoints,
  *             piecewise-constant hat derivative -i sign(x i - j) on the support
  *   dsurplus  FP64, state s at element offset hddm_lb_surplus_offset(s), rows in
  *             the uploaded node order, row length dofs(s): dS_k = sum_p B_k(x_p) gbar_p
- * Both outputs are overwritten; either may be NULL. The basis is recomputed.
- * dsurplus is summed with FP64 atomics, so its last bits vary between runs.
+ * Both outputs are overwritten (dx only in its first dim columns, and only in
+ * rows whose state id is valid); either may be NULL. The basis is recomputed.
+ * dx is summed in a fixed order (bitwise repeatable); dsurplus is summed with
+ * FP64 atomics within each XCD, so its last bits may vary between runs.
  * precision: HDDM_LB_FP64. Stream-ordered. */
 int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 	const double* x, int64_t ldx, const int32_t* state_ids,

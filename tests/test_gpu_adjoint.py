@@ -52,8 +52,10 @@ def grid_case(dim, start, maxl, p, seed, target, dofs, n, kink_level):
     return ii, jj, S, X, G
 
 
-@pytest.mark.parametrize("dim,dofs,n", [(6, 20, 300), (16, 32, 200), (17, 16, 150), (20, 40, 130), (32, 32, 120)])
+@pytest.mark.parametrize("dim,dofs,n", [(6, 20, 300), (16, 32, 200), (17, 16, 150), (20, 40, 130), (32, 32, 120),
+                                         (33, 24, 110), (59, 64, 100), (64, 40, 100)])
 def test_adjoint_matches_oracle(torch_mod, dim, dofs, n):
+    """One, two and four coordinate banks (D <= 16, <= 32, <= 64)."""
     ii, jj, S, X, G = grid_case(dim, 2, 5, 0.4, 3 + dim, 2500, dofs, n, 5)
     ctx = hddm_lb.Context(0)
     ctx.upload_state(0, ii, jj, S)
@@ -199,4 +201,88 @@ def test_adjoint_d20_100k_nodes(torch_mod):
             xm[0, d] -= h
             fd = ((oracle.forward(ii, jj, S, xp) - oracle.forward(ii, jj, S, xm)) * G[p]).sum() / (2 * h)
             assert abs(dx[p, d] - fd) <= 1e-7 * max(1.0, abs(fd))
+    ctx.close()
+
+
+def test_adjoint_padded_dx_rows(torch_mod):
+    """dx with a row stride above dim: the padding columns keep what the
+    caller put there (NaN here), the dim columns match the oracle."""
+    torch = torch_mod
+    ii, jj, S, X, G = grid_case(7, 2, 5, 0.4, 31, 1500, 24, 150, 5)
+    ctx = hddm_lb.Context(0)
+    ctx.upload_state(0, ii, jj, S)
+    x = torch.from_numpy(X).cuda()
+    g = torch.from_numpy(G).cuda()
+    dxp = torch.full((X.shape[0], 11), np.nan, dtype=torch.float64, device="cuda")
+    ctx.adjoint(x, g, dxp, None, None, hddm_lb.FP64)
+    torch.cuda.synchronize()
+    dx = dxp.cpu().numpy()
+    dxo, _ = oracle.adjoint(ii, jj, S, X, G)
+    assert np.isnan(dx[:, 7:]).all()
+    assert normwise_rel_err(dx[:, :7], dxo) <= 1e-10
+    ctx.close()
+
+
+def test_adjoint_dx_bitwise_repeatable(torch_mod):
+    """Node segments split every state here; the per-segment dx partials are
+    summed in a fixed order, so two runs agree to the last bit."""
+    torch = torch_mod
+    ii, jj, S, X, G = grid_case(20, 2, 5, 0.4, 55, 20000, 32, 4096, 5)
+    ctx = hddm_lb.Context(0)
+    ctx.set_option(hddm_lb.OPT_SEGMENT_NODES, 2000)
+    ctx.upload_state(0, ii, jj, S)
+    a, _ = run_adjoint(torch, ctx, X, G)
+    b, _ = run_adjoint(torch, ctx, X, G)
+    assert np.isfinite(a).all()
+    assert np.array_equal(a, b)
+    ctx.close()
+
+
+def test_adjoint_d59_four_states(torch_mod):
+    """High-dimensional multistate shape: 4 states, D=59, regular grids
+    sum(l-1) <= 3 (281,077 nodes each), dofs=128, 4,096 points in
+    [0.01, 0.99]^59 kept 1e-3 off every kink, gbar ~ N(0, 1). dx against the
+    oracle on 8 points per state and against central differences on a few
+    coordinates; dS against the oracle's B^T g over a 256-point slice."""
+    torch = torch_mod
+    ctx = hddm_lb.Context(0)
+    states = []
+    for s in range(4):
+        ii, jj, lv, _ = hddm_lb.generate_grid(59, 3, 4, 0.0, 500 + s, 0)
+        S = hddm_lb.synthetic_surplus(lv, 128, 600 + s)
+        ctx.upload_state(s, ii, jj, S)
+        states.append((ii, jj, S))
+    say("adjoint D=59 x 4 states, nno", states[0][0].shape[0])
+    n = 4096
+    X = hddm_lb.synthetic_points(n, 59, 92, 0.01, 0.99, kink_level=4, kink_gap=1e-3)
+    sid = np.random.default_rng(93).integers(0, 4, size=n).astype(np.int32)
+    G = np.random.default_rng(94).standard_normal((n, 128))
+    dx, _ = run_adjoint(torch, ctx, X, G, sid)
+    assert np.isfinite(dx).all()
+    rng = np.random.default_rng(95)
+    for s, (ii, jj, S) in enumerate(states):
+        pick = rng.choice(np.flatnonzero(sid == s), size=8, replace=False)
+        dxo, _ = oracle.adjoint(ii, jj, S, X[pick], G[pick])
+        err = normwise_rel_err(dx[pick], dxo)
+        say("  state", s, "dx err (8 points)", err)
+        assert err <= 1e-10
+        if s == 0:
+            h = 1e-4
+            for p in pick[:2]:
+                for d in (0, 30, 58):
+                    xp, xm = X[p:p + 1].copy(), X[p:p + 1].copy()
+                    xp[0, d] += h
+                    xm[0, d] -= h
+                    fd = ((oracle.forward(ii, jj, S, xp) - oracle.forward(ii, jj, S, xm)) * G[p]).sum() / (2 * h)
+                    assert abs(dx[p, d] - fd) <= 1e-7 * max(1.0, abs(fd))
+    m = 256
+    _, ds = run_adjoint(torch, ctx, X[:m], G[:m], sid[:m])
+    for s, (ii, jj, S) in enumerate(states):
+        sel = sid[:m] == s
+        _, dso = oracle.adjoint(ii, jj, S, X[:m][sel], G[:m][sel], nthreads=8)
+        off, _ = ctx.surplus_offset(s)
+        dss = ds[off:off + ii.shape[0] * 128].reshape(ii.shape[0], 128)
+        err = normwise_rel_err(dss, dso)
+        say("  state", s, "ds err (%d points)" % sel.sum(), err)
+        assert err <= 1e-12
     ctx.close()

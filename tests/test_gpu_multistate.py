@@ -99,7 +99,7 @@ def d59_four_states(torch_mod):
     ctx.close()
 
 
-@pytest.mark.parametrize("prec", [hddm_lb.FP64, hddm_lb.F16X2, hddm_lb.BF16X3])
+@pytest.mark.parametrize("prec", [hddm_lb.FP64, hddm_lb.F32, hddm_lb.F16X2, hddm_lb.BF16X3])
 def test_multistate_d59_dofs128(torch_mod, d59_four_states, prec):
     torch = torch_mod
     ctx, states, X, sid = d59_four_states
