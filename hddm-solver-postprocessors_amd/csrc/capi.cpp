@@ -94,7 +94,7 @@ hddm_lb_ctx::~hddm_lb_ctx()
 	states.clear();
 	scratch.release();
 	for (void* p : { (void*)d_states, (void*)d_cost_order, (void*)d_state_ok,
-		(void*)d_xstage, (void*)d_ystage, (void*)d_sidstage, (void*)d_wbuf, (void*)d_stats })
+		(void*)d_xstage, (void*)d_ystage, (void*)d_sidstage, (void*)d_wbuf, (void*)d_stats, (void*)d_status })
 		dev_free(p);
 	if (scratch_event) (void)hipEventDestroy(scratch_event);
 	for (hipEvent_t e : tev0) (void)hipEventDestroy(e);
@@ -103,6 +103,16 @@ hddm_lb_ctx::~hddm_lb_ctx()
 }
 
 extern "C" const char* hddm_lb_last_error(void) { return error_buffer(); }
+
+extern "C" int hddm_lb_device_status(hddm_lb_ctx* ctx, uint32_t* status, int clear)
+{
+	if (!ctx || !status) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "device_status: NULL argument");
+	DeviceGuard g(ctx->device);
+	HIP_TRY(hipDeviceSynchronize());
+	HIP_TRY(hipMemcpy(status, ctx->d_status, sizeof(uint32_t), hipMemcpyDeviceToHost));
+	if (clear) HIP_TRY(hipMemset(ctx->d_status, 0, sizeof(uint32_t)));
+	return HDDM_LB_SUCCESS;
+}
 
 extern "C" int hddm_lb_device_count(int* count)
 {
@@ -132,6 +142,12 @@ extern "C" int hddm_lb_create(int device, hddm_lb_ctx** out)
 	ctx->num_cus = cus;
 	ctx->max_lds = (size_t)lds;
 	HIP_TRY(hipEventCreateWithFlags(&ctx->scratch_event, hipEventDisableTiming));
+	{
+		int rc = 0;
+		ctx->d_status = (uint32_t*)dev_alloc(sizeof(uint32_t), rc);
+		if (rc) return rc;
+		HIP_TRY(hipMemset(ctx->d_status, 0, sizeof(uint32_t)));
+	}
 	for (int i = 0; i < hddm_lb_ctx::kTimingRing; i++)
 	{
 		hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -173,6 +189,9 @@ extern "C" int hddm_lb_set_option(hddm_lb_ctx* ctx, int option, int64_t value)
 	case HDDM_LB_OPT_WALK:
 		if (value < 0 || value > 2) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "set_option: walk must be 0, 1 or 2");
 		ctx->opt_walk = (int)value; break;
+	case HDDM_LB_OPT_FORMULATION:
+		if (value < 0 || value > 2) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "set_option: formulation must be 0, 1 or 2");
+		ctx->opt_form = (int)value; break;
 	default:
 		return set_error(HDDM_LB_ERROR_INVALID_VALUE, "set_option: unknown option %d", option);
 	}
@@ -440,6 +459,10 @@ TuneConfig default_config(hddm_lb_ctx* ctx, int prec)
 	if (!slab) slab = xbanks_for(ctx->dim) == 4 && ctx->dt_needed >= 2 ? 2 : 1;
 	if (prec == HDDM_LB_FP64) slab = 1;
 	c.slab = slab;
+	// automatic: wide states (D > 32) of two or more dof tiles are walked once
+	// by a walker wave feeding one consumer wave per tile
+	c.ws = ctx->opt_form == 2 || (ctx->opt_form == 0 && xbanks_for(ctx->dim) == 4 && ctx->dt_needed >= 2);
+	c.nc = ctx->dt_needed >= 2 ? 4 : 2;
 	return c;
 }
 
@@ -456,24 +479,29 @@ int enqueue_forward(hddm_lb_ctx* ctx, Scratch& sc, int64_t npoints, const double
 	HIP_TRY(launch_fill_u32(sc.hist, nb, 0u, st));
 	HIP_TRY(launch_bucket_key(npoints, x, ldx, sid, ctx->d_state_ok, ns, ctx->dim, bb, sc.bkey, sc.hist, st));
 	HIP_TRY(launch_scan_plan(sc.hist, sc.cursor, nb, bb, ns, ctx->d_states, ctx->d_cost_order,
-		sc.plans, sc.hdr, cfg.walk == kWalkLds ? cfg.waves : 1, 0, st));
+		sc.plans, sc.hdr, !cfg.ws && cfg.walk == kWalkLds ? cfg.waves : 1, 0, st));
 	HIP_TRY(launch_scatter(npoints, sc.bkey, sc.cursor, sc.perm, st));
 
-	const void* kern = sweep_kernel_ptr(prec, cfg.slab, xbanks_for(ctx->dim), ctx->fmax_all, cfg.walk);
+	const void* kern = cfg.ws ? sweep_ws_kernel_ptr(prec, xbanks_for(ctx->dim), ctx->fmax_all, cfg.nc)
+		: sweep_kernel_ptr(prec, cfg.slab, xbanks_for(ctx->dim), ctx->fmax_all, cfg.walk);
 	if (!kern) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "forward: no sweep kernel for precision %d, dim %d", prec, ctx->dim);
-	const size_t lds = sweep_lds_bytes(prec, cfg.waves, ctx->fmax_all, cfg.walk);
+	const int waves = cfg.ws ? cfg.nc + 1 : cfg.waves;
+	const size_t lds = cfg.ws ? sweep_ws_lds_bytes(prec, ctx->fmax_all) : sweep_lds_bytes(prec, cfg.waves, ctx->fmax_all, cfg.walk);
 	HIP_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
 	int per_cu = 0;
-	HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, cfg.waves * kWave, lds));
+	HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, waves * kWave, lds));
 	if (per_cu < 1)
-		return set_error(HDDM_LB_ERROR_UNSUPPORTED, "forward: sweep kernel does not fit a CU (%d waves)", cfg.waves);
+		return set_error(HDDM_LB_ERROR_UNSUPPORTED, "forward: sweep kernel does not fit a CU (%d waves)", waves);
 	const int grid = per_cu * ctx->num_cus;
+	// dof tiles per launch: the WS sweep's consumers cover 4 tiles (nc 4) or 1
+	const int tiles = cfg.ws ? (cfg.nc == 4 ? 4 : 1) : cfg.slab;
 
 	SweepArgs a;
 	std::memset(&a, 0, sizeof(a));
 	a.states = ctx->d_states; a.plans = sc.plans; a.cost_order = ctx->d_cost_order; a.hdr = sc.hdr;
 	a.perm = sc.perm; a.x = x; a.ldx = ldx; a.y = y; a.ldy = ldy; a.partial = sc.partial;
 	a.nstates = ns; a.dim = ctx->dim;
+	a.status = ctx->d_status;
 	if (ctx->opt_stats && !in_capture)
 	{
 		if (!ctx->d_stats)
@@ -485,14 +513,14 @@ int enqueue_forward(hddm_lb_ctx* ctx, Scratch& sc, int64_t npoints, const double
 		HIP_TRY(launch_fill_u32((uint32_t*)ctx->d_stats, 2 * hddm_lb_ctx::kStatCount, 0u, st));
 		a.stats = ctx->d_stats;
 	}
-	const int nslab = (ctx->dt_needed + cfg.slab - 1) / cfg.slab;
+	const int nslab = (ctx->dt_needed + tiles - 1) / tiles;
 	const int slot = (int)(ctx->timed_calls % hddm_lb_ctx::kTimingRing);
 	if (timed) HIP_TRY(hipEventRecord(ctx->tev0[slot], st));
 	for (int sl = 0; sl < nslab; sl++)
 	{
-		a.dt_base = sl * cfg.slab;
+		a.dt_base = sl * tiles;
 		if (sl > 0) HIP_TRY(launch_reset_counter(sc.hdr, st));
-		HIP_TRY(launch_sweep(kern, grid, cfg.waves, lds, a, st));
+		HIP_TRY(launch_sweep(kern, grid, waves, lds, a, st));
 	}
 	if (timed)
 	{

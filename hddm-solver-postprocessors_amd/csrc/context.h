@@ -60,6 +60,8 @@ struct TuneConfig
 	int waves = 4;     // waves per workgroup
 	int walk = 1;      // kWalkSmem (1) or kWalkLds (0)
 	int slab = 1;      // dof tiles of 32 per sweep pass
+	int ws = 0;        // 1: wave-specialized sweep (walker + consumer waves)
+	int nc = 4;        // consumer waves of the wave-specialized sweep (2 or 4)
 	int64_t seg = -1;  // node segment length used while tuning (-1: context option)
 };
 
@@ -81,6 +83,7 @@ struct hddm_lb_ctx
 	bool opt_timing = false;
 	int opt_slab = 0;
 	int opt_walk = 0;      // 0 automatic, 1 scalar loads, 2 LDS tiles
+	int opt_form = 0;      // 0 automatic, 1 one wave walks and contracts, 2 wave-specialized
 
 	// device tables
 	bool tables_dirty = true;
@@ -110,9 +113,10 @@ struct hddm_lb_ctx
 
 	// measurement
 	bool opt_stats = false;
-	// walk ops, prunes, chunks contracted, work items, MFMA instructions issued
-	static constexpr int kStatCount = 5;
+	// walk ops, prunes, chunks contracted, work items, MFMA instructions issued, nnz pairs
+	static constexpr int kStatCount = 6;
 	unsigned long long* d_stats = nullptr;
+	uint32_t* d_status = nullptr;   // HDDM_LB_STATUS_* bits raised by kernels
 	// ring of event pairs bracketing the sweep kernels of timed forwards
 	static constexpr int kTimingRing = 256;
 	std::vector<hipEvent_t> tev0, tev1;

@@ -24,7 +24,8 @@ struct SweepArgs
 	int nstates;
 	int dim;
 	int dt_base;               // first dof tile of 32 handled by this pass
-	unsigned long long* stats; // optional: ops evaluated, prunes, chunks contracted, items
+	unsigned long long* stats; // optional: ops evaluated, prunes, chunks contracted, items, MFMAs, nnz
+	uint32_t* status;          // device status bits (HDDM_LB_STATUS_*)
 };
 
 // How a sweep reads the op stream: each wave through scalar loads, waves
@@ -37,6 +38,11 @@ constexpr int kWalkSmem = 1;
 // per pass, register banks of 16 coordinates, max node depth, walk); NULL
 // when unsupported. FP64 always runs one dof tile per pass.
 const void* sweep_kernel_ptr(int prec, int dt, int xbanks, int depth, int walk);
+// Wave-specialized sweep (sweep_ws.hip): a walker wave and nc consumer waves
+// (2: one dof tile per pass, each consumer one half of it; 4: four tiles per
+// pass); NULL when unsupported.
+const void* sweep_ws_kernel_ptr(int prec, int xbanks, int depth, int nc);
+size_t sweep_ws_lds_bytes(int prec, int depth);
 // Register banks needed for dim (0: dim above 64 is unsupported).
 int xbanks_for(int dim);
 

@@ -26,7 +26,8 @@ FP64, F16X2, BF16X3, F32 = 0, 1, 2, 3
 PRECISIONS = {"fp64": FP64, "f16x2": F16X2, "bf16x3": BF16X3, "f32": F32}
 
 OPT_AUTOTUNE, OPT_SEGMENT_NODES, OPT_WAVES_PER_WG, OPT_SORT_POINTS = 1, 2, 3, 4
-OPT_TIMING, OPT_SLAB_TILES, OPT_STATS, OPT_WALK = 5, 6, 8, 9
+OPT_TIMING, OPT_SLAB_TILES, OPT_STATS, OPT_WALK, OPT_FORMULATION = 5, 6, 8, 9, 10
+STATUS_WAIT_LIMIT = 1
 
 SUCCESS = 0
 ERROR_NAMES = {1: "INVALID_VALUE", 2: "NO_DEVICE", 3: "OUT_OF_MEMORY", 4: "HIP",
@@ -70,6 +71,7 @@ _SIGS = {
     "hddm_lb_create": (_i32, [_i32, _c.POINTER(_vp)]),
     "hddm_lb_destroy": (_i32, [_vp]),
     "hddm_lb_last_error": (_c.c_char_p, []),
+    "hddm_lb_device_status": (_i32, [_vp, _c.POINTER(_c.c_uint32), _i32]),
     "hddm_lb_set_option": (_i32, [_vp, _i32, _i64]),
     "hddm_lb_upload_state": (_i32, [_vp, _i32, _i32, _i32, _i32, _vp, _vp, _i64, _vp, _i64]),
     "hddm_lb_state_info": (_i32, [_vp, _i32, _c.POINTER(_i32), _c.POINTER(_i32), _c.POINTER(_i32), _c.POINTER(_i32)]),
@@ -334,9 +336,16 @@ class Context:
         _check(lib.hddm_lb_sweep_time_summary(self._h, ctypes.byref(n), ctypes.byref(tot), ctypes.byref(mn), ctypes.byref(mx)))
         return n.value, tot.value, mn.value, mx.value
 
+    def device_status(self, clear=True):
+        """Status bits raised by this context's kernels since the last clearing call (0: none)."""
+        v = _c.c_uint32(0)
+        _check(lib.hddm_lb_device_status(self._h, _c.byref(v), int(clear)))
+        return v.value
+
     def last_stats(self, n=4):
         """(ops evaluated, subtrees pruned, chunks contracted, work items[, MFMA
-        instructions issued]) of the last forward: the first n (at most 5)."""
+        instructions issued[, non-zero (point, node) pairs]]) of the last
+        forward: the first n (at most 6)."""
         st = np.zeros(n, np.uint64)
         _check(lib.hddm_lb_last_stats_n(self._h, _ptr(st), n))
         return tuple(int(v) for v in st)

@@ -64,6 +64,12 @@ typedef struct hddm_lb_grid hddm_lb_grid;
 /* option 7 is unused: MFMA partial sums are folded into FP64 after every chunk */
 #define HDDM_LB_OPT_STATS         8  /* 1: count walk ops, prunes, contracted chunks and items (hddm_lb_last_stats) */
 #define HDDM_LB_OPT_WALK          9  /* op stream source: 0 automatic, 1 scalar loads per wave, 2 LDS tiles per workgroup */
+#define HDDM_LB_OPT_FORMULATION  10  /* sweep: 0 automatic, 1 each wave walks and contracts its block,
+                                        2 a walker wave feeds MFMA consumer waves (one per 32-dof tile) */
+
+/* bits of the device status word (hddm_lb_device_status) */
+#define HDDM_LB_STATUS_WAIT_LIMIT 1u /* a wave of a wave-specialized kernel stopped waiting for another
+                                        wave of its workgroup: the results of that call are invalid */
 
 /* ---- devices and contexts ---- */
 
@@ -76,6 +82,10 @@ int hddm_lb_destroy(hddm_lb_ctx* ctx);
 
 /* Text of the last error on this thread (never NULL). */
 const char* hddm_lb_last_error(void);
+
+/* Waits for the device and returns the status bits kernels of this context
+ * have raised since the last clearing call (0: none); clear = 1 resets them. */
+int hddm_lb_device_status(hddm_lb_ctx* ctx, uint32_t* status, int clear);
 
 int hddm_lb_set_option(hddm_lb_ctx* ctx, int option, int64_t value);
 
@@ -150,9 +160,12 @@ int hddm_lb_sweep_time_summary(hddm_lb_ctx* ctx, int64_t* ncalls, double* total_
  * contracted (8 nodes on the FP64 path, 32 on F32, 16 on F16X2 / BF16X3),
  * [3] work items. Waits for the forward to finish. */
 int hddm_lb_last_stats(hddm_lb_ctx* ctx, uint64_t* stats);
-/* The first n (1 to 5) of the same counters; stats[4] is the number of MFMA
+/* The first n (1 to 6) of the same counters; stats[4] is the number of MFMA
  * instructions the contraction issued (v_mfma_f32_32x32x2f32 on F32,
- * v_mfma_f32_32x32x16_{f16,bf16} on the split paths, 0 on FP64). */
+ * v_mfma_f32_32x32x16_{f16,bf16} on the split paths, v_mfma_f64_16x16x4f64
+ * on the FP64 path of the wave-specialized sweep, 0 on the FP64 VALU path);
+ * stats[5] is the number of (point, node) pairs with a non-zero basis value
+ * that the contraction received (wave-specialized sweep only, else 0). */
 int hddm_lb_last_stats_n(hddm_lb_ctx* ctx, uint64_t* stats, int n);
 
 /* ---- host-side node layout, for inspection without a device ---- */
