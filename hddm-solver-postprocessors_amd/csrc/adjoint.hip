@@ -31,26 +31,7 @@ namespace hddm_lb {
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
 
-struct AdjointArgs
-{
-	const StateDesc* states;
-	const StatePlan* plans;
-	const int32_t* cost_order;
-	PlanHeader* hdr;
-	const uint32_t* perm;
-	const double* x;
-	int64_t ldx;
-	const double* g;
-	int64_t ldg;
-	double* dx;        // may be null
-	int64_t lddx;
-	double* dxpart;    // node-segment partials of dx for split states ([nseg][count][dim] per state)
-	double* ds;        // may be null; state s starts at states[s].ds_offset
-	int64_t ds_xcd;    // > 0: ds holds one copy of this many elements per XCD
-	int nstates;
-	int dim;
-	int dt_base;       // dof slab of 32 handled by this launch
-};
+
 
 // Per-wave LDS of the dx walk: P, R [PM][64]; dx [dim rounded to even][64]
 // (doubles); per depth of the path the edge's op words {dim, i, -j, 0}
@@ -492,14 +473,39 @@ using namespace hddm_lb;
 
 #define HIP_TRY(expr) do { hipError_t e_ = (expr); if (e_ != hipSuccess) return hip_fail(e_, #expr, __FILE__, __LINE__); } while (0)
 
+// After the adjoint kernels: dx of split states summed over node segments,
+// dS summed over the per-XCD copies, both in a fixed order.
+static int adjoint_tail(hddm_lb_ctx* ctx, Scratch& sc, double* dx, int64_t lddx, double* dsurplus,
+	int64_t ds_total, bool xcd_copies, int ns, hipStream_t st)
+{
+	if (dx && ctx->max_nseg > 1)
+	{
+		hipLaunchKernelGGL(reduce_dx_partials_kernel, dim3(1024, ns), dim3(64, 4), 0, st,
+			(const StateDesc*)ctx->d_states, (const StatePlan*)sc.plans, (const uint32_t*)sc.perm,
+			(const double*)sc.dxpart, ctx->dim, dx, lddx);
+		HIP_TRY(hipGetLastError());
+	}
+	if (xcd_copies)
+	{
+		int blocks = (int)std::min<int64_t>(4096, (ds_total + 255) / 256);
+		hipLaunchKernelGGL(sum_ds_copies_kernel, dim3(std::max(blocks, 1)), dim3(256), 0, st,
+			(const double*)sc.dspart, ds_total, dsurplus);
+		HIP_TRY(hipGetLastError());
+	}
+	return scratch_mark(ctx, st);
+}
+
 extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 	const double* x, int64_t ldx, const int32_t* state_ids,
 	const double* gbar, int64_t ldg, double* dx, int64_t lddx,
 	double* dsurplus, int precision, void* stream)
 {
 	if (!ctx) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "adjoint: ctx is NULL");
-	if (precision != HDDM_LB_FP64)
-		return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: precision %d is not available; use HDDM_LB_FP64", precision);
+	if (precision != HDDM_LB_FP64 && precision != HDDM_LB_F32)
+		return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: precision %d is not available; use HDDM_LB_FP64 or HDDM_LB_F32",
+			precision);
+	if (precision == HDDM_LB_F32 && ctx->opt_form == 1)
+		return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: HDDM_LB_F32 runs on the wave-specialized kernel only");
 	if (npoints < 0 || npoints > ((int64_t)1 << 31) - 1)
 		return set_error(HDDM_LB_ERROR_INVALID_VALUE, "adjoint: npoints %lld out of range", (long long)npoints);
 	if (ctx->dim < 1) return set_error(HDDM_LB_ERROR_STATE, "adjoint: no state is loaded");
@@ -558,6 +564,49 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 		}
 	}
 
+	Scratch& sc = ctx->scratch;
+	const int ns = ctx->tables_n;
+	const int bb = bucket_bits(npoints, ns, ctx->opt_sort);
+	const int64_t nb = (int64_t)(ns + 1) << bb;
+	AdjointArgs a;
+	std::memset(&a, 0, sizeof(a));
+	a.states = ctx->d_states; a.cost_order = ctx->d_cost_order; a.hdr = sc.hdr;
+	a.perm = sc.perm; a.x = x; a.ldx = ldx; a.g = gbar; a.ldg = ldg; a.dx = dx; a.lddx = lddx;
+	a.dxpart = sc.dxpart; a.plans = sc.plans;
+	a.ds = xcd_copies ? sc.dspart : dsurplus; a.ds_xcd = xcd_copies ? ds_total : 0;
+	a.nstates = ns; a.dim = ctx->dim;
+	a.status = ctx->d_status;
+
+	const bool ws = precision == HDDM_LB_F32 || ctx->opt_form != 1;
+	if (ws)
+	{
+		// one walker wave and four consumer waves per workgroup, four dof
+		// tiles per launch
+		const void* kern = adjoint_ws_kernel_ptr(precision, xbanks_for(ctx->dim), ctx->fmax_all);
+		if (!kern) return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: dim %d is not supported", ctx->dim);
+		const size_t lds = adjoint_ws_lds_bytes(precision, ctx->fmax_all, ctx->dim);
+		if (lds > ctx->max_lds)
+			return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: LDS of one workgroup (%zu B) exceeds the CU", lds);
+		HIP_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+		int per_cu = 0;
+		HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 5 * kWave, lds));
+		if (per_cu < 1) return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: kernel does not fit a CU");
+		HIP_TRY(launch_fill_u32(sc.hist, nb, 0u, st));
+		HIP_TRY(launch_bucket_key(npoints, x, ldx, state_ids, ctx->d_state_ok, ns, ctx->dim, bb, sc.bkey, sc.hist, st));
+		HIP_TRY(launch_scan_plan(sc.hist, sc.cursor, nb, bb, ns, ctx->d_states, ctx->d_cost_order, sc.plans, sc.hdr, 1,
+			ctx->dim, st));
+		HIP_TRY(launch_scatter(npoints, sc.bkey, sc.cursor, sc.perm, st));
+		const int npass = (ctx->dt_needed + 3) / 4;
+		for (int ps = 0; ps < npass; ps++)
+		{
+			a.dt_base = 4 * ps;
+			if (ps > 0) HIP_TRY(launch_reset_counter(sc.hdr, st));
+			void* params[] = { (void*)&a };
+			HIP_TRY(hipLaunchKernel(kern, dim3(per_cu * ctx->num_cus), dim3(5 * kWave), params, lds, st));
+		}
+		return adjoint_tail(ctx, sc, dx, lddx, dsurplus, ds_total, xcd_copies, ns, st);
+	}
+
 	// dx and dS together: the edge-sum walk for dx, then the forward's walk
 	// for dS (two lighter kernels run faster than one that carries both)
 	int modes[2] = { dx ? kAdjDx : kAdjDs, -1 };
@@ -591,23 +640,12 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 		kerns[m] = kern; grids[m] = best_per_cu * ctx->num_cus; groups[m] = bestG; ldss[m] = lds;
 	}
 
-	Scratch& sc = ctx->scratch;
-	const int ns = ctx->tables_n;
-	const int bb = bucket_bits(npoints, ns, ctx->opt_sort);
-	const int64_t nb = (int64_t)(ns + 1) << bb;
 	HIP_TRY(launch_fill_u32(sc.hist, nb, 0u, st));
 	HIP_TRY(launch_bucket_key(npoints, x, ldx, state_ids, ctx->d_state_ok, ns, ctx->dim, bb, sc.bkey, sc.hist, st));
 	HIP_TRY(launch_scan_plan(sc.hist, sc.cursor, nb, bb, ns, ctx->d_states, ctx->d_cost_order, sc.plans, sc.hdr, 1,
 		ctx->dim, st));
 	HIP_TRY(launch_scatter(npoints, sc.bkey, sc.cursor, sc.perm, st));
 
-	AdjointArgs a;
-	std::memset(&a, 0, sizeof(a));
-	a.states = ctx->d_states; a.plans = sc.plans; a.cost_order = ctx->d_cost_order; a.hdr = sc.hdr;
-	a.perm = sc.perm; a.x = x; a.ldx = ldx; a.g = gbar; a.ldg = ldg; a.dx = dx; a.lddx = lddx;
-	a.dxpart = sc.dxpart;
-	a.ds = xcd_copies ? sc.dspart : dsurplus; a.ds_xcd = xcd_copies ? ds_total : 0;
-	a.nstates = ns; a.dim = ctx->dim;
 	bool first = true;
 	for (int m = 0; m < 2 && kerns[m]; m++)
 		for (int sl = 0; sl < ctx->dt_needed; sl++)
@@ -618,19 +656,5 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 			void* params[] = { (void*)&a };
 			HIP_TRY(hipLaunchKernel(kerns[m], dim3(grids[m]), dim3(groups[m] * kWave), params, ldss[m], st));
 		}
-	if (dx && ctx->max_nseg > 1)
-	{
-		hipLaunchKernelGGL(reduce_dx_partials_kernel, dim3(1024, ns), dim3(64, 4), 0, st,
-			(const StateDesc*)ctx->d_states, (const StatePlan*)sc.plans, (const uint32_t*)sc.perm,
-			(const double*)sc.dxpart, ctx->dim, dx, lddx);
-		HIP_TRY(hipGetLastError());
-	}
-	if (xcd_copies)
-	{
-		int blocks = (int)std::min<int64_t>(4096, (ds_total + 255) / 256);
-		hipLaunchKernelGGL(sum_ds_copies_kernel, dim3(std::max(blocks, 1)), dim3(256), 0, st,
-			(const double*)sc.dspart, ds_total, dsurplus);
-		HIP_TRY(hipGetLastError());
-	}
-	return scratch_mark(ctx, st);
+	return adjoint_tail(ctx, sc, dx, lddx, dsurplus, ds_total, xcd_copies, ns, st);
 }

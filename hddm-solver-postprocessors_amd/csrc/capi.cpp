@@ -248,6 +248,7 @@ extern "C" int hddm_lb_upload_state(hddm_lb_ctx* ctx, int istate, int dim, int n
 	d.s16 = (const uint16_t*)up(L.s16.data(), L.s16.size() * sizeof(uint16_t));
 	d.sbf = (const uint16_t*)up(L.sbf.data(), L.sbf.size() * sizeof(uint16_t));
 	d.s32 = (const float*)up(L.s32.data(), L.s32.size() * sizeof(float));
+	d.s32r = (const float*)up(L.s32r.data(), L.s32r.size() * sizeof(float));
 	d.iscale = (const double*)up(L.iscale.data(), L.iscale.size() * sizeof(double));
 	d.order = (const int32_t*)up(L.order.data(), L.order.size() * sizeof(int32_t));
 	d.lomask = (const uint64_t*)up(L.lomask.data(), L.lomask.size() * sizeof(uint64_t));

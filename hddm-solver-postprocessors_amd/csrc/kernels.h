@@ -28,6 +28,29 @@ struct SweepArgs
 	uint32_t* status;          // device status bits (HDDM_LB_STATUS_*)
 };
 
+// Arguments of the adjoint kernels (adjoint.hip, adjoint_ws.hip).
+struct AdjointArgs
+{
+	const StateDesc* states;
+	const StatePlan* plans;
+	const int32_t* cost_order;
+	PlanHeader* hdr;
+	const uint32_t* perm;
+	const double* x;
+	int64_t ldx;
+	const double* g;
+	int64_t ldg;
+	double* dx;        // may be null
+	int64_t lddx;
+	double* dxpart;    // node-segment partials of dx for split states ([nseg][count][dim] per state)
+	double* ds;        // may be null; state s starts at states[s].ds_offset
+	int64_t ds_xcd;    // > 0: ds holds one copy of this many elements per XCD
+	int nstates;
+	int dim;
+	int dt_base;       // dof slab of 32 handled by this launch
+	uint32_t* status;  // device status bits (HDDM_LB_STATUS_*)
+};
+
 // How a sweep reads the op stream: each wave through scalar loads, waves
 // independent (kWalkSmem), or the workgroup through shared LDS tiles, its
 // waves walking neighbouring point blocks in step (kWalkLds).
@@ -43,6 +66,10 @@ const void* sweep_kernel_ptr(int prec, int dt, int xbanks, int depth, int walk);
 // pass); NULL when unsupported.
 const void* sweep_ws_kernel_ptr(int prec, int xbanks, int depth, int nc);
 size_t sweep_ws_lds_bytes(int prec, int depth);
+// Wave-specialized adjoint (adjoint_ws.hip): a walker wave and 4 consumer
+// waves; FP64 or F32; 4 dof tiles per launch. NULL when unsupported.
+const void* adjoint_ws_kernel_ptr(int prec, int xbanks, int depth);
+size_t adjoint_ws_lds_bytes(int prec, int depth, int dim);
 // Register banks needed for dim (0: dim above 64 is unsupported).
 int xbanks_for(int dim);
 

@@ -64,6 +64,7 @@ struct StateDesc
 	const uint16_t* s16;            // [nchunks][2][dofs_pad][16] f16 hi, lo (scaled)
 	const uint16_t* sbf;            // [nchunks][3][dofs_pad][16] bf16 terms
 	const float* s32;               // [nchunks][dofs_pad][2][8] f32
+	const float* s32r;              // [nno * dofs_pad] f32 rows, trie order
 	const double* iscale;           // [dofs_pad] 2^-sigma_c undoing the f16 scaling
 	const int32_t* order;           // trie position -> uploaded node index
 	const uint64_t* lomask;         // [nno] dims whose support lies in [0, 1/2]

@@ -265,6 +265,8 @@ int build_layout(int dim, int nno, int dofs,
 	L.s16.assign((size_t)NC * 2 * DP * kChunk, 0);
 	L.sbf.assign((size_t)NC * 3 * DP * kChunk, 0);
 	L.s32.assign((size_t)NC * DP * kChunk, 0.0f);
+	L.s32r.assign((size_t)nno * DP, 0.0f);
+	for (size_t e = 0; e < L.s64.size(); e++) L.s32r[e] = (float)L.s64[e];
 	for (int n = 0; n < nno; n++)
 	{
 		int c = n / kChunk, pos = n % kChunk;

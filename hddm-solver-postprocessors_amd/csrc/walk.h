@@ -54,7 +54,9 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 // copied to s91 on entry and restored at the one exit (L_done), so the
 // statement leaves m0 as it found it.
 // The chunk buffer holds doubles (rows of 512 bytes) or, for the F32 path,
-// floats (rows of 256 bytes): HW_BSH and HW_BST.
+// floats (rows of 256 bytes): HW_BROW (the row offset of a chunk position,
+// a shift for rows of 256 or 512 bytes, else a multiply: padded rows) and
+// HW_BST (the store).
 // Issues the read of P[t] for the op whose ctrl word is C1 (into HW_C, its
 // address in HW_D).
 #define HDDM_WALK_PREP(C1)                                                             \
@@ -86,7 +88,7 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 	"s_cbranch_scc1 L_new" X "_%=\n"                                                   \
 	"s_bfe_u32 s89, " X1 ", %[cf]\n"                                                   \
 	"s_bitset1_b32 %[mask], s89\n"                                                     \
-	"s_lshl_b32 s89, s89, " HW_BSH "\n"                                                \
+	HW_BROW                                                                            \
 	"v_add_u32 " HW_E ", s89, %[bl]\n"                                                 \
 	HW_BST                                                                             \
 	"L_adv" X "_%=:\n"                                                                 \
@@ -159,14 +161,15 @@ __device__ __forceinline__ uint32_t lds_offset(const void* p)
 	  [node] "=&s"(node), [val] "=&v"(val), "+{s[76:79]}"(opv)
 #define HDDM_WALK_INPUTS                                                               \
 	: [ob] "s"(ob), [qe] "s"(qe), [chunk] "s"(chunk), [pl] "v"(pl), [bl] "v"(bl),      \
-	  [cs] "i"(8 + CB), [cf] "i"((CB << 16) | 8)
+	  [cs] "i"(8 + CB), [cf] "i"((CB << 16) | 8), [rb] "i"(RB)
 #define HDDM_WALK_SCLOBBERS                                                            \
 	"s80", "s81", "s82", "s83", "s84", "s85", "s86", "s87", "s88", "s89", "s90",       \
 	"s91", "s94", "s95", "vcc", "scc", "memory"
 
 // One coordinate bank (D <= 16): x in v[64:95], temporaries v[96:103].
-// BF: the chunk buffer holds floats (rows of 256 bytes), else doubles (512).
-template <int CB, bool BF>
+// BF: the chunk buffer holds floats, else doubles; RB: bytes per row (256 or
+// 512 for 64 values, more for padded rows).
+template <int CB, bool BF, int RB>
 __device__ __forceinline__ int walk_op_stream(const char* ob, uint32_t& qb, uint32_t qe, int chunk,
 	u4_t& opv, uint32_t& mask, uint32_t& nops, uint32_t& nprunes, uint32_t pl, uint32_t bl,
 	const d16_t& xv, double& val)
@@ -181,22 +184,40 @@ __device__ __forceinline__ int walk_op_stream(const char* ob, uint32_t& qb, uint
 #define HW_C "v[100:101]"
 #define HW_D "v102"
 #define HW_E "v103"
-	if constexpr (BF)
+	if constexpr (BF && RB == 256)
 	{
-#define HW_BSH "8"
+#define HW_BROW "s_lshl_b32 s89, s89, 8\n"
 #define HW_BST "v_cvt_f32_f64 " HW_A0 ", " HW_A "\n" "ds_write_b32 " HW_E ", " HW_A0 "\n"
 		asm volatile(HDDM_WALK_ASM HDDM_WALK_OUTPUTS HDDM_WALK_INPUTS, "{v[64:95]}"(xv)
 			: HDDM_WALK_SCLOBBERS, "v96", "v97", "v98", "v99", "v100", "v101", "v102", "v103");
-#undef HW_BSH
+#undef HW_BROW
+#undef HW_BST
+	}
+	else if constexpr (!BF && RB == 512)
+	{
+#define HW_BROW "s_lshl_b32 s89, s89, 9\n"
+#define HW_BST "ds_write_b64 " HW_E ", " HW_A "\n"
+		asm volatile(HDDM_WALK_ASM HDDM_WALK_OUTPUTS HDDM_WALK_INPUTS, "{v[64:95]}"(xv)
+			: HDDM_WALK_SCLOBBERS, "v96", "v97", "v98", "v99", "v100", "v101", "v102", "v103");
+#undef HW_BROW
+#undef HW_BST
+	}
+	else if constexpr (BF)
+	{
+#define HW_BROW "s_mul_i32 s89, s89, %[rb]\n"
+#define HW_BST "v_cvt_f32_f64 " HW_A0 ", " HW_A "\n" "ds_write_b32 " HW_E ", " HW_A0 "\n"
+		asm volatile(HDDM_WALK_ASM HDDM_WALK_OUTPUTS HDDM_WALK_INPUTS, "{v[64:95]}"(xv)
+			: HDDM_WALK_SCLOBBERS, "v96", "v97", "v98", "v99", "v100", "v101", "v102", "v103");
+#undef HW_BROW
 #undef HW_BST
 	}
 	else
 	{
-#define HW_BSH "9"
+#define HW_BROW "s_mul_i32 s89, s89, %[rb]\n"
 #define HW_BST "ds_write_b64 " HW_E ", " HW_A "\n"
 		asm volatile(HDDM_WALK_ASM HDDM_WALK_OUTPUTS HDDM_WALK_INPUTS, "{v[64:95]}"(xv)
 			: HDDM_WALK_SCLOBBERS, "v96", "v97", "v98", "v99", "v100", "v101", "v102", "v103");
-#undef HW_BSH
+#undef HW_BROW
 #undef HW_BST
 	}
 #undef HW_A
@@ -213,7 +234,7 @@ __device__ __forceinline__ int walk_op_stream(const char* ob, uint32_t& qb, uint
 
 // Two coordinate banks (16 < D <= 32): x in v[64:127] (the indexed source
 // reaches dims 16-31 through the same index), temporaries v[128:135].
-template <int CB, bool BF>
+template <int CB, bool BF, int RB>
 __device__ __forceinline__ int walk_op_stream2(const char* ob, uint32_t& qb, uint32_t qe, int chunk,
 	u4_t& opv, uint32_t& mask, uint32_t& nops, uint32_t& nprunes, uint32_t pl, uint32_t bl,
 	const d16_t& xv0, const d16_t& xv1, double& val)
@@ -228,22 +249,40 @@ __device__ __forceinline__ int walk_op_stream2(const char* ob, uint32_t& qb, uin
 #define HW_C "v[132:133]"
 #define HW_D "v134"
 #define HW_E "v135"
-	if constexpr (BF)
+	if constexpr (BF && RB == 256)
 	{
-#define HW_BSH "8"
+#define HW_BROW "s_lshl_b32 s89, s89, 8\n"
 #define HW_BST "v_cvt_f32_f64 " HW_A0 ", " HW_A "\n" "ds_write_b32 " HW_E ", " HW_A0 "\n"
 		asm volatile(HDDM_WALK_ASM HDDM_WALK_OUTPUTS HDDM_WALK_INPUTS, "{v[64:95]}"(xv0), "{v[96:127]}"(xv1)
 			: HDDM_WALK_SCLOBBERS, "v128", "v129", "v130", "v131", "v132", "v133", "v134", "v135");
-#undef HW_BSH
+#undef HW_BROW
+#undef HW_BST
+	}
+	else if constexpr (!BF && RB == 512)
+	{
+#define HW_BROW "s_lshl_b32 s89, s89, 9\n"
+#define HW_BST "ds_write_b64 " HW_E ", " HW_A "\n"
+		asm volatile(HDDM_WALK_ASM HDDM_WALK_OUTPUTS HDDM_WALK_INPUTS, "{v[64:95]}"(xv0), "{v[96:127]}"(xv1)
+			: HDDM_WALK_SCLOBBERS, "v128", "v129", "v130", "v131", "v132", "v133", "v134", "v135");
+#undef HW_BROW
+#undef HW_BST
+	}
+	else if constexpr (BF)
+	{
+#define HW_BROW "s_mul_i32 s89, s89, %[rb]\n"
+#define HW_BST "v_cvt_f32_f64 " HW_A0 ", " HW_A "\n" "ds_write_b32 " HW_E ", " HW_A0 "\n"
+		asm volatile(HDDM_WALK_ASM HDDM_WALK_OUTPUTS HDDM_WALK_INPUTS, "{v[64:95]}"(xv0), "{v[96:127]}"(xv1)
+			: HDDM_WALK_SCLOBBERS, "v128", "v129", "v130", "v131", "v132", "v133", "v134", "v135");
+#undef HW_BROW
 #undef HW_BST
 	}
 	else
 	{
-#define HW_BSH "9"
+#define HW_BROW "s_mul_i32 s89, s89, %[rb]\n"
 #define HW_BST "ds_write_b64 " HW_E ", " HW_A "\n"
 		asm volatile(HDDM_WALK_ASM HDDM_WALK_OUTPUTS HDDM_WALK_INPUTS, "{v[64:95]}"(xv0), "{v[96:127]}"(xv1)
 			: HDDM_WALK_SCLOBBERS, "v128", "v129", "v130", "v131", "v132", "v133", "v134", "v135");
-#undef HW_BSH
+#undef HW_BROW
 #undef HW_BST
 	}
 #undef HW_A
@@ -259,7 +298,7 @@ __device__ __forceinline__ int walk_op_stream2(const char* ob, uint32_t& qb, uin
 }
 
 // Four coordinate banks (32 < D <= 64): x in v[64:191], temporaries v[192:199].
-template <int CB, bool BF>
+template <int CB, bool BF, int RB>
 __device__ __forceinline__ int walk_op_stream4(const char* ob, uint32_t& qb, uint32_t qe, int chunk,
 	u4_t& opv, uint32_t& mask, uint32_t& nops, uint32_t& nprunes, uint32_t pl, uint32_t bl,
 	const d16_t& xv0, const d16_t& xv1, const d16_t& xv2, const d16_t& xv3, double& val)
@@ -274,22 +313,40 @@ __device__ __forceinline__ int walk_op_stream4(const char* ob, uint32_t& qb, uin
 #define HW_C "v[196:197]"
 #define HW_D "v198"
 #define HW_E "v199"
-	if constexpr (BF)
+	if constexpr (BF && RB == 256)
 	{
-#define HW_BSH "8"
+#define HW_BROW "s_lshl_b32 s89, s89, 8\n"
 #define HW_BST "v_cvt_f32_f64 " HW_A0 ", " HW_A "\n" "ds_write_b32 " HW_E ", " HW_A0 "\n"
 		asm volatile(HDDM_WALK_ASM HDDM_WALK_OUTPUTS HDDM_WALK_INPUTS, "{v[64:95]}"(xv0), "{v[96:127]}"(xv1), "{v[128:159]}"(xv2), "{v[160:191]}"(xv3)
 			: HDDM_WALK_SCLOBBERS, "v192", "v193", "v194", "v195", "v196", "v197", "v198", "v199");
-#undef HW_BSH
+#undef HW_BROW
+#undef HW_BST
+	}
+	else if constexpr (!BF && RB == 512)
+	{
+#define HW_BROW "s_lshl_b32 s89, s89, 9\n"
+#define HW_BST "ds_write_b64 " HW_E ", " HW_A "\n"
+		asm volatile(HDDM_WALK_ASM HDDM_WALK_OUTPUTS HDDM_WALK_INPUTS, "{v[64:95]}"(xv0), "{v[96:127]}"(xv1), "{v[128:159]}"(xv2), "{v[160:191]}"(xv3)
+			: HDDM_WALK_SCLOBBERS, "v192", "v193", "v194", "v195", "v196", "v197", "v198", "v199");
+#undef HW_BROW
+#undef HW_BST
+	}
+	else if constexpr (BF)
+	{
+#define HW_BROW "s_mul_i32 s89, s89, %[rb]\n"
+#define HW_BST "v_cvt_f32_f64 " HW_A0 ", " HW_A "\n" "ds_write_b32 " HW_E ", " HW_A0 "\n"
+		asm volatile(HDDM_WALK_ASM HDDM_WALK_OUTPUTS HDDM_WALK_INPUTS, "{v[64:95]}"(xv0), "{v[96:127]}"(xv1), "{v[128:159]}"(xv2), "{v[160:191]}"(xv3)
+			: HDDM_WALK_SCLOBBERS, "v192", "v193", "v194", "v195", "v196", "v197", "v198", "v199");
+#undef HW_BROW
 #undef HW_BST
 	}
 	else
 	{
-#define HW_BSH "9"
+#define HW_BROW "s_mul_i32 s89, s89, %[rb]\n"
 #define HW_BST "ds_write_b64 " HW_E ", " HW_A "\n"
 		asm volatile(HDDM_WALK_ASM HDDM_WALK_OUTPUTS HDDM_WALK_INPUTS, "{v[64:95]}"(xv0), "{v[96:127]}"(xv1), "{v[128:159]}"(xv2), "{v[160:191]}"(xv3)
 			: HDDM_WALK_SCLOBBERS, "v192", "v193", "v194", "v195", "v196", "v197", "v198", "v199");
-#undef HW_BSH
+#undef HW_BROW
 #undef HW_BST
 	}
 #undef HW_A
@@ -320,9 +377,12 @@ __device__ __forceinline__ int walk_op_stream4(const char* ob, uint32_t& qb, uin
 //   kWalkSmem  scalar loads from the state's op stream, the sequential
 //              successor read one op ahead; waves are independent
 //   kWalkLds   the workgroup's LDS tile (wave-uniform vector reads)
-template <int XB, typename BT, int WALK, int PM, bool XSEL, int CB>
+// ROW: elements per row of the chunk buffer (64, or more for rows padded
+// against LDS bank conflicts of column reads).
+template <int XB, typename BT, int WALK, int PM, bool XSEL, int CB, int ROW = kWave>
 struct Walker
 {
+	static constexpr int kRB = ROW * (int)sizeof(BT);
 	static constexpr int kC = 1 << CB;        // nodes per chunk
 	static constexpr int kCM = kC - 1;
 	double* P;          // [PM][64] in LDS
@@ -402,13 +462,13 @@ struct Walker
 			const uint32_t bl = lds_offset(B) + (uint32_t)lane * (uint32_t)sizeof(BT);
 			int n;
 			if constexpr (XB == 1)
-				n = walk_op_stream<CB, sizeof(BT) == 4>((const char*)gops, qb, qe, chunk, opv, mask, nops, nprunes,
+				n = walk_op_stream<CB, sizeof(BT) == 4, kRB>((const char*)gops, qb, qe, chunk, opv, mask, nops, nprunes,
 					pl, bl, xv0, v);
 			else if constexpr (XB == 2)
-				n = walk_op_stream2<CB, sizeof(BT) == 4>((const char*)gops, qb, qe, chunk, opv, mask, nops, nprunes,
+				n = walk_op_stream2<CB, sizeof(BT) == 4, kRB>((const char*)gops, qb, qe, chunk, opv, mask, nops, nprunes,
 					pl, bl, xv0, xv1, v);
 			else
-				n = walk_op_stream4<CB, sizeof(BT) == 4>((const char*)gops, qb, qe, chunk, opv, mask, nops, nprunes,
+				n = walk_op_stream4<CB, sizeof(BT) == 4, kRB>((const char*)gops, qb, qe, chunk, opv, mask, nops, nprunes,
 					pl, bl, xv0, xv1, xv2, xv3, v);
 			op.dim = opv[0]; op.ctrl = opv[1]; op.ihi = opv[2]; op.njhi = opv[3];
 			q = (int)(qb / (uint32_t)sizeof(Op));
@@ -439,7 +499,7 @@ struct Walker
 				{
 					const int n = (int)(ctrl >> kOpNodeShift);
 					if ((n >> CB) != chunk) { node = n; val = p; return true; }
-					B[(n & kCM) * kWave + lane] = (BT)p;
+					B[(n & kCM) * ROW + lane] = (BT)p;
 					mask |= 1u << (n & kCM);
 				}
 			}
@@ -451,7 +511,7 @@ struct Walker
 	{
 		chunk = node >> CB;
 		mask = 1u << (node & kCM);
-		B[(node & kCM) * kWave + lane] = (BT)val;
+		B[(node & kCM) * ROW + lane] = (BT)val;
 	}
 };
 

@@ -70,6 +70,8 @@ typedef struct hddm_lb_grid hddm_lb_grid;
 /* bits of the device status word (hddm_lb_device_status) */
 #define HDDM_LB_STATUS_WAIT_LIMIT 1u /* a wave of a wave-specialized kernel stopped waiting for another
                                         wave of its workgroup: the results of that call are invalid */
+#define HDDM_LB_STATUS_WALK_MISMATCH 2u /* the adjoint's two walks of one trie disagreed on the surviving
+                                           nodes: the dx of that call is invalid */
 
 /* ---- devices and contexts ---- */
 

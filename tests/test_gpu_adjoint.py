@@ -1,5 +1,10 @@
 """GPU parity of the adjoint (dx, dsurplus) through the C ABI.
 
+Two kernels compute it: the wave-specialized one (default; a walker wave and
+MFMA consumer waves, FP64 or F32) and the per-tile pair of FP64 kernels
+(HDDM_LB_OPT_FORMULATION = 1). F32 is held to 1e-6 normwise, FP64 to the
+bounds below. Every call must leave the device status word clear.
+
 Checks, in FP64:
   * against the oracle's analytic adjoint (gold basis, the product of the
     other factors formed directly): dx and dS to 1e-10 normwise;
@@ -32,16 +37,31 @@ def say(*a):
     print(*a, flush=True)
 
 
-def run_adjoint(torch, ctx, X, G, sid=None):
+# (dx, dS) normwise bounds per precision
+TOL_ADJ = {hddm_lb.FP64: (1e-10, 1e-12), hddm_lb.F32: (1e-6, 1e-6)}
+# (precision, formulation): FP64 on both kernels, F32 on the wave-specialized one
+ADJ_PATHS = [(hddm_lb.FP64, 2), (hddm_lb.FP64, 1), (hddm_lb.F32, 2)]
+
+
+def run_adjoint(torch, ctx, X, G, sid=None, prec=hddm_lb.FP64):
     x = torch.from_numpy(X).cuda()
     g = torch.from_numpy(G).cuda()
     dx = torch.full(X.shape, np.nan, dtype=torch.float64, device="cuda")
     _, total = ctx.surplus_offset(-1)
     ds = torch.full((total,), np.nan, dtype=torch.float64, device="cuda")
     s = None if sid is None else torch.from_numpy(np.asarray(sid, np.int32)).cuda()
-    ctx.adjoint(x, g, dx, ds, s, hddm_lb.FP64)
+    ctx.adjoint(x, g, dx, ds, s, prec)
     torch.cuda.synchronize()
+    assert ctx.device_status() == 0
     return dx.cpu().numpy(), ds.cpu().numpy()
+
+
+def adj_ctx(form=0):
+    ctx = hddm_lb.Context(0)
+    if form:
+        ctx.set_option(hddm_lb.OPT_FORMULATION, form)
+    ctx.device_status(clear=True)
+    return ctx
 
 
 def grid_case(dim, start, maxl, p, seed, target, dofs, n, kink_level):
@@ -52,22 +72,39 @@ def grid_case(dim, start, maxl, p, seed, target, dofs, n, kink_level):
     return ii, jj, S, X, G
 
 
+@pytest.mark.parametrize("prec,form", ADJ_PATHS)
 @pytest.mark.parametrize("dim,dofs,n", [(6, 20, 300), (16, 32, 200), (17, 16, 150), (20, 40, 130), (32, 32, 120),
-                                         (33, 24, 110), (59, 64, 100), (64, 40, 100)])
-def test_adjoint_matches_oracle(torch_mod, dim, dofs, n):
-    """One, two and four coordinate banks (D <= 16, <= 32, <= 64)."""
+                                         (33, 24, 110), (59, 64, 100), (64, 40, 100), (12, 160, 90)])
+def test_adjoint_matches_oracle(torch_mod, dim, dofs, n, prec, form):
+    """One, two and four coordinate banks (D <= 16, <= 32, <= 64); one to
+    five dof tiles (two launches of the wave-specialized kernel)."""
     ii, jj, S, X, G = grid_case(dim, 2, 5, 0.4, 3 + dim, 2500, dofs, n, 5)
-    ctx = hddm_lb.Context(0)
+    ctx = adj_ctx(form)
     ctx.upload_state(0, ii, jj, S)
-    say("adjoint dim", dim, "nno", ii.shape[0], "dofs", dofs, "n", n)
-    dx, ds = run_adjoint(torch_mod, ctx, X, G)
+    say("adjoint dim", dim, "nno", ii.shape[0], "dofs", dofs, "n", n, "precision", prec, "formulation", form)
+    dx, ds = run_adjoint(torch_mod, ctx, X, G, prec=prec)
     dxo, dso = oracle.adjoint(ii, jj, S, X, G)
     ds = ds.reshape(ii.shape[0], dofs)
     say("  dx err", normwise_rel_err(dx, dxo), "ds err", normwise_rel_err(ds, dso))
     assert np.isfinite(dx).all() and np.isfinite(ds).all()
-    assert normwise_rel_err(dx, dxo) <= 1e-10
-    assert normwise_rel_err(ds, dso) <= 1e-12
+    assert normwise_rel_err(dx, dxo) <= TOL_ADJ[prec][0]
+    assert normwise_rel_err(ds, dso) <= TOL_ADJ[prec][1]
     ctx.close()
+
+
+def test_adjoint_kernels_agree(torch_mod):
+    """The wave-specialized FP64 adjoint and the per-tile FP64 kernels give
+    the same dx and dS to rounding (four dof tiles, node segments on)."""
+    ii, jj, S, X, G = grid_case(24, 2, 5, 0.4, 66, 12000, 128, 700, 5)
+    out = []
+    for form in (1, 2):
+        ctx = adj_ctx(form)
+        ctx.set_option(hddm_lb.OPT_SEGMENT_NODES, 3000)
+        ctx.upload_state(0, ii, jj, S)
+        out.append(run_adjoint(torch_mod, ctx, X, G))
+        ctx.close()
+    assert normwise_rel_err(out[1][0], out[0][0]) <= 1e-13
+    assert normwise_rel_err(out[1][1].reshape(-1, 128), out[0][1].reshape(-1, 128)) <= 1e-13
 
 
 @pytest.mark.parametrize("which", ["dx", "ds"])
@@ -238,7 +275,8 @@ def test_adjoint_dx_bitwise_repeatable(torch_mod):
     ctx.close()
 
 
-def test_adjoint_d59_four_states(torch_mod):
+@pytest.mark.parametrize("prec", [hddm_lb.FP64, hddm_lb.F32])
+def test_adjoint_d59_four_states(torch_mod, prec):
     """High-dimensional multistate shape: 4 states, D=59, regular grids
     sum(l-1) <= 3 (281,077 nodes each), dofs=128, 4,096 points in
     [0.01, 0.99]^59 kept 1e-3 off every kink, gbar ~ N(0, 1). dx against the
@@ -257,15 +295,17 @@ def test_adjoint_d59_four_states(torch_mod):
     X = hddm_lb.synthetic_points(n, 59, 92, 0.01, 0.99, kink_level=4, kink_gap=1e-3)
     sid = np.random.default_rng(93).integers(0, 4, size=n).astype(np.int32)
     G = np.random.default_rng(94).standard_normal((n, 128))
-    dx, _ = run_adjoint(torch, ctx, X, G, sid)
+    dx, _ = run_adjoint(torch, ctx, X, G, sid, prec)
     assert np.isfinite(dx).all()
+    tdx, tds = TOL_ADJ[prec]
+    fd_tol = 1e-7 if prec == hddm_lb.FP64 else 1e-5
     rng = np.random.default_rng(95)
     for s, (ii, jj, S) in enumerate(states):
         pick = rng.choice(np.flatnonzero(sid == s), size=8, replace=False)
         dxo, _ = oracle.adjoint(ii, jj, S, X[pick], G[pick])
         err = normwise_rel_err(dx[pick], dxo)
         say("  state", s, "dx err (8 points)", err)
-        assert err <= 1e-10
+        assert err <= tdx
         if s == 0:
             h = 1e-4
             for p in pick[:2]:
@@ -274,9 +314,9 @@ def test_adjoint_d59_four_states(torch_mod):
                     xp[0, d] += h
                     xm[0, d] -= h
                     fd = ((oracle.forward(ii, jj, S, xp) - oracle.forward(ii, jj, S, xm)) * G[p]).sum() / (2 * h)
-                    assert abs(dx[p, d] - fd) <= 1e-7 * max(1.0, abs(fd))
+                    assert abs(dx[p, d] - fd) <= fd_tol * max(1.0, abs(fd))
     m = 256
-    _, ds = run_adjoint(torch, ctx, X[:m], G[:m], sid[:m])
+    _, ds = run_adjoint(torch, ctx, X[:m], G[:m], sid[:m], prec)
     for s, (ii, jj, S) in enumerate(states):
         sel = sid[:m] == s
         _, dso = oracle.adjoint(ii, jj, S, X[:m][sel], G[:m][sel], nthreads=8)
@@ -284,5 +324,5 @@ def test_adjoint_d59_four_states(torch_mod):
         dss = ds[off:off + ii.shape[0] * 128].reshape(ii.shape[0], 128)
         err = normwise_rel_err(dss, dso)
         say("  state", s, "ds err (%d points)" % sel.sum(), err)
-        assert err <= 1e-12
+        assert err <= tds
     ctx.close()
