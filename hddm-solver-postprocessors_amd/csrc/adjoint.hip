@@ -597,12 +597,20 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 			ctx->dim, st));
 		HIP_TRY(launch_scatter(npoints, sc.bkey, sc.cursor, sc.perm, st));
 		const int npass = (ctx->dt_needed + 3) / 4;
+		auto& ring = ctx->tring[1];
+		const int tslot = (int)(ring.calls % hddm_lb_ctx::kTimingRing);
+		if (ctx->opt_timing) HIP_TRY(hipEventRecord(ring.e0[tslot], st));
 		for (int ps = 0; ps < npass; ps++)
 		{
 			a.dt_base = 4 * ps;
 			if (ps > 0) HIP_TRY(launch_reset_counter(sc.hdr, st));
 			void* params[] = { (void*)&a };
 			HIP_TRY(hipLaunchKernel(kern, dim3(per_cu * ctx->num_cus), dim3(5 * kWave), params, lds, st));
+		}
+		if (ctx->opt_timing)
+		{
+			HIP_TRY(hipEventRecord(ring.e1[tslot], st));
+			ring.calls++;
 		}
 		return adjoint_tail(ctx, sc, dx, lddx, dsurplus, ds_total, xcd_copies, ns, st);
 	}

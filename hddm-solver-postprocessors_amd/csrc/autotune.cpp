@@ -1,8 +1,15 @@
-// Autotuner of the sweep's launch shape, keyed by (precision, D, total nodes,
-// max dofs, log2 of the point count). On the first forward of a key it times
-// every candidate (op source x waves per workgroup x dof tiles per pass) on the caller's
-// data with hipEvents and keeps the fastest. Tuning runs write y like the
-// forward itself, so the caller's output is valid afterwards.
+// Autotuner of the sweep's formulation and launch shape, keyed by (precision,
+// D, total nodes, max dofs, log2 of the point count). On the first forward of
+// a key it times every candidate on the caller's data with hipEvents and
+// keeps the fastest:
+//   * the sweep where every wave walks and contracts its own block, with 2
+//     or 4 waves per workgroup and 1 or 2 dof tiles per pass (the split
+//     paths; FP64 always takes one);
+//   * the wave-specialized sweep (a walker wave feeding MFMA consumer waves).
+// Candidates are timed on the first min(npoints, kTunePoints) points, which
+// is enough work to fill the device and keeps tuning to a few forwards of a
+// small batch. Those rows of y are written like any forward's and are
+// overwritten by the forward that follows.
 #include "context.h"
 #include "kernels.h"
 #include "errors.h"
@@ -11,6 +18,8 @@
 #include <vector>
 
 namespace hddm_lb {
+
+constexpr int64_t kTunePoints = 65536;
 
 int autotune(hddm_lb_ctx* ctx, int64_t npoints, const double* x, int64_t ldx, const int32_t* sid,
 	double* y, int64_t ldy, int prec, hipStream_t st, TuneConfig& cfg)
@@ -23,13 +32,11 @@ int autotune(hddm_lb_ctx* ctx, int64_t npoints, const double* x, int64_t ldx, co
 	while (((int64_t)1 << (lg + 1)) <= npoints) lg++;
 	auto key = std::make_tuple(prec, ctx->dim, nodes, maxdofs, lg);
 	auto hit = ctx->tuned.find(key);
-	if (hit != ctx->tuned.end()) { cfg = hit->second; return HDDM_LB_SUCCESS; }
+	if (hit != ctx->tuned.end()) { cfg = hit->second; ctx->tune_hits++; return HDDM_LB_SUCCESS; }
 
 	std::vector<TuneConfig> cands;
-	for (int walk : { kWalkSmem, kWalkLds })
-	{
-		if (ctx->opt_walk && walk != (ctx->opt_walk == 2 ? kWalkLds : kWalkSmem)) continue;
-		for (int waves : { 1, 2, 4 })
+	if (ctx->opt_form != 2)
+		for (int waves : { 2, 4 })
 		{
 			if (ctx->opt_waves && waves != ctx->opt_waves) continue;
 			for (int slab : { 1, 2 })
@@ -37,15 +44,23 @@ int autotune(hddm_lb_ctx* ctx, int64_t npoints, const double* x, int64_t ldx, co
 				if (slab > ctx->dt_needed || (prec == HDDM_LB_FP64 && slab > 1)) continue;
 				if (ctx->opt_slab && slab != ctx->opt_slab) continue;
 				TuneConfig c = cfg;
-				c.walk = walk;
+				c.ws = 0;
+				c.walk = kWalkSmem;
 				c.waves = waves;
 				c.slab = slab;
 				cands.push_back(c);
 			}
 		}
+	if (ctx->opt_form != 1)
+	{
+		TuneConfig c = cfg;
+		c.ws = 1;
+		c.nc = ctx->dt_needed >= 2 ? 4 : 2;
+		cands.push_back(c);
 	}
 	if (cands.empty()) { ctx->tuned[key] = cfg; return HDDM_LB_SUCCESS; }
 
+	const int64_t m = std::min(npoints, kTunePoints);
 	hipEvent_t e0, e1;
 	if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
 		return set_error(HDDM_LB_ERROR_HIP, "autotune: hipEventCreate failed");
@@ -54,9 +69,9 @@ int autotune(hddm_lb_ctx* ctx, int64_t npoints, const double* x, int64_t ldx, co
 	int rc = HDDM_LB_SUCCESS;
 	for (const TuneConfig& c : cands)
 	{
-		if ((rc = enqueue_forward(ctx, ctx->scratch, npoints, x, ldx, sid, y, ldy, prec, st, c, false))) break;
+		if ((rc = enqueue_forward(ctx, ctx->scratch, m, x, ldx, sid, y, ldy, prec, st, c, false))) break;
 		(void)hipEventRecord(e0, st);
-		if ((rc = enqueue_forward(ctx, ctx->scratch, npoints, x, ldx, sid, y, ldy, prec, st, c, false))) break;
+		if ((rc = enqueue_forward(ctx, ctx->scratch, m, x, ldx, sid, y, ldy, prec, st, c, false))) break;
 		(void)hipEventRecord(e1, st);
 		if (hipEventSynchronize(e1) != hipSuccess) { rc = set_error(HDDM_LB_ERROR_HIP, "autotune: event sync failed"); break; }
 		float ms = 0.0f;
@@ -67,6 +82,7 @@ int autotune(hddm_lb_ctx* ctx, int64_t npoints, const double* x, int64_t ldx, co
 	(void)hipEventDestroy(e1);
 	if (rc) return rc;
 	ctx->tuned[key] = bestc;
+	ctx->tune_runs++;
 	cfg = bestc;
 	return HDDM_LB_SUCCESS;
 }

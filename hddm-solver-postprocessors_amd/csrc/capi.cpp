@@ -97,8 +97,11 @@ hddm_lb_ctx::~hddm_lb_ctx()
 		(void*)d_xstage, (void*)d_ystage, (void*)d_sidstage, (void*)d_wbuf, (void*)d_stats, (void*)d_status })
 		dev_free(p);
 	if (scratch_event) (void)hipEventDestroy(scratch_event);
-	for (hipEvent_t e : tev0) (void)hipEventDestroy(e);
-	for (hipEvent_t e : tev1) (void)hipEventDestroy(e);
+	for (auto& r : tring)
+	{
+		for (hipEvent_t e : r.e0) (void)hipEventDestroy(e);
+		for (hipEvent_t e : r.e1) (void)hipEventDestroy(e);
+	}
 	if (own_stream) (void)hipStreamDestroy(own_stream);
 }
 
@@ -148,14 +151,15 @@ extern "C" int hddm_lb_create(int device, hddm_lb_ctx** out)
 		if (rc) return rc;
 		HIP_TRY(hipMemset(ctx->d_status, 0, sizeof(uint32_t)));
 	}
-	for (int i = 0; i < hddm_lb_ctx::kTimingRing; i++)
-	{
-		hipEvent_t e0 = nullptr, e1 = nullptr;
-		HIP_TRY(hipEventCreate(&e0));
-		ctx->tev0.push_back(e0);
-		HIP_TRY(hipEventCreate(&e1));
-		ctx->tev1.push_back(e1);
-	}
+	for (auto& r : ctx->tring)
+		for (int i = 0; i < hddm_lb_ctx::kTimingRing; i++)
+		{
+			hipEvent_t e0 = nullptr, e1 = nullptr;
+			HIP_TRY(hipEventCreate(&e0));
+			r.e0.push_back(e0);
+			HIP_TRY(hipEventCreate(&e1));
+			r.e1.push_back(e1);
+		}
 	*out = ctx.release();
 	return HDDM_LB_SUCCESS;
 }
@@ -515,8 +519,9 @@ int enqueue_forward(hddm_lb_ctx* ctx, Scratch& sc, int64_t npoints, const double
 		a.stats = ctx->d_stats;
 	}
 	const int nslab = (ctx->dt_needed + tiles - 1) / tiles;
-	const int slot = (int)(ctx->timed_calls % hddm_lb_ctx::kTimingRing);
-	if (timed) HIP_TRY(hipEventRecord(ctx->tev0[slot], st));
+	auto& ring = ctx->tring[0];
+	const int slot = (int)(ring.calls % hddm_lb_ctx::kTimingRing);
+	if (timed) HIP_TRY(hipEventRecord(ring.e0[slot], st));
 	for (int sl = 0; sl < nslab; sl++)
 	{
 		a.dt_base = sl * tiles;
@@ -525,8 +530,8 @@ int enqueue_forward(hddm_lb_ctx* ctx, Scratch& sc, int64_t npoints, const double
 	}
 	if (timed)
 	{
-		HIP_TRY(hipEventRecord(ctx->tev1[slot], st));
-		ctx->timed_calls++;
+		HIP_TRY(hipEventRecord(ring.e1[slot], st));
+		ring.calls++;
 	}
 	if (ctx->max_nseg_dp > 0)
 		HIP_TRY(launch_reduce(ctx->d_states, sc.plans, sc.perm, sc.partial, y, ldy, ns, st));
@@ -567,6 +572,7 @@ extern "C" int hddm_lb_forward(hddm_lb_ctx* ctx, int64_t npoints,
 	if (ctx->opt_autotune && (rc = autotune(ctx, npoints, x, ldx, state_ids, y, ldy, precision, st, cfg)))
 		return rc;
 	ctx->last_timed = ctx->opt_timing;
+	ctx->last_cfg = cfg;
 	rc = enqueue_forward(ctx, ctx->scratch, npoints, x, ldx, state_ids, y, ldy, precision, st, cfg, ctx->opt_timing);
 	if (rc) return rc;
 	return scratch_mark(ctx, st);
@@ -584,6 +590,18 @@ extern "C" int hddm_lb_last_stats_n(hddm_lb_ctx* ctx, uint64_t* stats, int n)
 	return HDDM_LB_SUCCESS;
 }
 
+extern "C" int hddm_lb_last_config(hddm_lb_ctx* ctx, int* config)
+{
+	if (!ctx || !config) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "last_config: NULL argument");
+	const TuneConfig& c = ctx->last_cfg;
+	config[0] = c.ws ? 2 : 1;
+	config[1] = c.ws ? c.nc + 1 : c.waves;
+	config[2] = c.ws ? (c.nc == 4 ? 4 : 1) : c.slab;
+	config[3] = (int)ctx->tune_runs;
+	config[4] = (int)ctx->tune_hits;
+	return HDDM_LB_SUCCESS;
+}
+
 extern "C" int hddm_lb_last_stats(hddm_lb_ctx* ctx, uint64_t* stats)
 {
 	return hddm_lb_last_stats_n(ctx, stats, 4);
@@ -591,38 +609,47 @@ extern "C" int hddm_lb_last_stats(hddm_lb_ctx* ctx, uint64_t* stats)
 
 extern "C" double hddm_lb_last_sweep_ms(hddm_lb_ctx* ctx)
 {
-	if (!ctx || !ctx->last_timed || ctx->timed_calls == 0) return -1.0;
+	if (!ctx || !ctx->last_timed || ctx->tring[0].calls == 0) return -1.0;
+	auto& ring = ctx->tring[0];
 	DeviceGuard g(ctx->device);
-	const int slot = (int)((ctx->timed_calls - 1) % hddm_lb_ctx::kTimingRing);
-	if (hipEventSynchronize(ctx->tev1[slot]) != hipSuccess) { (void)hipGetLastError(); return -1.0; }
+	const int slot = (int)((ring.calls - 1) % hddm_lb_ctx::kTimingRing);
+	if (hipEventSynchronize(ring.e1[slot]) != hipSuccess) { (void)hipGetLastError(); return -1.0; }
 	float ms = -1.0f;
-	if (hipEventElapsedTime(&ms, ctx->tev0[slot], ctx->tev1[slot]) != hipSuccess) { (void)hipGetLastError(); return -1.0; }
+	if (hipEventElapsedTime(&ms, ring.e0[slot], ring.e1[slot]) != hipSuccess) { (void)hipGetLastError(); return -1.0; }
 	return (double)ms;
+}
+
+extern "C" int hddm_lb_kernel_time_summary(hddm_lb_ctx* ctx, int kind, int64_t* ncalls, double* total_ms,
+	double* min_ms, double* max_ms)
+{
+	if (!ctx || !ncalls || !total_ms) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "kernel_time_summary: NULL argument");
+	if (kind < 0 || kind > 1) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "kernel_time_summary: kind %d is not 0 or 1", kind);
+	DeviceGuard g(ctx->device);
+	auto& ring = ctx->tring[kind];
+	const int64_t first = std::max(ring.read, ring.calls - hddm_lb_ctx::kTimingRing);
+	double tot = 0.0, mn = 0.0, mx = 0.0;
+	for (int64_t c = first; c < ring.calls; c++)
+	{
+		const int slot = (int)(c % hddm_lb_ctx::kTimingRing);
+		HIP_TRY(hipEventSynchronize(ring.e1[slot]));
+		float ms = 0.0f;
+		HIP_TRY(hipEventElapsedTime(&ms, ring.e0[slot], ring.e1[slot]));
+		tot += ms;
+		mn = c == first ? ms : std::min(mn, (double)ms);
+		mx = c == first ? ms : std::max(mx, (double)ms);
+	}
+	*ncalls = ring.calls - first;
+	*total_ms = tot;
+	if (min_ms) *min_ms = mn;
+	if (max_ms) *max_ms = mx;
+	ring.read = ring.calls;
+	return HDDM_LB_SUCCESS;
 }
 
 extern "C" int hddm_lb_sweep_time_summary(hddm_lb_ctx* ctx, int64_t* ncalls, double* total_ms,
 	double* min_ms, double* max_ms)
 {
-	if (!ctx || !ncalls || !total_ms) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "sweep_time_summary: NULL argument");
-	DeviceGuard g(ctx->device);
-	const int64_t first = std::max(ctx->timed_read, ctx->timed_calls - hddm_lb_ctx::kTimingRing);
-	double tot = 0.0, mn = 0.0, mx = 0.0;
-	for (int64_t c = first; c < ctx->timed_calls; c++)
-	{
-		const int slot = (int)(c % hddm_lb_ctx::kTimingRing);
-		HIP_TRY(hipEventSynchronize(ctx->tev1[slot]));
-		float ms = 0.0f;
-		HIP_TRY(hipEventElapsedTime(&ms, ctx->tev0[slot], ctx->tev1[slot]));
-		tot += ms;
-		mn = c == first ? ms : std::min(mn, (double)ms);
-		mx = c == first ? ms : std::max(mx, (double)ms);
-	}
-	*ncalls = ctx->timed_calls - first;
-	*total_ms = tot;
-	if (min_ms) *min_ms = mn;
-	if (max_ms) *max_ms = mx;
-	ctx->timed_read = ctx->timed_calls;
-	return HDDM_LB_SUCCESS;
+	return hddm_lb_kernel_time_summary(ctx, 0, ncalls, total_ms, min_ms, max_ms);
 }
 
 extern "C" int hddm_lb_forward_host(hddm_lb_ctx* ctx, int64_t npoints,

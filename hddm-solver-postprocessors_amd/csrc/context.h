@@ -117,15 +117,22 @@ struct hddm_lb_ctx
 	static constexpr int kStatCount = 6;
 	unsigned long long* d_stats = nullptr;
 	uint32_t* d_status = nullptr;   // HDDM_LB_STATUS_* bits raised by kernels
-	// ring of event pairs bracketing the sweep kernels of timed forwards
+	// rings of event pairs bracketing the kernels of timed calls: [0] the
+	// sweep kernels of forwards, [1] the adjoint kernels
 	static constexpr int kTimingRing = 256;
-	std::vector<hipEvent_t> tev0, tev1;
-	int64_t timed_calls = 0;      // pairs recorded since create
-	int64_t timed_read = 0;       // pairs already summarised
+	struct TimingRing
+	{
+		std::vector<hipEvent_t> e0, e1;
+		int64_t calls = 0;   // pairs recorded since create
+		int64_t read = 0;    // pairs already summarised
+	};
+	TimingRing tring[2];
 	bool last_timed = false;
 
 	// autotuner cache: (prec, dim, total nodes, max dofs, log2 npoints) -> config
 	std::map<std::tuple<int, int, int64_t, int, int>, hddm_lb::TuneConfig> tuned;
+	int64_t tune_runs = 0, tune_hits = 0;   // keys tuned, forwards served from the cache
+	hddm_lb::TuneConfig last_cfg;           // configuration of the last eager forward
 
 	~hddm_lb_ctx();
 };

@@ -84,8 +84,10 @@ _SIGS = {
     "hddm_lb_plan_destroy": (_i32, [_vp]),
     "hddm_lb_last_sweep_ms": (_dbl, [_vp]),
     "hddm_lb_last_stats": (_i32, [_vp, _vp]),
+    "hddm_lb_last_config": (_i32, [_vp, _vp]),
     "hddm_lb_last_stats_n": (_i32, [_vp, _vp, _i32]),
     "hddm_lb_sweep_time_summary": (_i32, [_vp, _vp, _vp, _vp, _vp]),
+    "hddm_lb_kernel_time_summary": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp]),
     "hddm_lb_plt_read": (_i32, [_c.c_char_p, _i32, _c.POINTER(_i32), _c.POINTER(_i32), _c.POINTER(_i32),
                                 _c.POINTER(_i32), _c.POINTER(_I32P), _c.POINTER(_I32P), _c.POINTER(_DP)]),
     "hddm_lb_plt_write_compressed": (_i32, [_c.c_char_p, _i32, _i32, _i32, _i32, _vp, _vp, _vp]),
@@ -329,12 +331,20 @@ class Context:
     def last_sweep_ms(self):
         return lib.hddm_lb_last_sweep_ms(self._h)
 
-    def sweep_time_summary(self):
-        """(calls, total ms, min ms, max ms) of the sweep kernels of timed forwards since the last call."""
+    def sweep_time_summary(self, kind=0):
+        """(calls, total ms, min ms, max ms) of the kernels of timed calls since the last call of
+        this kind: 0 the forwards' sweep kernels, 1 the wave-specialized adjoint kernels."""
         n = ctypes.c_int64(0)
         tot, mn, mx = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_double(0)
-        _check(lib.hddm_lb_sweep_time_summary(self._h, ctypes.byref(n), ctypes.byref(tot), ctypes.byref(mn), ctypes.byref(mx)))
+        _check(lib.hddm_lb_kernel_time_summary(self._h, kind, ctypes.byref(n), ctypes.byref(tot), ctypes.byref(mn),
+                                               ctypes.byref(mx)))
         return n.value, tot.value, mn.value, mx.value
+
+    def last_config(self):
+        """Sweep configuration of the last forward: dict(formulation, waves, tiles, tuned, cache_hits)."""
+        v = np.zeros(5, np.int32)
+        _check(lib.hddm_lb_last_config(self._h, _ptr(v)))
+        return dict(zip(("formulation", "waves", "tiles", "tuned", "cache_hits"), (int(a) for a in v)))
 
     def device_status(self, clear=True):
         """Status bits raised by this context's kernels since the last clearing call (0: none)."""

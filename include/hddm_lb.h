@@ -54,7 +54,8 @@ typedef struct hddm_lb_grid hddm_lb_grid;
 #define HDDM_LB_F32     3  /* MFMA f32: basis f32 x surplus f32 */
 
 /* options for hddm_lb_set_option */
-#define HDDM_LB_OPT_AUTOTUNE      1  /* 1: time tile candidates on first use of a (D, nno, dofs) key */
+#define HDDM_LB_OPT_AUTOTUNE      1  /* 1: on first use of a (precision, D, nodes, dofs, log2 points) key, time
+                                        the sweep formulations and launch shapes and keep the fastest */
 #define HDDM_LB_OPT_SEGMENT_NODES 2  /* node-range length of one work item (0: automatic, eighths of states
                                         of >= 16k nodes; a value >= nno keeps states whole) */
 #define HDDM_LB_OPT_WAVES_PER_WG  3  /* waves per workgroup (1, 2, 4; 0: automatic) */
@@ -157,6 +158,10 @@ double hddm_lb_last_sweep_ms(hddm_lb_ctx* ctx);
  * NULL). Waits for their events. */
 int hddm_lb_sweep_time_summary(hddm_lb_ctx* ctx, int64_t* ncalls, double* total_ms,
 	double* min_ms, double* max_ms);
+/* The same for one kind of kernel: 0 the sweep kernels of forwards (as
+ * above), 1 the wave-specialized adjoint kernels of hddm_lb_adjoint. */
+int hddm_lb_kernel_time_summary(hddm_lb_ctx* ctx, int kind, int64_t* ncalls, double* total_ms,
+	double* min_ms, double* max_ms);
 /* With HDDM_LB_OPT_STATS set: counts of the last forward, summed over all
  * work items: stats[0] ops evaluated, [1] subtrees pruned, [2] chunks
  * contracted (8 nodes on the FP64 path, 32 on F32, 16 on F16X2 / BF16X3),
@@ -169,6 +174,12 @@ int hddm_lb_last_stats(hddm_lb_ctx* ctx, uint64_t* stats);
  * stats[5] is the number of (point, node) pairs with a non-zero basis value
  * that the contraction received (wave-specialized sweep only, else 0). */
 int hddm_lb_last_stats_n(hddm_lb_ctx* ctx, uint64_t* stats, int n);
+
+/* Sweep configuration of the last hddm_lb_forward: config[0] formulation
+ * (1 each wave walks and contracts, 2 wave-specialized), [1] waves per
+ * workgroup, [2] dof tiles of 32 per launch, [3] keys the autotuner has
+ * tuned on this context, [4] forwards it served from its cache. */
+int hddm_lb_last_config(hddm_lb_ctx* ctx, int* config);
 
 /* ---- host-side node layout, for inspection without a device ---- */
 typedef struct hddm_lb_layout hddm_lb_layout;
