@@ -1,22 +1,32 @@
 #!/usr/bin/env python3
 """Headline benchmark of the LinearBasis interpolation engine on one MI355X.
 
-Workload: single-state forward, adaptive sparse grid D=16 (seeded refinement
-p=0.3 from level sum <= 2, max level 7, stopped at 60,018 nodes), dofs=32,
-1,048,576 points x ~ U[0,1)^16, all resident in HBM before timing.
+Headline workload: high-dimensional multistate forward + adjoint.
+  4 states, D=59, regular sparse grids sum(l-1) <= 3 (281,077 nodes each),
+  dofs=128, 1,048,576 points x ~ U[0,1)^59 with seeded state ids, upstream
+  gradient N(0, 1); everything resident in HBM before timing.
+A step is one forward (y) and one adjoint (dx and dsurplus) over the whole
+batch. The mixed step runs the forward on f16 MFMA with split hi/lo operands
+(HDDM_LB_F16X2) and the adjoint on FP32 MFMA (HDDM_LB_F32); the plain FP64
+step runs both in FP64 with the same work queue, trie walk and zero-skip.
+value = points x dofs / mixed step time (point.dof evaluations per second,
+each with its gradient). The sweep formulation and launch shape of every
+forward are the autotuner's choice, reported in config.
 
-A step is one forward over the whole batch. The headline path is the
-mixed-precision MFMA sweep (basis formed in FP64, contraction on
-v_mfma_f32_32x32x2f32 with FP32 accumulation folded into FP64 after every
-32-node chunk); the plain FP64 GPU sweep (same walk, FP64 contraction) is
-timed beside it. Prints one JSON line.
+Secondary lines (shorter): single-state forward D=16 (60k nodes, 32 dofs,
+2^20 points), 12-state multistate forward D=20, and forward + adjoint on the
+differentiated workload D=20 (100k nodes, 32 dofs, 65,536 points), each
+mixed against FP64 with its error against the oracle; the per-point legacy
+API through the drop-in plugin; the reference's CPU algorithm on the host.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--precision f32]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 5]
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -24,97 +34,235 @@ sys.path.insert(0, os.path.join(ROOT, "hddm-solver-postprocessors_amd"))
 
 METRIC = "point·dof evals/sec on 1 MI355X, mixed/MFMA vs FP64 plain at ≤1e-6 rel err"
 PEAK_TFLOPS = {"f16x2": 2500.0, "bf16x3": 2500.0, "f32": 157.3, "fp64": 78.6}
-# FLOP of one MFMA instruction of each path's contraction
-MFMA_FLOPS = {"f32": 2 * 32 * 32 * 2, "f16x2": 2 * 32 * 32 * 16, "bf16x3": 2 * 32 * 32 * 16}
-DTYPE = {"f16x2": "f16", "bf16x3": "bf16", "f32": "f32", "fp64": "f64"}
+# published per-state time of the reference's CUDA backend on the per-point API
+# (LinearBasis/tests/InterpolateArrayManyMultistate/medium/daint.log:152)
+REF_CUDA_US_PER_STATE = 122.467
 
 
-def build_workload(npoints):
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def normwise_err(y, yref):
+    """max over dof columns of max |y - yref| / max |yref| (columns of zeros: absolute)."""
+    import numpy as np
+    y = np.asarray(y, np.float64)
+    yref = np.asarray(yref, np.float64)
+    den = np.abs(yref).max(axis=0)
+    num = np.abs(y - yref).max(axis=0)
+    den = np.where(den > 0, den, 1.0)
+    return float((num / den).max()) if y.size else 0.0
+
+
+def oracle_mod():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    return oracle
+
+
+# ------------------------------------------------------------------ workloads
+
+def d59_states(ctx):
     import hddm_lb
-    ii, jj, lv, _ = hddm_lb.generate_grid(16, 2, 7, 0.3, 1, 60000)
-    S = hddm_lb.synthetic_surplus(lv, 32, 2)
-    X = hddm_lb.synthetic_points(npoints, 16, 3)
-    return ii, jj, S, X
+    states = []
+    for s in range(4):
+        ii, jj, lv, _ = hddm_lb.generate_grid(59, 3, 4, 0.0, 500 + s, 0)
+        S = hddm_lb.synthetic_surplus(lv, 128, 600 + s)
+        ctx.upload_state(s, ii, jj, S)
+        states.append((ii, jj, S))
+    return states
 
 
-def time_path(ctx, x, y, prec, steps, warmup):
-    """Wall time per step over `steps` forwards, and the sweep kernels' mean
-    duration from hipEvents recorded on the same stream in the same steps."""
+def timed_steps(step, steps, warmup, ctx=None):
+    """Wall ms per step over `steps` calls after `warmup`; with ctx, the mean
+    sweep (kind 0) and adjoint (kind 1) kernel ms per step from the library's
+    HIP events on the calls' stream, recorded in the same steps."""
     import torch
     import hddm_lb
     for _ in range(warmup):
-        ctx.forward(x, y, None, prec)
+        step()
     torch.cuda.synchronize()
-    ctx.set_option(hddm_lb.OPT_TIMING, 1)
-    ctx.sweep_time_summary()  # drop the warmup events
+    if ctx is not None:
+        ctx.set_option(hddm_lb.OPT_TIMING, 1)
+        ctx.sweep_time_summary(0)
+        ctx.sweep_time_summary(1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        ctx.forward(x, y, None, prec)
+        step()
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    n, tot, mn, mx = ctx.sweep_time_summary()
-    ctx.set_option(hddm_lb.OPT_TIMING, 0)
-    return (t1 - t0) * 1e3 / steps, tot / max(n, 1), mn, mx
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    kern = None
+    if ctx is not None:
+        kern = []
+        for kind in (0, 1):
+            n, tot, mn, mx = ctx.sweep_time_summary(kind)
+            kern.append({"launches": n, "ms_per_step": tot / steps, "ms_per_launch": tot / max(n, 1),
+                         "min_ms": mn, "max_ms": mx})
+        ctx.set_option(hddm_lb.OPT_TIMING, 0)
+    return ms, kern
 
 
-def time_multistate(args, prec, which):
-    """Multistate forwards in one launch, timed like the headline (events
-    around whole calls, inputs resident):
-    d20x12: 12 states, D=20, adaptive grids (max level 6) with node counts
-            log-uniform in [2k, 200k], dofs per state from {16, 32, 64},
-            262,144 points with seeded state ids;
-    d59x4:  4 states, D=59, regular grids sum(l-1) <= 3 (281,077 nodes each),
-            dofs=128, 2^20 points with seeded state ids.
-    The unit counts every output element: sum over points of the dofs of the
-    point's state."""
+def headline(args):
+    """4 states x D=59 forward + adjoint: mixed vs FP64."""
     import numpy as np
     import torch
     import hddm_lb
     ctx = hddm_lb.Context(0)
-    rng = np.random.default_rng(2024)
-    if which == "d20x12":
-        nnos = np.exp(rng.uniform(np.log(2000), np.log(200000), size=12)).astype(int)
-        dofs = rng.choice([16, 32, 64], size=12)
-        specs = [(20, 1, 6, 0.3, 300 + s, int(nnos[s]), int(dofs[s])) for s in range(12)]
-        n, dim, steps = 262144, 20, max(2, args.steps // 2)
-    else:
-        specs = [(59, 3, 4, 0.0, 500 + s, 0, 128) for s in range(4)]
-        n, dim, steps = 1 << 20, 59, 2
-    nno_tot, dofs_of = 0, []
-    for s, (d, start, maxl, p, seed, target, nd) in enumerate(specs):
-        ii, jj, lv, _ = hddm_lb.generate_grid(d, start, maxl, p, seed, target)
-        S = hddm_lb.synthetic_surplus(lv, nd, seed + 100)
-        ctx.upload_state(s, ii, jj, S)
-        nno_tot += ii.shape[0]
-        dofs_of.append(nd)
-    X = hddm_lb.synthetic_points(n, dim, 77)
-    w = rng.uniform(0.5, 1.5, size=len(specs))
-    sid = rng.choice(len(specs), size=n, p=w / w.sum()).astype(np.int32)
-    work = float(np.asarray(dofs_of)[sid].sum())
+    ctx.set_option(hddm_lb.OPT_AUTOTUNE, 1)
+    t0 = time.perf_counter()
+    states = d59_states(ctx)
+    say("bench: D=59 states uploaded in %.1f s" % (time.perf_counter() - t0))
+    n = args.points
+    X = hddm_lb.synthetic_points(n, 59, 77)
+    sid = np.random.default_rng(2024).integers(0, 4, size=n).astype(np.int32)
+    G = np.random.default_rng(94).standard_normal((n, 128))
     x = torch.from_numpy(X).cuda()
     sd = torch.from_numpy(sid).cuda()
-    y = torch.empty((n, max(dofs_of)), dtype=torch.float64, device="cuda")
-    res = {"workload": "%d states, D=%d, %d nodes in all, %d points" % (len(specs), dim, nno_tot, n)}
-    for name, pr in (("mixed", prec), ("fp64", hddm_lb.FP64)):
-        ctx.forward(x, y, sd, pr)
+    g = torch.from_numpy(G).cuda()
+    y = torch.empty((n, 128), dtype=torch.float64, device="cuda")
+    dx = torch.empty((n, 59), dtype=torch.float64, device="cuda")
+    _, total = ctx.surplus_offset(-1)
+    ds = torch.empty((total,), dtype=torch.float64, device="cuda")
+    legs = {"mixed": (hddm_lb.F16X2, hddm_lb.F32), "fp64": (hddm_lb.FP64, hddm_lb.FP64)}
+    res = {}
+    for name, (pf, pa) in legs.items():
+        def step(pf=pf, pa=pa):
+            ctx.forward(x, y, sd, pf)
+            ctx.adjoint(x, g, dx, ds, sd, pa)
+        steps = args.steps if name == "mixed" else max(2, args.steps // 5)
+        warm = args.warmup if name == "mixed" else 1
+        ms, kern = timed_steps(step, steps, warm, ctx)
+        cfg = ctx.last_config()
+        status = ctx.device_status()
+        res[name] = {"ms_per_step": ms, "steps": steps, "warmup": warm, "kernels": kern, "forward_config": cfg,
+                     "status": status}
+        say("bench: D=59 %s step %.1f ms (forward sweep %.1f, adjoint %.1f), config %s"
+            % (name, ms, kern[0]["ms_per_step"], kern[1]["ms_per_step"], cfg))
+        assert status == 0, "device status %d after the %s leg" % (status, name)
+        # errors against the oracle on a seeded subset: y and dx at 4 points per state
+        Y = y.cpu().numpy()
+        DX = dx.cpu().numpy()
+        oracle = oracle_mod()
+        rng = np.random.default_rng(95)
+        ey, edx = 0.0, 0.0
+        for s, (ii, jj, S) in enumerate(states):
+            pick = rng.choice(np.flatnonzero(sid == s), size=4, replace=False)
+            ey = max(ey, normwise_err(Y[pick], oracle.forward(ii, jj, S, X[pick])))
+            dxo, _ = oracle.adjoint(ii, jj, S, X[pick], G[pick])
+            edx = max(edx, normwise_err(DX[pick], dxo))
+        res[name]["max_rel_err"] = {"y": ey, "dx": edx}
+    # dS error on a 128-point slice (the oracle's dS cost grows with the points)
+    m = 128
+    for name, (pf, pa) in legs.items():
+        ds2 = torch.empty_like(ds)
+        ctx.adjoint(x[:m], g[:m], None, ds2, sd[:m], pa)
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            ctx.forward(x, y, sd, pr)
-        torch.cuda.synchronize()
-        ms = (time.perf_counter() - t0) * 1e3 / steps
-        res[name] = {"ms_per_step": ms, "value": work / (ms * 1e-3)}
-    res["unit"] = "point·dof/s"
+        D = ds2.cpu().numpy()
+        oracle = oracle_mod()
+        e = 0.0
+        for s, (ii, jj, S) in enumerate(states):
+            sel = sid[:m] == s
+            _, dso = oracle.adjoint(ii, jj, S, X[:m][sel], G[:m][sel], nthreads=min(16, os.cpu_count() or 1))
+            off, _ = ctx.surplus_offset(s)
+            e = max(e, normwise_err(D[off:off + ii.shape[0] * 128].reshape(-1, 128), dso))
+        res[name]["max_rel_err"]["dsurplus"] = e
+    # executed work: nnz pairs and contraction chunks of one forward on each
+    # formulation's statistics pass (outside the timed steps)
+    ctx.set_option(hddm_lb.OPT_STATS, 1)
+    ctx.set_option(hddm_lb.OPT_FORMULATION, 2)
+    ctx.forward(x, y, sd, hddm_lb.F32)
+    st = ctx.last_stats(6)
+    ctx.set_option(hddm_lb.OPT_STATS, 0)
+    ctx.set_option(hddm_lb.OPT_FORMULATION, 0)
     ctx.close()
-    return res
+    return res, {"nnz": st[5], "walk_ops": st[0], "chunks32": st[2], "items": st[3], "nno": states[0][0].shape[0]}
 
 
-def time_adjoint(args, prec):
+def single_state_d16(args):
+    """Single-state forward: D=16, ~60k nodes (seeded refinement p=0.3 from
+    level sum <= 2, max level 7), dofs=32, 2^20 points."""
+    import numpy as np
+    import torch
+    import hddm_lb
+    ii, jj, lv, _ = hddm_lb.generate_grid(16, 2, 7, 0.3, 1, 60000)
+    S = hddm_lb.synthetic_surplus(lv, 32, 2)
+    n = 1 << 20
+    X = hddm_lb.synthetic_points(n, 16, 3)
+    ctx = hddm_lb.Context(0)
+    ctx.set_option(hddm_lb.OPT_AUTOTUNE, 1)
+    ctx.upload_state(0, ii, jj, S)
+    x = torch.from_numpy(X).cuda()
+    y = torch.empty((n, 32), dtype=torch.float64, device="cuda")
+    out = {"workload": "single-state forward, D=16, %d nodes, dofs=32, %d points" % (ii.shape[0], n)}
+    oracle = oracle_mod()
+    pick = np.random.default_rng(5).choice(n, size=256, replace=False)
+    yo = oracle.forward(ii, jj, S, X[pick])
+    for name, pr in (("mixed", hddm_lb.F32), ("fp64", hddm_lb.FP64)):
+        ms, kern = timed_steps(lambda: ctx.forward(x, y, None, pr), max(3, args.steps // 2), 2, ctx)
+        out[name] = {"ms_per_step": ms, "sweep_ms": kern[0]["ms_per_launch"], "value": n * 32 / (ms * 1e-3),
+                     "precision": "f32" if name == "mixed" else "fp64", "config": ctx.last_config(),
+                     "max_rel_err": normwise_err(y.cpu().numpy()[pick], yo)}
+    out["mixed_over_fp64"] = out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"]
+    # the same forward through host buffers: x in and y out over PCIe
+    yh = np.empty((n, 32))
+    ctx.forward_host(X, yh, None, hddm_lb.F32)
+    t0 = time.perf_counter()
+    ctx.forward_host(X, yh, None, hddm_lb.F32)
+    host_ms = (time.perf_counter() - t0) * 1e3
+    out["host_buffers_f32"] = {"ms_per_call": host_ms, "value": n * 32 / (host_ms * 1e-3),
+                               "note": "x and y in pageable host memory, copies included"}
+    ctx.close()
+    return out, (ii, jj, S, X)
+
+
+def multistate_d20(args):
+    """12 states, D=20, adaptive grids (max level 6) with node counts
+    log-uniform in [2k, 200k], dofs per state from {16, 32, 64}, 262,144
+    points with seeded state ids, one launch."""
+    import numpy as np
+    import torch
+    import hddm_lb
+    rng = np.random.default_rng(2024)
+    nnos = np.exp(rng.uniform(np.log(2000), np.log(200000), size=12)).astype(int)
+    dofs = rng.choice([16, 32, 64], size=12)
+    ctx = hddm_lb.Context(0)
+    ctx.set_option(hddm_lb.OPT_AUTOTUNE, 1)
+    states = []
+    for s in range(12):
+        ii, jj, lv, _ = hddm_lb.generate_grid(20, 1, 6, 0.3, 300 + s, int(nnos[s]))
+        S = hddm_lb.synthetic_surplus(lv, int(dofs[s]), 400 + s)
+        ctx.upload_state(s, ii, jj, S)
+        states.append((ii, jj, S))
+    n = 262144
+    X = hddm_lb.synthetic_points(n, 20, 77)
+    w = rng.uniform(0.5, 1.5, size=12)
+    sid = rng.choice(12, size=n, p=w / w.sum()).astype(np.int32)
+    work = float(dofs[sid].sum())
+    x = torch.from_numpy(X).cuda()
+    sd = torch.from_numpy(sid).cuda()
+    y = torch.empty((n, 64), dtype=torch.float64, device="cuda")
+    out = {"workload": "12 states, D=20, %d nodes in all, dofs 16/32/64, %d points"
+                       % (sum(s[0].shape[0] for s in states), n)}
+    oracle = oracle_mod()
+    for name, pr in (("mixed", hddm_lb.F32), ("fp64", hddm_lb.FP64)):
+        ms, _ = timed_steps(lambda: ctx.forward(x, y, sd, pr), max(3, args.steps // 2), 2)
+        Y = y.cpu().numpy()
+        e = 0.0
+        r2 = np.random.default_rng(6)
+        for s, (ii, jj, S) in enumerate(states):
+            pick = r2.choice(np.flatnonzero(sid == s), size=8, replace=False)
+            e = max(e, normwise_err(Y[pick, :S.shape[1]], oracle.forward(ii, jj, S, X[pick])))
+        out[name] = {"ms_per_step": ms, "value": work / (ms * 1e-3), "config": ctx.last_config(), "max_rel_err": e}
+    out["mixed_over_fp64"] = out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"]
+    ctx.close()
+    return out
+
+
+def forward_adjoint_d20(args):
     """Forward + adjoint on the differentiated workload: D=20 adaptive grid of
-    ~100k nodes (max level 6), dofs=32, 65,536 points in [0.01, 0.99]^20,
-    upstream gradient N(0, 1). A step is one forward (mixed path) and one FP64
-    adjoint (dx and dsurplus) over the batch."""
+    ~100k nodes (max level 6), dofs=32, 65,536 points in [0.01, 0.99]^20 kept
+    1e-3 off every kink, upstream gradient N(0, 1)."""
     import numpy as np
     import torch
     import hddm_lb
@@ -124,6 +272,7 @@ def time_adjoint(args, prec):
     X = hddm_lb.synthetic_points(n, 20, 406, 0.01, 0.99, kink_level=6, kink_gap=1e-3)
     G = np.random.default_rng(407).standard_normal((n, 32))
     ctx = hddm_lb.Context(0)
+    ctx.set_option(hddm_lb.OPT_AUTOTUNE, 1)
     ctx.upload_state(0, ii, jj, S)
     x = torch.from_numpy(X).cuda()
     g = torch.from_numpy(G).cuda()
@@ -131,66 +280,155 @@ def time_adjoint(args, prec):
     dx = torch.empty((n, 20), dtype=torch.float64, device="cuda")
     _, total = ctx.surplus_offset(-1)
     ds = torch.empty((total,), dtype=torch.float64, device="cuda")
-
-    def step():
-        ctx.forward(x, y, None, prec)
-        ctx.adjoint(x, g, dx, ds, None, hddm_lb.FP64)
-
-    for _ in range(max(1, args.warmup // 2)):
-        step()
-    torch.cuda.synchronize()
-    steps = max(1, args.steps // 2)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        step()
-    torch.cuda.synchronize()
-    ms = (time.perf_counter() - t0) * 1e3 / steps
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        ctx.forward(x, y, None, prec)
-    torch.cuda.synchronize()
-    fwd_ms = (time.perf_counter() - t0) * 1e3 / steps
+    out = {"workload": "forward + adjoint, D=20, %d nodes, dofs=32, %d points" % (ii.shape[0], n)}
+    oracle = oracle_mod()
+    pick = np.random.default_rng(408).choice(n, size=32, replace=False)
+    dxo, _ = oracle.adjoint(ii, jj, S, X[pick], G[pick])
+    yo = oracle.forward(ii, jj, S, X[pick])
+    for name, (pf, pa) in (("mixed", (hddm_lb.F32, hddm_lb.F32)), ("fp64", (hddm_lb.FP64, hddm_lb.FP64))):
+        def step(pf=pf, pa=pa):
+            ctx.forward(x, y, None, pf)
+            ctx.adjoint(x, g, dx, ds, None, pa)
+        ms, kern = timed_steps(step, max(3, args.steps // 2), 2, ctx)
+        out[name] = {"ms_per_step": ms, "forward_sweep_ms": kern[0]["ms_per_step"],
+                     "adjoint_kernel_ms": kern[1]["ms_per_step"], "value": n * 32 / (ms * 1e-3),
+                     "max_rel_err": {"y": normwise_err(y.cpu().numpy()[pick], yo),
+                                     "dx": normwise_err(dx.cpu().numpy()[pick], dxo)}}
+    out["mixed_over_fp64"] = out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"]
     ctx.close()
-    return {"workload": "forward + adjoint, adaptive grid D=20, %d nodes, dofs=32, %d points" % (ii.shape[0], n),
-            "ms_per_step": ms, "forward_ms": fwd_ms, "adjoint_ms": ms - fwd_ms,
-            "point_dof_per_s": n * 32 / (ms * 1e-3),
-            "adjoint_precision": "fp64", "forward_precision": args.precision}
+    return out
 
 
-def normwise_err(y, yref):
-    import numpy as np
-    den = np.abs(yref).max()
-    return float(np.abs(y - yref).max() / den) if den > 0 else float(np.abs(y - yref).max())
+def legacy_per_point(reps):
+    """Interpolator::interpolate through the drop-in plugin, one point per
+    call (the reference host's API), timed by the stand-in host over `reps`
+    calls on the reference's own fixture (surplus.plt: D=59, 118 dofs)."""
+    host = os.path.join(ROOT, "tests", "plugin", "_build", "plugin_host")
+    plugin = os.path.join(ROOT, "hddm-solver-postprocessors_amd", "lib", "libpostprocessor.so")
+    if not (os.path.exists(host) and os.path.exists(plugin)):
+        return None
+    res = {}
+    with tempfile.TemporaryDirectory() as d:
+        import shutil
+        shutil.copy(os.path.join(ROOT, "tests", "golden", "surplus.plt"), os.path.join(d, "surplus.plt"))
+        for prec in ("fp64", "f16x2"):
+            with open(os.path.join(d, "hddm-solver.cfg"), "w") as f:
+                f.write("n_agents 59\npriorityHIP 4\nprecisionHIP %s\n" % prec)
+            p = subprocess.run([host, plugin, "latency", "surplus.plt", str(reps)], cwd=d,
+                               capture_output=True, text=True, timeout=300)
+            us = None
+            for line in p.stdout.splitlines():
+                f = line.split()
+                if f and f[0] == "us_per_call":
+                    us = float(f[1])
+            res[prec] = us
+    return {"us_per_call": res, "calls": reps, "points_per_call": 1, "dofs": 118,
+            "reference_cuda_us_per_state": REF_CUDA_US_PER_STATE,
+            "note": "host pointers in and out, synchronous, as Interpolator::interpolate is called"}
 
 
-def cpu_baseline(ii, jj, S, X, ygpu, target_s):
-    """The reference's x86 xps/chains algorithm (the oracle's port of it) on
-    the host cores, on a bounded prefix of the same points; also the checker
-    for the GPU results on that prefix (the port against the FP64 gold
-    restatement on its first 256 points)."""
-    import numpy as np
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle
+def cpu_baseline(ii, jj, S, X, target_s, what):
+    """The reference's x86 xps/chains algorithm (the oracle's port of it; the
+    reference has no adjoint, so the forward) on the host cores: OpenMP over
+    points on a prefix sized to about target_s seconds, plus one thread on a
+    fixed 128-point subset."""
+    oracle = oracle_mod()
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     st = oracle.X86State(ii, jj, S)
+    # rate probe long enough to be stable (~0.5 s)
     probe = 256
-    t0 = time.perf_counter()
-    st.forward(X[:probe], threads)
-    dt = time.perf_counter() - t0
+    while True:
+        t0 = time.perf_counter()
+        st.forward(X[:probe], threads)
+        dt = time.perf_counter() - t0
+        if dt > 0.5 or probe >= X.shape[0]:
+            break
+        probe = min(X.shape[0], probe * 4)
     m = int(min(X.shape[0], max(probe, probe * target_s / max(dt, 1e-6))))
     t0 = time.perf_counter()
     yc = st.forward(X[:m], threads)
     dt = time.perf_counter() - t0
-    gold = oracle.forward(ii, jj, S, X[:probe])
-    errs = {k: normwise_err(v[:m], yc) for k, v in ygpu.items()}
+    t0 = time.perf_counter()
+    st.forward(X[:128], 1)
+    dt1 = time.perf_counter() - t0
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    gold = oracle.forward(ii, jj, S, X[:16])
     return {
         "value": m * S.shape[1] / dt,
         "unit": "point·dof/s",
         "cores": threads,
         "kind": "port",
-        "sample": "first %d of the %d points, x86 xps/chains algorithm (FP64), OpenMP static over points, %.1f s"
-                  % (m, X.shape[0], dt),
-    }, errs, normwise_err(yc[:probe], gold)
+        "sample": "first %d of the %d points of %s: forward only, x86 xps/chains algorithm (FP64), OpenMP "
+                  "static over points, %.1f s" % (m, X.shape[0], what, dt),
+        "one_thread": {"value": 128 * S.shape[1] / dt1, "sample": "first 128 points, 1 thread, %.2f s" % dt1},
+        "cpu_model": model,
+        "port_vs_gold_max_rel_err": normwise_err(yc[:16], gold),
+    }
+
+
+def cpu_regular_d4():
+    """The plain FP64 CPU case: regular grid D=4, sum(l-1) <= 4 (401 nodes),
+    dofs=8, 4,096 points; the x86 port on one thread and the gold
+    restatement on one thread."""
+    import numpy as np
+    import hddm_lb
+    oracle = oracle_mod()
+    ii, jj, lv, _ = hddm_lb.generate_grid(4, 4, 5, 0.0, 1, 0)
+    S = hddm_lb.synthetic_surplus(lv, 8, 2)
+    X = hddm_lb.synthetic_points(4096, 4, 3)
+    st = oracle.X86State(ii, jj, S)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        st.forward(X, 1)
+    dtx = (time.perf_counter() - t0) / 5
+    t0 = time.perf_counter()
+    oracle.forward(ii, jj, S, X, 1)
+    dtg = time.perf_counter() - t0
+    return {"workload": "regular grid D=4, %d nodes, dofs=8, 4096 points" % ii.shape[0],
+            "x86_port_one_thread": 4096 * 8 / dtx, "gold_one_thread": 4096 * 8 / dtg, "unit": "point·dof/s"}
+
+
+def roofline_of(head, work):
+    """Roofline of the dominant kernel of the mixed step, from algorithmic
+    FLOPs (non-zero (point, node) pairs only) over its HIP-event duration."""
+    mixed = head["mixed"]
+    fwd_ms = mixed["kernels"][0]["ms_per_launch"]
+    adj_ms = mixed["kernels"][1]["ms_per_launch"]
+    nnz = float(work["nnz"])
+    dofs = 128
+    # adjoint: dS (2 nnz dofs) and g.S (2 nnz dofs); forward: y (2 nnz dofs)
+    adj_flops = 4.0 * nnz * dofs
+    fwd_flops = 2.0 * nnz * dofs
+    # executed MFMA work of the adjoint kernel: per 32-node chunk and 32-dof
+    # tile, 32 MFMAs for dS and 32 for g.S, 4,096 FLOP each
+    exec_flops = float(work["chunks32"]) * 4 * 64 * 4096
+    dominant = "adjoint_ws_kernel (f32)" if adj_ms >= fwd_ms else "forward sweep (f16x2)"
+    if adj_ms >= fwd_ms:
+        t, flops, peak = adj_ms, adj_flops, PEAK_TFLOPS["f32"]
+    else:
+        t, flops, peak = fwd_ms, fwd_flops, PEAK_TFLOPS["f16x2"]
+    achieved = flops / (t * 1e-3) / 1e12
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("adjoint_ws_f32_d59x4" if adj_ms >= fwd_ms else "sweep_f16x2_d59x4")
+    return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+            "traffic": traffic, "kernel": dominant, "kernel_ms": t, "flops_per_launch": flops,
+            "flops_basis": "non-zero (point, node) pairs: %d; per pair and dof 2 FLOP (forward), 4 (adjoint)" % int(nnz),
+            "frac_executed": exec_flops / (adj_ms * 1e-3) / 1e12 / PEAK_TFLOPS["f32"],
+            "flops_dense_per_launch": 4.0 * work["nno"] * dofs * 1048576,
+            "note": "frac: algorithmic FLOPs of the non-zero pairs only; frac_executed: the MFMA work the "
+                    "adjoint kernel issues on its 32-node chunks (a 64-point block contracts every node "
+                    "that is non-zero at any of its points)"}
 
 
 def main():
@@ -198,14 +436,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--precision", default="f32", choices=["f16x2", "bf16x3", "f32"])
     ap.add_argument("--points", type=int, default=1 << 20)
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target length of the CPU baseline sample")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and error check")
-    ap.add_argument("--no-adjoint", action="store_true",
-                    help="skip the forward+adjoint measurement on the differentiated workload")
-    ap.add_argument("--no-multistate", action="store_true",
-                    help="skip the two multistate measurements (D=20 x 12 states, D=59 x 4 states)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-secondary", action="store_true", help="skip the secondary workloads")
+    ap.add_argument("--latency-calls", type=int, default=2000)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device 3 times (calibrates profiler byte counters)")
     args = ap.parse_args()
@@ -214,9 +449,8 @@ def main():
               "run one process per GPU on its own share). Use --gpus 1.", file=sys.stderr)
         return 2
 
-    import numpy as np
     import torch
-    import hddm_lb
+    import hddm_lb  # noqa: F401
 
     if args.copy_probe > 0:
         src = torch.ones(args.copy_probe << 17, dtype=torch.float64, device="cuda")
@@ -226,102 +460,67 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    precs = {"f16x2": hddm_lb.F16X2, "bf16x3": hddm_lb.BF16X3, "f32": hddm_lb.F32, "fp64": hddm_lb.FP64}
-    ii, jj, S, X = build_workload(args.points)
-    nno, dofs, N = ii.shape[0], S.shape[1], X.shape[0]
-    factors = int(np.count_nonzero((ii != 0) | (jj != 0)))
-    ctx = hddm_lb.Context(0)
-    ctx.upload_state(0, ii, jj, S)
-    x = torch.from_numpy(X).cuda()
-    y = torch.zeros((N, dofs), dtype=torch.float64, device="cuda")
-
-    res, ygpu = {}, {}
-    for name in (args.precision, "fp64"):
-        ms, sweep_ms, mn, mx = time_path(ctx, x, y, precs[name], args.steps, args.warmup)
-        res[name] = (ms, sweep_ms, mn, mx)
-        ygpu[name] = y.cpu().numpy()
-    ms, sweep_ms, _, _ = res[args.precision]
-    # executed work of one sweep (outside the timed steps): trie ops evaluated
-    # per 64-point block, chunks contracted and MFMA instructions issued
-    ctx.set_option(hddm_lb.OPT_STATS, 1)
-    ctx.forward(x, y, None, precs[args.precision])
-    nops_ex, nprune, nchunk, nitem, nmfma = ctx.last_stats(5)
-    ctx.set_option(hddm_lb.OPT_STATS, 0)
-
-    # algorithmic work of one launch: dense contraction 2*nno*dofs plus
-    # 4 flops per active factor, per point
-    flops = N * (2.0 * nno * dofs + 4.0 * factors)
-    # contraction work executed: the MFMA instructions issued (empty K-steps
-    # and all-zero 32-point halves are skipped); on the FP64 path (VALU) the
-    # 8-node chunks counted whole
-    if args.precision in MFMA_FLOPS:
-        exec_flops = nmfma * float(MFMA_FLOPS[args.precision])
-    else:
-        exec_flops = nchunk * 2.0 * 8 * 64 * 32
-    chunk_tflops = exec_flops / (sweep_ms * 1e-3) / 1e12
-    achieved = flops / (sweep_ms * 1e-3) / 1e12
-    peak = PEAK_TFLOPS[args.precision]
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("sweep_%s_d16_60k_1m" % args.precision)
-
+    head, work = headline(args)
+    n = args.points
+    mixed_ms = head["mixed"]["ms_per_step"]
+    fp64_ms = head["fp64"]["ms_per_step"]
     out = {
         "metric": METRIC,
-        "value": N * dofs / (ms * 1e-3),
+        "value": n * 128 / (mixed_ms * 1e-3),
         "unit": "point·dof/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": mixed_ms,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": DTYPE[args.precision],
-        "data": "synthetic (seeded grid, surpluses U[-1,1]*2^-sum(l), points U[0,1)^16)",
-        "config": {"workload": "single-state forward, adaptive sparse grid D=16, %d nodes, dofs=%d, %d points"
-                               % (nno, dofs, N),
-                   "dim": 16, "nno": nno, "dofs": dofs, "points": N, "precision": args.precision},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": traffic,
-                     "kernel": "sweep_kernel (%s)" % args.precision, "kernel_ms": sweep_ms,
-                     "flops_per_launch": flops,
-                     "frac_executed": chunk_tflops / peak,
-                     "note": "achieved counts the dense contraction over all nodes; the trie walk "
-                             "skips nodes whose basis is zero on a whole 64-point block, so frac "
-                             "can exceed 1. frac_executed: FLOP of the MFMA instructions the "
-                             "contraction issued over the same peak"},
-        "fp64_plain": {"value": N * dofs / (res["fp64"][0] * 1e-3), "ms_per_step": res["fp64"][0],
-                       "kernel_ms": res["fp64"][1],
-                       "roofline_frac": flops / (res["fp64"][1] * 1e-3) / 1e12 / PEAK_TFLOPS["fp64"]},
-        "mixed_over_fp64": res["fp64"][0] / ms,
-        "executed": {"walk_ops_per_block": nops_ex / max(nitem, 1),
-                     "pruned_runs_per_block": nprune / max(nitem, 1),
-                     "chunks_per_block": nchunk / max(nitem, 1), "blocks": nitem,
-                     "mfma_per_block": nmfma / max(nitem, 1), "mfma_tflops": chunk_tflops},
+        "dtype": "f16",
+        "data": "synthetic (seeded regular grids, surpluses U[-1,1]*2^-sum(l), points U[0,1)^59, gbar N(0,1))",
+        "config": {"workload": "multistate forward + adjoint, 4 states, D=59, regular grids sum(l-1) <= 3 "
+                               "(%d nodes each), dofs=128, %d points" % (work["nno"], n),
+                   "states": 4, "dim": 59, "nno_per_state": work["nno"], "dofs": 128, "points": n,
+                   "forward_precision": "f16x2 (f16 hi/lo MFMA, FP32 accumulate, FP64 fold per chunk)",
+                   "adjoint_precision": "f32 (FP32 MFMA; w summed and carried in FP64)",
+                   "forward_config": head["mixed"]["forward_config"]},
+        "roofline": roofline_of(head, work),
+        "fp64_plain": {"value": n * 128 / (fp64_ms * 1e-3), "ms_per_step": fp64_ms, "steps": head["fp64"]["steps"],
+                       "kernels": head["fp64"]["kernels"], "forward_config": head["fp64"]["forward_config"],
+                       "max_rel_err": head["fp64"]["max_rel_err"]},
+        "mixed_over_fp64": fp64_ms / mixed_ms,
+        "mixed_kernels": head["mixed"]["kernels"],
+        "max_rel_err": head["mixed"]["max_rel_err"],
+        "executed": work,
     }
-    # the same forward through host buffers (what the plugin's interpolate()
-    # uses): x in and y out over PCIe, synchronous
-    yh = np.empty((N, dofs))
-    ctx.forward_host(X, yh, None, precs[args.precision])
-    t0 = time.perf_counter()
-    ctx.forward_host(X, yh, None, precs[args.precision])
-    host_ms = (time.perf_counter() - t0) * 1e3
-    out["host_buffers"] = {"ms_per_step": host_ms, "value": N * dofs / (host_ms * 1e-3),
-                           "note": "x and y in pageable host memory, copies included"}
-    if not args.no_adjoint:
-        out["forward_adjoint"] = time_adjoint(args, precs[args.precision])
-    if not args.no_multistate:
-        out["multistate"] = {k: time_multistate(args, precs[args.precision], k) for k in ("d20x12", "d59x4")}
+    say("bench: headline done")
+    if not args.no_secondary:
+        out["single_state_d16"], c2 = single_state_d16(args)
+        say("bench: single-state done")
+        out["multistate_d20x12"] = multistate_d20(args)
+        say("bench: multistate done")
+        out["forward_adjoint_d20"] = forward_adjoint_d20(args)
+        say("bench: forward+adjoint D=20 done")
+        out["legacy_per_point"] = legacy_per_point(args.latency_calls)
+        say("bench: per-point latency done")
+    else:
+        c2 = None
     if not args.no_cpu:
-        cb, errs, port_err = cpu_baseline(ii, jj, S, X, ygpu, args.cpu_seconds)
-        out["cpu_baseline"] = cb
-        out["max_rel_err"] = {k: v for k, v in errs.items()}
-        out["max_rel_err"]["cpu_port_vs_gold"] = port_err
+        import numpy as np
+        import hddm_lb as h
+        # the headline workload's state 0 and its points
+        ii, jj, lv, _ = h.generate_grid(59, 3, 4, 0.0, 500, 0)
+        S = h.synthetic_surplus(lv, 128, 600)
+        X = h.synthetic_points(n, 59, 77)
+        sid = np.random.default_rng(2024).integers(0, 4, size=n).astype(np.int32)
+        out["cpu_baseline"] = cpu_baseline(ii, jj, S, np.ascontiguousarray(X[sid == 0]), args.cpu_seconds,
+                                           "state 0 of the D=59 workload (281,077 nodes, 128 dofs)")
+        if c2 is not None:
+            out["cpu_baseline"]["single_state_d16"] = cpu_baseline(*c2, max(2.0, args.cpu_seconds / 4),
+                                                                   "the single-state D=16 forward")
+        out["cpu_baseline"]["regular_d4"] = cpu_regular_d4()
+        say("bench: CPU baseline done")
     else:
         out["cpu_baseline"] = None
-    ctx.close()
     print(json.dumps(out), flush=True)
     return 0
 

@@ -578,6 +578,17 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 	a.status = ctx->d_status;
 
 	const bool ws = precision == HDDM_LB_F32 || ctx->opt_form != 1;
+	if (ws && ctx->opt_stats)
+	{
+		if (!ctx->d_adj_stats)
+		{
+			int arc = 0;
+			ctx->d_adj_stats = (unsigned long long*)dev_alloc(hddm_lb_ctx::kAdjStatCount * 8, arc);
+			if (arc) return arc;
+		}
+		HIP_TRY(launch_fill_u32((uint32_t*)ctx->d_adj_stats, 2 * hddm_lb_ctx::kAdjStatCount, 0u, st));
+		a.stats = ctx->d_adj_stats;
+	}
 	if (ws)
 	{
 		// one walker wave and four consumer waves per workgroup, four dof

@@ -504,12 +504,20 @@ __global__ void __launch_bounds__(64 * (kANC + 1)) adjoint_ws_kernel(AdjointArgs
 		wb.DX = (double*)(lbase + L.dx);
 		int npub = 0, nchunks = 0;
 		bool ok = true;
+		// statistics mode: shader-clock cycles of walk A, walk B, waits for w
+		// and waits for ring slots; items and chunks
+		const bool st_on = a.stats != nullptr;
+		uint64_t cyc_a = 0, cyc_b = 0, cyc_w = 0, cyc_s = 0, st_items = 0, st_chunks = 0;
+		auto clk = [&]() -> uint64_t { return st_on ? __builtin_amdgcn_s_memtime() : 0; };
 		auto acquire = [&]() -> bool {
 			if (npub < kARing) return true;
+			const uint64_t t0 = clk();
+			bool r = true;
 #pragma unroll 1
 			for (int c = 0; c < kANC; c++)
-				if (!a_wait(&sy->done[c], npub - kARing + 1, sy, a.status)) return false;
-			return true;
+				if (!a_wait(&sy->done[c], npub - kARing + 1, sy, a.status)) { r = false; break; }
+			cyc_s += clk() - t0;
+			return r;
 		};
 		auto publish = [&](int type, int a0, int b0, int np, int64_t start, int64_t lpos) {
 			AMsg* m = msgs + (npub & (kARing - 1));
@@ -523,10 +531,13 @@ __global__ void __launch_bounds__(64 * (kANC + 1)) adjoint_ws_kernel(AdjointArgs
 		};
 		// waits until every consumer has handled message m
 		auto wait_msg = [&](int m) -> bool {
+			const uint64_t t0 = clk();
+			bool r = true;
 #pragma unroll 1
 			for (int c = 0; c < kANC; c++)
-				if (!a_wait(&sy->done[c], m + 1, sy, a.status)) return false;
-			return true;
+				if (!a_wait(&sy->done[c], m + 1, sy, a.status)) { r = false; break; }
+			cyc_w += clk() - t0;
+			return r;
 		};
 		for (;;)
 		{
@@ -590,16 +601,21 @@ __global__ void __launch_bounds__(64 * (kANC + 1)) adjoint_ws_kernel(AdjointArgs
 				if (prev_chunk < 0 || !want_dx) return true;
 				if (!wait_msg(prev_msg)) return false;
 				wpar = prev_par;
+				const uint64_t t0 = clk();
 				wb.advance(prev_chunk, wof, lane, xv0, xv1, xv2, xv3);
+				cyc_b += clk() - t0;
 				return true;
 			};
 			for (;;)
 			{
 				int node = 0;
 				double val = 0.0;
+				const uint64_t ta = clk();
 				const bool more = wa.run(nullptr, nullptr, q0, q1, lane, xb0, xb1, xb2, xb3, node, val);
+				cyc_a += clk() - ta;
 				if (wa.chunk >= 0)
 				{
+					st_chunks++;
 					const int m = npub, par = nchunks & 1;
 					publish(kAMsgChunk, wa.chunk, (int)wa.mask, 0, 0, 0);
 					nchunks++;
@@ -620,9 +636,12 @@ __global__ void __launch_bounds__(64 * (kANC + 1)) adjoint_ws_kernel(AdjointArgs
 			// walk B ends with walk A's last chunk: a node left to emit beyond
 			// it means the two walks disagreed
 			wpar = prev_par;
+			const uint64_t tb = clk();
 			if (!wb.advance(prev_chunk, wof, lane, xv0, xv1, xv2, xv3))
 				if (lane == 0) atomicOr(a.status, (uint32_t)HDDM_LB_STATUS_WALK_MISMATCH);
 			wb.leave_from(0, lane, xv0, xv1, xv2, xv3);
+			cyc_b += clk() - tb;
+			st_items++;
 			if (lane < np)
 			{
 				double* out = ldc(&sd->nseg) > 1
@@ -637,6 +656,15 @@ __global__ void __launch_bounds__(64 * (kANC + 1)) adjoint_ws_kernel(AdjointArgs
 		}
 		if (ok && acquire()) publish(kAMsgExit, 0, 0, 0, 0, 0);
 		else a_release(&sy->abort, 1);
+		if (st_on && lane == 0)
+		{
+			atomicAdd(&a.stats[0], (unsigned long long)cyc_a);
+			atomicAdd(&a.stats[1], (unsigned long long)cyc_b);
+			atomicAdd(&a.stats[2], (unsigned long long)cyc_w);
+			atomicAdd(&a.stats[3], (unsigned long long)cyc_s);
+			atomicAdd(&a.stats[7], (unsigned long long)st_items);
+			atomicAdd(&a.stats[8], (unsigned long long)st_chunks);
+		}
 		return;
 	}
 
@@ -648,9 +676,15 @@ __global__ void __launch_bounds__(64 * (kANC + 1)) adjoint_ws_kernel(AdjointArgs
 	int role = 0, tile = 0, dofs = 0, DP = 32;
 	int nchunks = 0;
 	double* dsb = nullptr;
+	const bool st_on = a.stats != nullptr;
+	uint64_t cyc_ds = 0, cyc_wc = 0, cyc_idle = 0;
+	auto clk = [&]() -> uint64_t { return st_on ? __builtin_amdgcn_s_memtime() : 0; };
 	for (int n = 0;; n++)
 	{
-		if (!a_wait(&sy->produced, n + 1, sy, a.status)) break;
+		const uint64_t t0 = clk();
+		const bool got = a_wait(&sy->produced, n + 1, sy, a.status);
+		cyc_idle += clk() - t0;
+		if (!got) break;
 		const AMsg* m = msgs + (n & (kARing - 1));
 		const int type = uniform(m->type);
 		if (type == kAMsgChunk)
@@ -662,8 +696,12 @@ __global__ void __launch_bounds__(64 * (kANC + 1)) adjoint_ws_kernel(AdjointArgs
 				const int chunk = uniform(m->a);
 				const uint32_t mask = (uint32_t)uniform(m->b);
 				const T* Bc = slot(n);
+				const uint64_t t1 = clk();
 				if ((role & kRoleDs) && a.ds) cons.ds(a, sd, dsb, Bc, chunk, mask, tile, dofs, lane);
+				const uint64_t t2 = clk();
 				if (role & kRoleW) cons.w(sd, wbuf(par, c), chunk, mask, tile, DP, lane);
+				cyc_ds += t2 - t1;
+				cyc_wc += clk() - t2;
 			}
 		}
 		else if (type == kAMsgItem)
@@ -705,6 +743,12 @@ __global__ void __launch_bounds__(64 * (kANC + 1)) adjoint_ws_kernel(AdjointArgs
 			break;
 		}
 		a_release(&sy->done[c], n + 1);
+	}
+	if (st_on && lane == 0)
+	{
+		atomicAdd(&a.stats[4], (unsigned long long)cyc_wc);
+		atomicAdd(&a.stats[5], (unsigned long long)cyc_ds);
+		atomicAdd(&a.stats[6], (unsigned long long)cyc_idle);
 	}
 }
 

@@ -91,10 +91,14 @@ using namespace hddm_lb;
 hddm_lb_ctx::~hddm_lb_ctx()
 {
 	DeviceGuard g(device);
+	for (auto& kv : small_plans) hddm_lb_plan_destroy(kv.second);
+	small_plans.clear();
+	for (void* p : { (void*)h_xstage, (void*)h_ystage, (void*)h_sidstage })
+		if (p) (void)hipHostFree(p);
 	states.clear();
 	scratch.release();
 	for (void* p : { (void*)d_states, (void*)d_cost_order, (void*)d_state_ok,
-		(void*)d_xstage, (void*)d_ystage, (void*)d_sidstage, (void*)d_wbuf, (void*)d_stats, (void*)d_status })
+		(void*)d_xstage, (void*)d_ystage, (void*)d_sidstage, (void*)d_wbuf, (void*)d_stats, (void*)d_status, (void*)d_adj_stats })
 		dev_free(p);
 	if (scratch_event) (void)hipEventDestroy(scratch_event);
 	for (auto& r : tring)
@@ -305,6 +309,9 @@ extern "C" int hddm_lb_surplus_offset(hddm_lb_ctx* ctx, int istate, int64_t* off
 
 namespace hddm_lb {
 
+// Host-pointer calls of at most this many points replay a captured forward.
+constexpr int64_t kSmallBatch = 64;
+
 int scratch_wait(hddm_lb_ctx* ctx, hipStream_t st)
 {
 	if (ctx->scratch_pending && ctx->scratch_stream != st)
@@ -418,6 +425,28 @@ static int grow(T*& p, size_t& cap, size_t need)
 	size_t n = std::max<size_t>(need, 1);
 	p = (T*)dev_alloc(sizeof(T) * n, rc);
 	if (rc) return rc;
+	cap = n;
+	return HDDM_LB_SUCCESS;
+}
+
+// Pinned host buffer of at least `need` elements (hipHostMalloc; never
+// hipHostRegister).
+template <typename T>
+static int grow_pinned(T*& p, size_t& cap, size_t need)
+{
+	if (need <= cap && p) return HDDM_LB_SUCCESS;
+	if (p) { HIP_TRY(hipDeviceSynchronize()); (void)hipHostFree(p); }
+	p = nullptr;
+	cap = 0;
+	const size_t n = std::max<size_t>(need, 64);
+	void* q = nullptr;
+	hipError_t e = hipHostMalloc(&q, sizeof(T) * n, hipHostMallocDefault);
+	if (e != hipSuccess)
+	{
+		(void)hipGetLastError();
+		return set_error(HDDM_LB_ERROR_OUT_OF_MEMORY, "hipHostMalloc(%zu): %s", sizeof(T) * n, hipGetErrorString(e));
+	}
+	p = (T*)q;
 	cap = n;
 	return HDDM_LB_SUCCESS;
 }
@@ -602,6 +631,19 @@ extern "C" int hddm_lb_last_config(hddm_lb_ctx* ctx, int* config)
 	return HDDM_LB_SUCCESS;
 }
 
+extern "C" int hddm_lb_last_adjoint_stats(hddm_lb_ctx* ctx, uint64_t* stats, int n)
+{
+	if (!ctx || !stats) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "last_adjoint_stats: NULL argument");
+	if (n < 1 || n > hddm_lb_ctx::kAdjStatCount)
+		return set_error(HDDM_LB_ERROR_INVALID_VALUE, "last_adjoint_stats: %d counters requested, 1 to %d available", n,
+			hddm_lb_ctx::kAdjStatCount);
+	if (!ctx->d_adj_stats) return set_error(HDDM_LB_ERROR_STATE, "last_adjoint_stats: HDDM_LB_OPT_STATS was not set for an adjoint");
+	DeviceGuard g(ctx->device);
+	HIP_TRY(hipDeviceSynchronize());
+	HIP_TRY(hipMemcpy(stats, ctx->d_adj_stats, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToHost));
+	return HDDM_LB_SUCCESS;
+}
+
 extern "C" int hddm_lb_last_stats(hddm_lb_ctx* ctx, uint64_t* stats)
 {
 	return hddm_lb_last_stats_n(ctx, stats, 4);
@@ -652,33 +694,6 @@ extern "C" int hddm_lb_sweep_time_summary(hddm_lb_ctx* ctx, int64_t* ncalls, dou
 	return hddm_lb_kernel_time_summary(ctx, 0, ncalls, total_ms, min_ms, max_ms);
 }
 
-extern "C" int hddm_lb_forward_host(hddm_lb_ctx* ctx, int64_t npoints,
-	const double* x, int64_t ldx, const int32_t* state_ids,
-	double* y, int64_t ldy, int precision)
-{
-	int rc = check_forward_args(ctx, npoints, x, ldx, y, ldy, precision);
-	if (rc) return rc;
-	if (npoints == 0) return HDDM_LB_SUCCESS;
-	DeviceGuard g(ctx->device);
-	if (!ctx->own_stream) HIP_TRY(hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking));
-	if ((rc = grow(ctx->d_xstage, ctx->cap_xstage, (size_t)npoints * ldx))) return rc;
-	if ((rc = grow(ctx->d_ystage, ctx->cap_ystage, (size_t)npoints * ldy))) return rc;
-	if (state_ids && (rc = grow(ctx->d_sidstage, ctx->cap_sidstage, (size_t)npoints))) return rc;
-	hipStream_t st = ctx->own_stream;
-	HIP_TRY(hipMemcpyAsync(ctx->d_xstage, x, sizeof(double) * npoints * ldx, hipMemcpyHostToDevice, st));
-	if (state_ids) HIP_TRY(hipMemcpyAsync(ctx->d_sidstage, state_ids, sizeof(int32_t) * npoints, hipMemcpyHostToDevice, st));
-	// rows of points with an invalid state stay as the caller passed them
-	HIP_TRY(hipMemcpyAsync(ctx->d_ystage, y, sizeof(double) * npoints * ldy, hipMemcpyHostToDevice, st));
-	rc = hddm_lb_forward(ctx, npoints, ctx->d_xstage, ldx, state_ids ? ctx->d_sidstage : nullptr,
-		ctx->d_ystage, ldy, precision, st);
-	if (rc) return rc;
-	HIP_TRY(hipMemcpyAsync(y, ctx->d_ystage, sizeof(double) * npoints * ldy, hipMemcpyDeviceToHost, st));
-	HIP_TRY(hipStreamSynchronize(st));
-	return HDDM_LB_SUCCESS;
-}
-
-// ---- hipGraph plans ----
-
 struct hddm_lb_plan
 {
 	hddm_lb_ctx* ctx;
@@ -692,6 +707,78 @@ struct hddm_lb_plan
 		if (graph) (void)hipGraphDestroy(graph);
 	}
 };
+
+extern "C" int hddm_lb_forward_host(hddm_lb_ctx* ctx, int64_t npoints,
+	const double* x, int64_t ldx, const int32_t* state_ids,
+	double* y, int64_t ldy, int precision)
+{
+	int rc = check_forward_args(ctx, npoints, x, ldx, y, ldy, precision);
+	if (rc) return rc;
+	if (npoints == 0) return HDDM_LB_SUCCESS;
+	DeviceGuard g(ctx->device);
+	if (!ctx->own_stream) HIP_TRY(hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking));
+	const bool small = npoints <= kSmallBatch;
+	// small batches size the device staging for kSmallBatch points, so the
+	// captured forwards that point at it stay valid until a row stride grows
+	const size_t nx = (size_t)(small ? kSmallBatch : npoints) * ldx;
+	const size_t ny = (size_t)(small ? kSmallBatch : npoints) * ldy;
+	const size_t ns = (size_t)(small ? kSmallBatch : npoints);
+	if (nx > ctx->cap_xstage || ny > ctx->cap_ystage || (state_ids && ns > ctx->cap_sidstage))
+	{
+		for (auto& kv : ctx->small_plans) hddm_lb_plan_destroy(kv.second);
+		ctx->small_plans.clear();
+	}
+	if ((rc = grow(ctx->d_xstage, ctx->cap_xstage, nx))) return rc;
+	if ((rc = grow(ctx->d_ystage, ctx->cap_ystage, ny))) return rc;
+	if (state_ids && (rc = grow(ctx->d_sidstage, ctx->cap_sidstage, ns))) return rc;
+	if ((rc = grow_pinned(ctx->h_xstage, ctx->cap_hx, (size_t)npoints * ldx))) return rc;
+	if ((rc = grow_pinned(ctx->h_ystage, ctx->cap_hy, (size_t)npoints * ldy))) return rc;
+	if (state_ids && (rc = grow_pinned(ctx->h_sidstage, ctx->cap_hsid, (size_t)npoints))) return rc;
+	hipStream_t st = ctx->own_stream;
+	// pageable caller memory -> pinned staging on the host, one DMA each way
+	std::memcpy(ctx->h_xstage, x, sizeof(double) * npoints * ldx);
+	if (state_ids) std::memcpy(ctx->h_sidstage, state_ids, sizeof(int32_t) * npoints);
+	// rows of points with an invalid state stay as the caller passed them
+	std::memcpy(ctx->h_ystage, y, sizeof(double) * npoints * ldy);
+	if ((rc = scratch_wait(ctx, st))) return rc;
+	HIP_TRY(hipMemcpyAsync(ctx->d_xstage, ctx->h_xstage, sizeof(double) * npoints * ldx, hipMemcpyHostToDevice, st));
+	if (state_ids)
+		HIP_TRY(hipMemcpyAsync(ctx->d_sidstage, ctx->h_sidstage, sizeof(int32_t) * npoints, hipMemcpyHostToDevice, st));
+	HIP_TRY(hipMemcpyAsync(ctx->d_ystage, ctx->h_ystage, sizeof(double) * npoints * ldy, hipMemcpyHostToDevice, st));
+	if (small)
+	{
+		auto key = std::make_tuple(npoints, ldx, ldy, state_ids ? 1 : 0, precision);
+		auto it = ctx->small_plans.find(key);
+		if (it != ctx->small_plans.end() && it->second->generation != ctx->generation)
+		{
+			hddm_lb_plan_destroy(it->second);
+			ctx->small_plans.erase(it);
+			it = ctx->small_plans.end();
+		}
+		if (it == ctx->small_plans.end())
+		{
+			hddm_lb_plan* plan = nullptr;
+			if ((rc = hddm_lb_plan_create(ctx, npoints, ctx->d_xstage, ldx, state_ids ? ctx->d_sidstage : nullptr,
+					ctx->d_ystage, ldy, precision, st, &plan)))
+				return rc;
+			it = ctx->small_plans.emplace(key, plan).first;
+		}
+		if ((rc = hddm_lb_plan_launch(it->second, st))) return rc;
+	}
+	else
+	{
+		rc = hddm_lb_forward(ctx, npoints, ctx->d_xstage, ldx, state_ids ? ctx->d_sidstage : nullptr,
+			ctx->d_ystage, ldy, precision, st);
+		if (rc) return rc;
+	}
+	HIP_TRY(hipMemcpyAsync(ctx->h_ystage, ctx->d_ystage, sizeof(double) * npoints * ldy, hipMemcpyDeviceToHost, st));
+	HIP_TRY(hipStreamSynchronize(st));
+	std::memcpy(y, ctx->h_ystage, sizeof(double) * npoints * ldy);
+	return HDDM_LB_SUCCESS;
+}
+
+// ---- hipGraph plans ----
+
 
 extern "C" int hddm_lb_plan_create(hddm_lb_ctx* ctx, int64_t npoints,
 	const double* x, int64_t ldx, const int32_t* state_ids,

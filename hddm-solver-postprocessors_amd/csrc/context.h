@@ -102,6 +102,13 @@ struct hddm_lb_ctx
 	double* d_ystage = nullptr; size_t cap_ystage = 0;
 	int32_t* d_sidstage = nullptr; size_t cap_sidstage = 0;
 	hipStream_t own_stream = nullptr;
+	// host-pointer calls: pinned host staging (hipHostMalloc) and, for small
+	// batches, captured forwards keyed by (points, ldx, ldy, state ids given,
+	// precision) that replay with one graph launch
+	double* h_xstage = nullptr; size_t cap_hx = 0;
+	double* h_ystage = nullptr; size_t cap_hy = 0;
+	int32_t* h_sidstage = nullptr; size_t cap_hsid = 0;
+	std::map<std::tuple<int64_t, int64_t, int64_t, int, int>, hddm_lb_plan*> small_plans;
 	// Eager calls share `scratch`: each one records scratch_event on its
 	// stream, and the next call on another stream waits for it first.
 	hipEvent_t scratch_event = nullptr;
@@ -117,6 +124,11 @@ struct hddm_lb_ctx
 	static constexpr int kStatCount = 6;
 	unsigned long long* d_stats = nullptr;
 	uint32_t* d_status = nullptr;   // HDDM_LB_STATUS_* bits raised by kernels
+	// wave-specialized adjoint in statistics mode: walker cycles in walk A,
+	// walk B, waiting for w, waiting for ring slots; consumer cycles in w, in
+	// dS, idle; items; chunks
+	static constexpr int kAdjStatCount = 9;
+	unsigned long long* d_adj_stats = nullptr;
 	// rings of event pairs bracketing the kernels of timed calls: [0] the
 	// sweep kernels of forwards, [1] the adjoint kernels
 	static constexpr int kTimingRing = 256;

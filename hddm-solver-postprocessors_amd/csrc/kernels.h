@@ -49,6 +49,7 @@ struct AdjointArgs
 	int dim;
 	int dt_base;       // dof slab of 32 handled by this launch
 	uint32_t* status;  // device status bits (HDDM_LB_STATUS_*)
+	unsigned long long* stats;  // optional cycle and work counters of the wave-specialized adjoint
 };
 
 // How a sweep reads the op stream: each wave through scalar loads, waves

@@ -85,6 +85,7 @@ _SIGS = {
     "hddm_lb_last_sweep_ms": (_dbl, [_vp]),
     "hddm_lb_last_stats": (_i32, [_vp, _vp]),
     "hddm_lb_last_config": (_i32, [_vp, _vp]),
+    "hddm_lb_last_adjoint_stats": (_i32, [_vp, _vp, _i32]),
     "hddm_lb_last_stats_n": (_i32, [_vp, _vp, _i32]),
     "hddm_lb_sweep_time_summary": (_i32, [_vp, _vp, _vp, _vp, _vp]),
     "hddm_lb_kernel_time_summary": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp]),
@@ -339,6 +340,12 @@ class Context:
         _check(lib.hddm_lb_kernel_time_summary(self._h, kind, ctypes.byref(n), ctypes.byref(tot), ctypes.byref(mn),
                                                ctypes.byref(mx)))
         return n.value, tot.value, mn.value, mx.value
+
+    def last_adjoint_stats(self, n=9):
+        """Counters of the last wave-specialized adjoint (statistics mode), see include/hddm_lb.h."""
+        st = np.zeros(n, np.uint64)
+        _check(lib.hddm_lb_last_adjoint_stats(self._h, _ptr(st), n))
+        return tuple(int(v) for v in st)
 
     def last_config(self):
         """Sweep configuration of the last forward: dict(formulation, waves, tiles, tuned, cache_hits)."""

@@ -10,10 +10,12 @@
 //   plugin_host <plugin.so> multistate <surplus.plt> <nstates>
 //   plugin_host <plugin.so> reload <surplus.plt>                 (loads state 0 twice: abort path)
 //   plugin_host <plugin.so> dofs <surplus.plt>                   (wrong DofPerNode: abort path)
+//   plugin_host <plugin.so> latency <surplus.plt> <calls>        (per-point calls: prints us_per_call)
 #include "interpolator.h"
 #include "process.h"
 
 #include <dlfcn.h>
+#include <time.h>
 #include <unistd.h>
 
 #include <cstdarg>
@@ -124,6 +126,7 @@ int main(int argc, char* argv[])
 	if (second) releaseDevice(second);
 
 	const int dim = 59, dofs = 118;
+	struct timespec t0 = {}, t1 = {};
 	if (!strcmp(mode, "kat") || !strcmp(mode, "dofs") || !strcmp(mode, "reload"))
 	{
 		Data* data = getData(1);
@@ -135,6 +138,28 @@ int main(int argc, char* argv[])
 		for (int r = 0; r < reps; r++)
 			interp->interpolate(device, reinterpret_cast<NAMESPACE::Data*>(data), 0, x.data(),
 				!strcmp(mode, "dofs") ? dofs - 1 : dofs, y.data());
+		printf("value");
+		for (int c = 0; c < dofs; c++) printf(" %.17g", y[c]);
+		printf("\n");
+		delete data;
+	}
+	else if (!strcmp(mode, "latency"))
+	{
+		// per-point calls, as the reference host makes them: reps calls after
+		// 20 warm-up calls, each on a point moved slightly along the diagonal
+		Data* data = getData(1);
+		data->load(plt, 0);
+		std::vector<double> x(dim), y(dofs, 0.0);
+		const int reps = argc > 4 ? atoi(argv[4]) : 1000;
+		for (int r = -20; r < reps; r++)
+		{
+			if (r == 0) clock_gettime(CLOCK_MONOTONIC, &t0);
+			for (int d = 0; d < dim; d++) x[d] = (d + 1) / 118.0 + 1e-6 * ((r + 20) % 97);
+			interp->interpolate(device, reinterpret_cast<NAMESPACE::Data*>(data), 0, x.data(), dofs, y.data());
+		}
+		clock_gettime(CLOCK_MONOTONIC, &t1);
+		const double us = ((t1.tv_sec - t0.tv_sec) * 1e9 + (t1.tv_nsec - t0.tv_nsec)) * 1e-3 / (reps > 0 ? reps : 1);
+		printf("us_per_call %.3f\n", us);
 		printf("value");
 		for (int c = 0; c < dofs; c++) printf(" %.17g", y[c]);
 		printf("\n");

@@ -326,3 +326,55 @@ def test_padded_rows(ctx):
         yh = y.cpu().numpy()
         assert np.array_equal(yh[:, :dofs], ref)
         assert np.isnan(yh[:, dofs:]).all()
+
+
+def test_forward_host_small_batches_replay_graphs(ctx):
+    """Host-pointer calls of up to 64 points replay a captured forward: sizes
+    on both sides of that bound, repeated calls, state ids with invalid rows,
+    a padded y, and a state re-uploaded between calls (the captured forward
+    is rebuilt)."""
+    ii, jj, lv, _ = hddm_lb.generate_grid(9, 2, 5, 0.4, 21, 3000)
+    S = hddm_lb.synthetic_surplus(lv, 24, 22)
+    S2 = hddm_lb.synthetic_surplus(lv, 24, 23)
+    c = hddm_lb.Context(0)
+    c.upload_state(0, ii, jj, S)
+    for n in (1, 3, 64, 65, 1, 64):
+        X = hddm_lb.synthetic_points(n, 9, 100 + n)
+        sid = np.zeros(n, np.int32)
+        sid[::3] = 5   # not loaded: row left as passed
+        y = np.full((n, 30), np.nan)
+        c.forward_host(X, y, sid, hddm_lb.FP64)
+        yo = oracle.forward(ii, jj, S, X)
+        ok = sid == 0
+        assert np.isnan(y[~ok]).all() and np.isnan(y[:, 24:]).all()
+        assert normwise_rel_err(y[ok, :24], yo[ok]) <= 1e-12, n
+    c.upload_state(0, ii, jj, S2)
+    X = hddm_lb.synthetic_points(3, 9, 103)
+    y = np.full((3, 24), np.nan)
+    c.forward_host(X, y, None, hddm_lb.F32)
+    assert normwise_rel_err(y, oracle.forward(ii, jj, S2, X)) <= 1e-6
+    c.close()
+
+
+def test_autotuner_picks_and_caches(ctx):
+    """With the autotuner on, the first forward of a key times the sweep
+    formulations and keeps one; later forwards of the key come from the
+    cache; every result matches the oracle."""
+    ii, jj, lv, _ = hddm_lb.generate_grid(40, 2, 4, 0.3, 31, 6000)
+    S = hddm_lb.synthetic_surplus(lv, 96, 32)
+    X = hddm_lb.synthetic_points(3000, 40, 33)
+    c = hddm_lb.Context(0)
+    c.set_option(hddm_lb.OPT_AUTOTUNE, 1)
+    c.upload_state(0, ii, jj, S)
+    pick = np.random.default_rng(1).choice(3000, size=64, replace=False)
+    yo = oracle.forward(ii, jj, S, X[pick])
+    for rep in range(3):
+        for prec in (hddm_lb.F32, hddm_lb.FP64):
+            y = run_forward(c, X, precision=prec)
+            assert normwise_rel_err(y[pick], yo) <= TOL[prec]
+    cfg = c.last_config()
+    say("autotuner config", cfg)
+    assert cfg["tuned"] == 2 and cfg["cache_hits"] == 4
+    assert cfg["formulation"] in (1, 2)
+    assert c.device_status() == 0
+    c.close()
