@@ -140,6 +140,9 @@ def headline(args):
         say("bench: D=59 %s step %.1f ms (forward sweep %.1f, adjoint %.1f), config %s"
             % (name, ms, kern[0]["ms_per_step"], kern[1]["ms_per_step"], cfg))
         assert status == 0, "device status %d after the %s leg" % (status, name)
+        if args.no_errors:
+            res[name]["max_rel_err"] = None
+            continue
         # errors against the oracle on a seeded subset: y and dx at 4 points per state
         Y = y.cpu().numpy()
         DX = dx.cpu().numpy()
@@ -154,7 +157,7 @@ def headline(args):
         res[name]["max_rel_err"] = {"y": ey, "dx": edx}
     # dS error on a 128-point slice (the oracle's dS cost grows with the points)
     m = 128
-    for name, (pf, pa) in legs.items():
+    for name, (pf, pa) in (() if args.no_errors else legs.items()):
         ds2 = torch.empty_like(ds)
         ctx.adjoint(x[:m], g[:m], None, ds2, sd[:m], pa)
         torch.cuda.synchronize()
@@ -440,6 +443,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-secondary", action="store_true", help="skip the secondary workloads")
+    ap.add_argument("--no-errors", action="store_true",
+                    help="skip the error checks against the oracle (profiling runs)")
     ap.add_argument("--latency-calls", type=int, default=2000)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device 3 times (calibrates profiler byte counters)")

@@ -154,8 +154,8 @@ __device__ __forceinline__ void leave_edge(const u4_t& e, double r, const double
 	rc = fma(rc, fma(-ph, rc, 1.0), rc);
 	rc = fma(rc, fma(-ph, rc, 1.0), rc);
 	const double qv = phi > 0.0 ? (u > 0.0 ? -iv : (u < 0.0 ? iv : 0.0)) * rc : 0.0;
-	DX[dd * kWave + lane] += r * qv;
-	if (tt > 0) R[(tt - 1) * kWave + lane] += r;
+	lds_add(DX + dd * kWave + lane, r * qv);
+	if (tt > 0) lds_add(R + (tt - 1) * kWave + lane, r);
 }
 
 // dx along the edge-sum walk.
@@ -244,8 +244,8 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 			{
 				if (wave_any(r_top != 0.0))
 				{
-					DX[dim_top * kWave + lane] += r_top * q_top;
-					if (tt > 0) R[(tt - 1) * kWave + lane] += r_top;
+					lds_add(DX + dim_top * kWave + lane, r_top * q_top);
+					if (tt > 0) lds_add(R + (tt - 1) * kWave + lane, r_top);
 				}
 				tt--;
 			}
@@ -315,8 +315,8 @@ __global__ void __launch_bounds__(256) adjoint_kernel(AdjointArgs a)
 		{
 			if (wave_any(r_top != 0.0))
 			{
-				DX[dim_top * kWave + lane] += r_top * q_top;
-				if (tt > 0) R[(tt - 1) * kWave + lane] += r_top;
+				lds_add(DX + dim_top * kWave + lane, r_top * q_top);
+				if (tt > 0) lds_add(R + (tt - 1) * kWave + lane, r_top);
 			}
 			tt--;
 		}
@@ -577,7 +577,13 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 	a.nstates = ns; a.dim = ctx->dim;
 	a.status = ctx->d_status;
 
-	const bool ws = precision == HDDM_LB_F32 || ctx->opt_form != 1;
+	// automatic: the wave-specialized kernel for wide states (D > 32) of two or
+	// more dof tiles, which it walks once instead of once per tile (measured:
+	// D=59, 128 dofs, 2^16 points: FP64 1255 -> 838 ms); the per-tile kernels
+	// otherwise (D=20, 32 dofs: 32 vs 110 ms). F32 runs on the wave-specialized
+	// kernel only.
+	const bool ws = precision == HDDM_LB_F32 || ctx->opt_form == 2
+		|| (ctx->opt_form == 0 && xbanks_for(ctx->dim) == 4 && ctx->dt_needed >= 2);
 	if (ws && ctx->opt_stats)
 	{
 		if (!ctx->d_adj_stats)

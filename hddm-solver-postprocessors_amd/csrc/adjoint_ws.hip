@@ -156,6 +156,7 @@ struct DxWalk
 	double* DX;      // [dim][64]
 	u4_t* edges;     // [PM] op words {2 dim, i, -j, 0} of the path's edges
 	const Op* gops;
+	Op nop;          // op q + 1, loaded one step ahead
 	int q, q1, top;
 	bool top_leaf;
 	double r_top, q_top, cur, base_prev;
@@ -180,6 +181,7 @@ struct DxWalk
 		pend_p = 0.0;
 		P[lane] = 1.0;
 		op = ldc_op(gops + q);
+		nop = ldc_op(gops + q + 1);
 	}
 
 	__device__ __forceinline__ void leave(const u4_t& e, double r, int tt, int lane,
@@ -188,8 +190,8 @@ struct DxWalk
 		const uint32_t d2 = __builtin_amdgcn_readfirstlane(e[0]);
 		const double iv = f64_from_high(__builtin_amdgcn_readfirstlane(e[1]));
 		const double u = fma(xpin<XB>(xv0, xv1, xv2, xv3, d2), iv, f64_from_high(__builtin_amdgcn_readfirstlane(e[2])));
-		DX[(d2 >> 1) * kWave + lane] += r * dphi_over_phi(u, iv);
-		if (tt > 0) R[(tt - 1) * kWave + lane] += r;
+		lds_add(DX + (d2 >> 1) * kWave + lane, r * dphi_over_phi(u, iv));
+		if (tt > 0) lds_add(R + (tt - 1) * kWave + lane, r);
 	}
 
 	// Leaves every edge at depth >= t.
@@ -201,8 +203,8 @@ struct DxWalk
 		{
 			if (wave_any(r_top != 0.0))
 			{
-				DX[dim_top * kWave + lane] += r_top * q_top;
-				if (tt > 0) R[(tt - 1) * kWave + lane] += r_top;
+				lds_add(DX + dim_top * kWave + lane, r_top * q_top);
+				if (tt > 0) lds_add(R + (tt - 1) * kWave + lane, r_top);
 			}
 			tt--;
 		}
@@ -233,8 +235,9 @@ struct DxWalk
 		}
 		while (q < q1)
 		{
-			// the stream ends with kOpPad sentinels, so q + 1 is in bounds
-			const Op nop = ldc_op(gops + q + 1);
+			// the stream ends with kOpPad sentinels, so q + 2 is in bounds;
+			// op q + 2 is loaded now and waited for a step later
+			const Op nnop = ldc_op(gops + q + 2);
 			__builtin_amdgcn_sched_barrier(0);
 			const uint32_t dimw = op.dim, ctrl = op.ctrl;
 			const uint32_t d2 = dimw & kOpDimMask;
@@ -267,11 +270,14 @@ struct DxWalk
 			{
 				const int rs = (int)(dimw >> kOpSkipShift);
 				q += rs;
-				op = rs == 1 ? nop : ldc_op(gops + q);
+				if (rs == 1) { op = nop; nop = nnop; }
+				else if (rs == 2) { op = nnop; nop = ldc_op(gops + q + 1); }
+				else { op = ldc_op(gops + q); nop = ldc_op(gops + q + 1); }
 				continue;
 			}
 			q++;
 			op = nop;
+			nop = nnop;
 			if (top_leaf) q_top = dphi_over_phi(u, iv);
 			if (ctrl & kOpLast)
 			{

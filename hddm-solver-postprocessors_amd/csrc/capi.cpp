@@ -493,9 +493,13 @@ TuneConfig default_config(hddm_lb_ctx* ctx, int prec)
 	if (!slab) slab = xbanks_for(ctx->dim) == 4 && ctx->dt_needed >= 2 ? 2 : 1;
 	if (prec == HDDM_LB_FP64) slab = 1;
 	c.slab = slab;
-	// automatic: wide states (D > 32) of two or more dof tiles are walked once
-	// by a walker wave feeding one consumer wave per tile
-	c.ws = ctx->opt_form == 2 || (ctx->opt_form == 0 && xbanks_for(ctx->dim) == 4 && ctx->dt_needed >= 2);
+	// automatic: FP64 sweeps of wide states (D > 32) of two or more dof tiles
+	// are walked once by a walker wave feeding one consumer wave per tile
+	// (measured: D=59, 128 dofs, 2^18 points: FP64 962 -> 731 ms; the split
+	// and F32 paths are faster with every wave walking its own block, 408 vs
+	// 505 ms for f16x2)
+	c.ws = ctx->opt_form == 2
+		|| (ctx->opt_form == 0 && prec == HDDM_LB_FP64 && xbanks_for(ctx->dim) == 4 && ctx->dt_needed >= 2);
 	c.nc = ctx->dt_needed >= 2 ? 4 : 2;
 	return c;
 }

@@ -13,6 +13,16 @@ __device__ __forceinline__ double f64_from_high(uint32_t hi)
 	return __builtin_bit_cast(double, (uint64_t)hi << 32);
 }
 
+// *p += v on an LDS row owned by the calling wave, as an LDS atomic add
+// whose result is not returned (no wait for it): a wave's LDS operations
+// complete in issue order, so later reads see the sum and the additions
+// happen in program order, as with a read-modify-write.
+__device__ __forceinline__ void lds_add(double* p, double v)
+{
+	(void)__hip_atomic_fetch_add((__attribute__((address_space(3))) double*)p, v, __ATOMIC_RELAXED,
+		__HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // One work item: G neighbouring 64-point blocks of one state (one block per
 // wave of the workgroup) against one node segment of that state.
 struct ItemInfo

@@ -299,7 +299,7 @@ def test_stats_counters(ctx):
             else:
                 assert mfma == 0
         with pytest.raises(hddm_lb.HddmLbError):
-            ctx.last_stats(6)
+            ctx.last_stats(7)
     finally:
         ctx.set_option(hddm_lb.OPT_STATS, 0)
 
@@ -341,7 +341,7 @@ def test_forward_host_small_batches_replay_graphs(ctx):
     for n in (1, 3, 64, 65, 1, 64):
         X = hddm_lb.synthetic_points(n, 9, 100 + n)
         sid = np.zeros(n, np.int32)
-        sid[::3] = 5   # not loaded: row left as passed
+        sid[1::3] = 5   # not loaded: row left as passed
         y = np.full((n, 30), np.nan)
         c.forward_host(X, y, sid, hddm_lb.FP64)
         yo = oracle.forward(ii, jj, S, X)
