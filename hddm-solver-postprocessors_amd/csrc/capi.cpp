@@ -52,9 +52,10 @@ Scratch::~Scratch() { release(); }
 
 void Scratch::release()
 {
-	for (void* p : { (void*)bkey, (void*)perm, (void*)hist, (void*)cursor, (void*)partial, (void*)dspart,
-			(void*)dxpart, (void*)plans, (void*)hdr })
-		dev_free(p);
+	if (owned)
+		for (void* p : { (void*)bkey, (void*)perm, (void*)hist, (void*)cursor, (void*)partial, (void*)dspart,
+				(void*)dxpart, (void*)plans, (void*)hdr })
+			dev_free(p);
 	bkey = perm = hist = cursor = nullptr;
 	partial = dspart = dxpart = nullptr;
 	cap_dspart = cap_dxpart = 0;
@@ -155,6 +156,10 @@ extern "C" int hddm_lb_create(int device, hddm_lb_ctx** out)
 		if (rc) return rc;
 		HIP_TRY(hipMemset(ctx->d_status, 0, sizeof(uint32_t)));
 	}
+	{
+		int rc = noalloc_init(ctx.get());
+		if (rc) return rc;
+	}
 	for (auto& r : ctx->tring)
 		for (int i = 0; i < hddm_lb_ctx::kTimingRing; i++)
 		{
@@ -182,7 +187,9 @@ extern "C" int hddm_lb_set_option(hddm_lb_ctx* ctx, int option, int64_t value)
 	case HDDM_LB_OPT_AUTOTUNE: ctx->opt_autotune = value != 0; break;
 	case HDDM_LB_OPT_SEGMENT_NODES:
 		if (value < 0) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "set_option: segment nodes must be >= 0");
-		ctx->opt_seg_nodes = value; ctx->tables_dirty = true; ctx->generation++; break;
+		ctx->opt_seg_nodes = value; ctx->tables_dirty = true; ctx->generation++;
+		if (ctx->dim > 0) return prepare_tables(ctx);
+		break;
 	case HDDM_LB_OPT_WAVES_PER_WG:
 		if (!(value == 0 || value == 1 || value == 2 || value == 4))
 			return set_error(HDDM_LB_ERROR_INVALID_VALUE, "set_option: waves per workgroup must be 0, 1, 2 or 4");
@@ -274,7 +281,9 @@ extern "C" int hddm_lb_upload_state(hddm_lb_ctx* ctx, int istate, int dim, int n
 	ctx->tables_dirty = true;
 	ctx->generation++;
 	ctx->tuned.clear();
-	return HDDM_LB_SUCCESS;
+	// the descriptor tables are built now (not at the next forward), so that
+	// the allocation-free forward finds them ready
+	return prepare_tables(ctx);
 }
 
 extern "C" int hddm_lb_state_info(hddm_lb_ctx* ctx, int istate, int* dim, int* nno, int* dofs, int* max_factors)

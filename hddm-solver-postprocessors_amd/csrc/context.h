@@ -50,6 +50,7 @@ struct Scratch
 	double* dxpart = nullptr; size_t cap_dxpart = 0;   // node-segment partials of the adjoint's dx
 	StatePlan* plans = nullptr; size_t cap_plans = 0;
 	PlanHeader* hdr = nullptr; size_t cap_hdr = 0;
+	bool owned = true;   // false: the pointers lie in memory the scratch does not own
 	~Scratch();
 	void release();
 };
@@ -124,6 +125,9 @@ struct hddm_lb_ctx
 	static constexpr int kStatCount = 6;
 	unsigned long long* d_stats = nullptr;
 	uint32_t* d_status = nullptr;   // HDDM_LB_STATUS_* bits raised by kernels
+	// device address of this device's copy of the allocation-free entry
+	// point's static scratch array (hipGetSymbolAddress at create)
+	char* noalloc_scratch = nullptr;
 	// wave-specialized adjoint in statistics mode: walker cycles in walk A,
 	// walk B, waiting for w, waiting for ring slots; consumer cycles in w, in
 	// dS, idle; items; chunks
@@ -152,6 +156,8 @@ struct hddm_lb_ctx
 namespace hddm_lb {
 
 int prepare_tables(hddm_lb_ctx* ctx);
+// Looks up this device's copy of the allocation-free scratch array (noalloc.hip).
+int noalloc_init(hddm_lb_ctx* ctx);
 // Orders a call on st after the last eager call that used ctx->scratch on another stream.
 int scratch_wait(hddm_lb_ctx* ctx, hipStream_t st);
 // Marks the end of an eager call's use of ctx->scratch on st.
@@ -163,5 +169,7 @@ int enqueue_forward(hddm_lb_ctx* ctx, Scratch& sc, int64_t npoints, const double
 int autotune(hddm_lb_ctx* ctx, int64_t npoints, const double* x, int64_t ldx, const int32_t* sid,
 	double* y, int64_t ldy, int prec, hipStream_t st, TuneConfig& cfg);
 int bucket_bits(int64_t npoints, int nstates, int sort);
+int check_forward_args(hddm_lb_ctx* ctx, int64_t npoints, const double* x, int64_t ldx,
+	double* y, int64_t ldy, int prec);
 
 } // namespace hddm_lb

@@ -77,6 +77,7 @@ _SIGS = {
     "hddm_lb_state_info": (_i32, [_vp, _i32, _c.POINTER(_i32), _c.POINTER(_i32), _c.POINTER(_i32), _c.POINTER(_i32)]),
     "hddm_lb_forward": (_i32, [_vp, _i64, _vp, _i64, _vp, _vp, _i64, _i32, _vp]),
     "hddm_lb_forward_host": (_i32, [_vp, _i64, _vp, _i64, _vp, _vp, _i64, _i32]),
+    "hddm_lb_forward_noalloc": (_i32, [_vp, _i64, _vp, _i64, _vp, _vp, _i64, _i32, _vp]),
     "hddm_lb_adjoint": (_i32, [_vp, _i64, _vp, _i64, _vp, _vp, _i64, _vp, _i64, _vp, _i32, _vp]),
     "hddm_lb_surplus_offset": (_i32, [_vp, _i32, _c.POINTER(_i64), _c.POINTER(_i64)]),
     "hddm_lb_plan_create": (_i32, [_vp, _i64, _vp, _i64, _vp, _vp, _i64, _i32, _vp, _c.POINTER(_vp)]),

@@ -115,6 +115,21 @@ int hddm_lb_forward(hddm_lb_ctx* ctx, int64_t npoints,
 	const double* x, int64_t ldx, const int32_t* state_ids,
 	double* y, int64_t ldy, int precision, void* stream);
 
+/* hddm_lb_forward without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under hipGraph
+ * capture, serving loops, harnesses that forbid hipMalloc). Same arguments,
+ * results and stream semantics as hddm_lb_forward with the default sweep
+ * configuration (no autotuning). All per-call scratch comes from one static
+ * device array of 64 MiB per device, shared by every context on that device:
+ * calls that overlap on one device race. A batch whose scratch (8 bytes per
+ * point, plus 8 * nseg * dofs_pad bytes per point for states split into node
+ * segments) exceeds 64 MiB is rejected with HDDM_LB_ERROR_INVALID_VALUE;
+ * statistics mode is rejected; the state tables must be ready (they are after
+ * every upload and option change). */
+int hddm_lb_forward_noalloc(hddm_lb_ctx* ctx, int64_t npoints,
+	const double* x, int64_t ldx, const int32_t* state_ids,
+	double* y, int64_t ldy, int precision, void* stream);
+
 /* Same contract with host pointers; synchronous. This is what the legacy
  * per-point Interpolator::interpolate maps onto. */
 int hddm_lb_forward_host(hddm_lb_ctx* ctx, int64_t npoints,
