@@ -130,7 +130,9 @@ def headline(args):
         def step(pf=pf, pa=pa):
             ctx.forward(x, y, sd, pf)
             ctx.adjoint(x, g, dx, ds, sd, pa)
-        steps = args.steps if name == "mixed" else max(2, args.steps // 5)
+        # the plain FP64 leg is the slower one: it runs a fifth of the steps
+        # (at least 2) after one warm-up step, and says so in its line
+        steps = args.steps if name == "mixed" else max(2, args.steps // 10)
         warm = args.warmup if name == "mixed" else 1
         ms, kern = timed_steps(step, steps, warm, ctx)
         cfg = ctx.last_config()
