@@ -5,8 +5,10 @@ Headline workload: high-dimensional multistate forward + adjoint.
   4 states, D=59, regular sparse grids sum(l-1) <= 3 (281,077 nodes each),
   dofs=128, 1,048,576 points x ~ U[0,1)^59 with seeded state ids, upstream
   gradient N(0, 1); everything resident in HBM before timing.
-A step is one forward (y) and one adjoint (dx and dsurplus) over the whole
-batch. The mixed step runs the forward on f16 MFMA with split hi/lo operands
+A step is one forward (y) and one adjoint (dx and dsurplus) over a batch of
+262,144 of those points (all four states; the workload's 2^20 points are four
+steps); one step over all 2^20 points is timed after the steps as a check
+that the rate does not depend on the batch. The mixed step runs the forward on f16 MFMA with split hi/lo operands
 (HDDM_LB_F16X2) and the adjoint on FP32 MFMA (HDDM_LB_F32); the plain FP64
 step runs both in FP64 with the same work queue, trie walk and zero-skip.
 value = points x dofs / mixed step time (point.dof evaluations per second,
@@ -73,14 +75,18 @@ def d59_states(ctx):
     return states
 
 
-def timed_steps(step, steps, warmup, ctx=None):
+def timed_steps(step, steps, warmup, ctx=None, label=None):
     """Wall ms per step over `steps` calls after `warmup`; with ctx, the mean
     sweep (kind 0) and adjoint (kind 1) kernel ms per step from the library's
-    HIP events on the calls' stream, recorded in the same steps."""
+    HIP events on the calls' stream, recorded in the same steps. With a
+    label, every step is synchronized and reported on stderr (long steps)."""
     import torch
     import hddm_lb
-    for _ in range(warmup):
+    for i in range(warmup):
         step()
+        if label:
+            torch.cuda.synchronize()
+            say("bench: %s warm-up step %d/%d" % (label, i + 1, warmup))
     torch.cuda.synchronize()
     if ctx is not None:
         ctx.set_option(hddm_lb.OPT_TIMING, 1)
@@ -88,8 +94,11 @@ def timed_steps(step, steps, warmup, ctx=None):
         ctx.sweep_time_summary(1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for i in range(steps):
         step()
+        if label:
+            torch.cuda.synchronize()
+            say("bench: %s step %d/%d, %.0f ms so far" % (label, i + 1, steps, (time.perf_counter() - t0) * 1e3))
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / steps
     kern = None
@@ -134,7 +143,7 @@ def headline(args):
         # (at least 2) after one warm-up step, and says so in its line
         steps = args.steps if name == "mixed" else max(2, args.steps // 10)
         warm = args.warmup if name == "mixed" else 1
-        ms, kern = timed_steps(step, steps, warm, ctx)
+        ms, kern = timed_steps(step, steps, warm, ctx, label="D=59 " + name)
         cfg = ctx.last_config()
         status = ctx.device_status()
         res[name] = {"ms_per_step": ms, "steps": steps, "warmup": warm, "kernels": kern, "forward_config": cfg,
@@ -172,6 +181,26 @@ def headline(args):
             off, _ = ctx.surplus_offset(s)
             e = max(e, normwise_err(D[off:off + ii.shape[0] * 128].reshape(-1, 128), dso))
         res[name]["max_rel_err"]["dsurplus"] = e
+    # one mixed step over all 2^20 points of the workload (rate check)
+    full = None
+    if args.full_check and n < (1 << 20):
+        nf = 1 << 20
+        Xf = hddm_lb.synthetic_points(nf, 59, 78)
+        sf = np.random.default_rng(2025).integers(0, 4, size=nf).astype(np.int32)
+        xf = torch.from_numpy(Xf).cuda()
+        sdf = torch.from_numpy(sf).cuda()
+        gf = torch.from_numpy(np.random.default_rng(96).standard_normal((nf, 128))).cuda()
+        yf = torch.empty((nf, 128), dtype=torch.float64, device="cuda")
+        dxf = torch.empty((nf, 59), dtype=torch.float64, device="cuda")
+        pf, pa = legs["mixed"]
+
+        def fstep():
+            ctx.forward(xf, yf, sdf, pf)
+            ctx.adjoint(xf, gf, dxf, ds, sdf, pa)
+        fms, _ = timed_steps(fstep, 1, 0, None, label="D=59 mixed, all 2^20 points")
+        full = {"points": nf, "ms_per_step": fms, "value": nf * 128 / (fms * 1e-3)}
+        del xf, sdf, gf, yf, dxf
+    res["full_batch_check"] = full
     # executed work: nnz pairs and contraction chunks of one forward on each
     # formulation's statistics pass (outside the timed steps)
     ctx.set_option(hddm_lb.OPT_STATS, 1)
@@ -401,7 +430,7 @@ def cpu_regular_d4():
             "x86_port_one_thread": 4096 * 8 / dtx, "gold_one_thread": 4096 * 8 / dtg, "unit": "point·dof/s"}
 
 
-def roofline_of(head, work):
+def roofline_of(head, work, npoints):
     """Roofline of the dominant kernel of the mixed step, from algorithmic
     FLOPs (non-zero (point, node) pairs only) over its HIP-event duration."""
     mixed = head["mixed"]
@@ -430,7 +459,7 @@ def roofline_of(head, work):
             "traffic": traffic, "kernel": dominant, "kernel_ms": t, "flops_per_launch": flops,
             "flops_basis": "non-zero (point, node) pairs: %d; per pair and dof 2 FLOP (forward), 4 (adjoint)" % int(nnz),
             "frac_executed": exec_flops / (adj_ms * 1e-3) / 1e12 / PEAK_TFLOPS["f32"],
-            "flops_dense_per_launch": 4.0 * work["nno"] * dofs * 1048576,
+            "flops_dense_per_launch": 4.0 * work["nno"] * dofs * npoints,
             "note": "frac: algorithmic FLOPs of the non-zero pairs only; frac_executed: the MFMA work the "
                     "adjoint kernel issues on its 32-node chunks (a 64-point block contracts every node "
                     "that is non-zero at any of its points)"}
@@ -441,7 +470,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--points", type=int, default=1 << 20)
+    ap.add_argument("--points", type=int, default=1 << 18,
+                    help="points per step of the headline (all four states)")
+    ap.add_argument("--no-full-check", dest="full_check", action="store_false",
+                    help="skip the one step over all 2^20 points")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-secondary", action="store_true", help="skip the secondary workloads")
@@ -485,12 +517,13 @@ def main():
         "dtype": "f16",
         "data": "synthetic (seeded regular grids, surpluses U[-1,1]*2^-sum(l), points U[0,1)^59, gbar N(0,1))",
         "config": {"workload": "multistate forward + adjoint, 4 states, D=59, regular grids sum(l-1) <= 3 "
-                               "(%d nodes each), dofs=128, %d points" % (work["nno"], n),
+                               "(%d nodes each), dofs=128, 2^20 points in steps of %d" % (work["nno"], n),
                    "states": 4, "dim": 59, "nno_per_state": work["nno"], "dofs": 128, "points": n,
+                   "points_total": 1 << 20,
                    "forward_precision": "f16x2 (f16 hi/lo MFMA, FP32 accumulate, FP64 fold per chunk)",
                    "adjoint_precision": "f32 (FP32 MFMA; w summed and carried in FP64)",
                    "forward_config": head["mixed"]["forward_config"]},
-        "roofline": roofline_of(head, work),
+        "roofline": roofline_of(head, work, n),
         "fp64_plain": {"value": n * 128 / (fp64_ms * 1e-3), "ms_per_step": fp64_ms, "steps": head["fp64"]["steps"],
                        "kernels": head["fp64"]["kernels"], "forward_config": head["fp64"]["forward_config"],
                        "max_rel_err": head["fp64"]["max_rel_err"]},
@@ -498,6 +531,7 @@ def main():
         "mixed_kernels": head["mixed"]["kernels"],
         "max_rel_err": head["mixed"]["max_rel_err"],
         "executed": work,
+        "full_batch_check": head["full_batch_check"],
     }
     say("bench: headline done")
     if not args.no_secondary:

@@ -129,6 +129,6 @@ class X86State:
         return y
 
     def __del__(self):
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and lib is not None:
             lib.oracle_x86_free(self._h)
             self._h = None
