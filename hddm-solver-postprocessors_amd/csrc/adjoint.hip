@@ -504,8 +504,6 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 	if (precision != HDDM_LB_FP64 && precision != HDDM_LB_F32)
 		return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: precision %d is not available; use HDDM_LB_FP64 or HDDM_LB_F32",
 			precision);
-	if (precision == HDDM_LB_F32 && ctx->opt_form == 1)
-		return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: HDDM_LB_F32 runs on the wave-specialized kernel only");
 	if (npoints < 0 || npoints > ((int64_t)1 << 31) - 1)
 		return set_error(HDDM_LB_ERROR_INVALID_VALUE, "adjoint: npoints %lld out of range", (long long)npoints);
 	if (ctx->dim < 1) return set_error(HDDM_LB_ERROR_STATE, "adjoint: no state is loaded");
@@ -577,13 +575,15 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 	a.nstates = ns; a.dim = ctx->dim;
 	a.status = ctx->d_status;
 
-	// automatic: the wave-specialized kernel for wide states (D > 32) of two or
-	// more dof tiles, which it walks once instead of once per tile (measured:
-	// D=59, 128 dofs, 2^16 points: FP64 1255 -> 838 ms); the per-tile kernels
-	// otherwise (D=20, 32 dofs: 32 vs 110 ms). F32 runs on the wave-specialized
-	// kernel only.
-	const bool ws = precision == HDDM_LB_F32 || ctx->opt_form == 2
-		|| (ctx->opt_form == 0 && xbanks_for(ctx->dim) == 4 && ctx->dt_needed >= 2);
+	// automatic: the wave-specialized kernel for wide states (D > 32), which
+	// walks each trie once instead of once per dof tile (measured: D=59, 128
+	// dofs, 2^16 points: FP64 1255 -> 838 ms; its F32 consumers take dS to the
+	// MFMA units); the per-tile kernels otherwise (D=20, 32 dofs: FP64 32 ms
+	// against 110 ms wave-specialized, F32 105 ms). The per-tile kernels compute
+	// in FP64 whatever the precision asked for, which meets the F32 bound too.
+	const bool wide = xbanks_for(ctx->dim) == 4;
+	const bool ws = ctx->opt_form == 2
+		|| (ctx->opt_form == 0 && wide && (precision == HDDM_LB_F32 || ctx->dt_needed >= 2));
 	if (ws && ctx->opt_stats)
 	{
 		if (!ctx->d_adj_stats)

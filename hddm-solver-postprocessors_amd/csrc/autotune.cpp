@@ -6,10 +6,12 @@
 //     or 4 waves per workgroup and 1 or 2 dof tiles per pass (the split
 //     paths; FP64 always takes one);
 //   * the wave-specialized sweep (a walker wave feeding MFMA consumer waves).
-// Candidates are timed on the first min(npoints, kTunePoints) points, which
-// is enough work to fill the device and keeps tuning to a few forwards of a
-// small batch. Those rows of y are written like any forward's and are
-// overwritten by the forward that follows.
+// Candidates are timed on the first min(npoints, kTunePoints) points. The
+// prefix has to be long: the per-block sweep hides its latency with many
+// resident waves, and on a short prefix (2^16 points, four waves per CU) it
+// loses to the wave-specialized sweep that it beats two to one on 2^20
+// points (D=16, FP64). Those rows of y are written like any forward's and
+// are overwritten by the forward that follows.
 #include "context.h"
 #include "kernels.h"
 #include "errors.h"
@@ -19,7 +21,7 @@
 
 namespace hddm_lb {
 
-constexpr int64_t kTunePoints = 65536;
+constexpr int64_t kTunePoints = (int64_t)1 << 20;
 
 int autotune(hddm_lb_ctx* ctx, int64_t npoints, const double* x, int64_t ldx, const int32_t* sid,
 	double* y, int64_t ldy, int prec, hipStream_t st, TuneConfig& cfg)

@@ -204,6 +204,7 @@ extern "C" int hddm_lb_set_option(hddm_lb_ctx* ctx, int option, int64_t value)
 	case HDDM_LB_OPT_WALK:
 		if (value < 0 || value > 2) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "set_option: walk must be 0, 1 or 2");
 		ctx->opt_walk = (int)value; break;
+	case HDDM_LB_OPT_COALESCE: ctx->opt_coalesce = value != 0; break;
 	case HDDM_LB_OPT_FORMULATION:
 		if (value < 0 || value > 2) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "set_option: formulation must be 0, 1 or 2");
 		ctx->opt_form = (int)value; break;
@@ -721,13 +722,12 @@ struct hddm_lb_plan
 	}
 };
 
-extern "C" int hddm_lb_forward_host(hddm_lb_ctx* ctx, int64_t npoints,
+// The staged forward of host buffers; the caller holds ctx->run_mu.
+static int forward_host_run(hddm_lb_ctx* ctx, int64_t npoints,
 	const double* x, int64_t ldx, const int32_t* state_ids,
 	double* y, int64_t ldy, int precision)
 {
-	int rc = check_forward_args(ctx, npoints, x, ldx, y, ldy, precision);
-	if (rc) return rc;
-	if (npoints == 0) return HDDM_LB_SUCCESS;
+	int rc = HDDM_LB_SUCCESS;
 	DeviceGuard g(ctx->device);
 	if (!ctx->own_stream) HIP_TRY(hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking));
 	const bool small = npoints <= kSmallBatch;
@@ -787,6 +787,105 @@ extern "C" int hddm_lb_forward_host(hddm_lb_ctx* ctx, int64_t npoints,
 	HIP_TRY(hipMemcpyAsync(ctx->h_ystage, ctx->d_ystage, sizeof(double) * npoints * ldy, hipMemcpyDeviceToHost, st));
 	HIP_TRY(hipStreamSynchronize(st));
 	std::memcpy(y, ctx->h_ystage, sizeof(double) * npoints * ldy);
+	return HDDM_LB_SUCCESS;
+}
+
+// Requests taken into one coalesced batch, in points.
+constexpr int64_t kCoalesceMax = 4096;
+
+// Runs queued requests of one precision as one forward: rows packed with
+// ldx = dim and ldy = the widest state's dofs (the callers' y rows are packed
+// too, so columns and rows the forward leaves alone come back unchanged).
+static int run_coalesced(hddm_lb_ctx* ctx, const std::vector<HostRequest*>& batch)
+{
+	std::lock_guard<std::mutex> run(ctx->run_mu);
+	if (batch.size() == 1)
+	{
+		const HostRequest* r = batch[0];
+		return forward_host_run(ctx, r->n, r->x, r->ldx, r->sid, r->y, r->ldy, r->prec);
+	}
+	int maxdofs = 0;
+	for (auto& s : ctx->states) if (s && s->loaded) maxdofs = std::max(maxdofs, s->desc.dofs);
+	const int dim = ctx->dim;
+	int64_t tot = 0;
+	for (const HostRequest* r : batch) tot += r->n;
+	std::vector<double> xs((size_t)tot * dim), ys((size_t)tot * maxdofs);
+	std::vector<int32_t> sid((size_t)tot);
+	int64_t row = 0;
+	for (const HostRequest* r : batch)
+	{
+		const int64_t w = std::min<int64_t>(r->ldy, maxdofs);
+		for (int64_t p = 0; p < r->n; p++, row++)
+		{
+			std::memcpy(&xs[(size_t)row * dim], r->x + p * r->ldx, sizeof(double) * dim);
+			std::memcpy(&ys[(size_t)row * maxdofs], r->y + p * r->ldy, sizeof(double) * w);
+			sid[(size_t)row] = r->sid ? r->sid[p] : 0;
+		}
+	}
+	const int rc = forward_host_run(ctx, tot, xs.data(), dim, sid.data(), ys.data(), maxdofs, batch[0]->prec);
+	if (rc) return rc;
+	row = 0;
+	for (const HostRequest* r : batch)
+	{
+		const int64_t w = std::min<int64_t>(r->ldy, maxdofs);
+		for (int64_t p = 0; p < r->n; p++, row++)
+			std::memcpy(r->y + p * r->ldy, &ys[(size_t)row * maxdofs], sizeof(double) * w);
+	}
+	return HDDM_LB_SUCCESS;
+}
+
+extern "C" int hddm_lb_forward_host(hddm_lb_ctx* ctx, int64_t npoints,
+	const double* x, int64_t ldx, const int32_t* state_ids,
+	double* y, int64_t ldy, int precision)
+{
+	int rc = check_forward_args(ctx, npoints, x, ldx, y, ldy, precision);
+	if (rc) return rc;
+	if (npoints == 0) return HDDM_LB_SUCCESS;
+	if (npoints > kSmallBatch || !ctx->opt_coalesce)
+	{
+		std::lock_guard<std::mutex> run(ctx->run_mu);
+		return forward_host_run(ctx, npoints, x, ldx, state_ids, y, ldy, precision);
+	}
+	// small call: queue it; whichever waiting thread finds no batch running
+	// takes every queued request of its precision (up to kCoalesceMax points)
+	// and runs them as one forward
+	HostRequest me{ npoints, x, ldx, state_ids, y, ldy, precision, HDDM_LB_SUCCESS, false };
+	std::unique_lock<std::mutex> lk(ctx->comb_mu);
+	ctx->comb_queue.push_back(&me);
+	while (!me.done)
+	{
+		if (ctx->comb_busy) { ctx->comb_cv.wait(lk); continue; }
+		ctx->comb_busy = true;
+		std::vector<HostRequest*> batch;
+		int64_t tot = 0;
+		for (auto it = ctx->comb_queue.begin(); it != ctx->comb_queue.end();)
+		{
+			if ((*it)->prec == precision && tot + (*it)->n <= kCoalesceMax)
+			{
+				tot += (*it)->n;
+				batch.push_back(*it);
+				it = ctx->comb_queue.erase(it);
+			}
+			else ++it;
+		}
+		lk.unlock();
+		const int brc = batch.empty() ? HDDM_LB_SUCCESS : run_coalesced(ctx, batch);
+		lk.lock();
+		for (HostRequest* r : batch) { r->rc = brc; r->done = true; }
+		ctx->comb_requests += (int64_t)batch.size();
+		ctx->comb_batches += batch.empty() ? 0 : 1;
+		ctx->comb_busy = false;
+		ctx->comb_cv.notify_all();
+	}
+	return me.rc;
+}
+
+extern "C" int hddm_lb_host_call_stats(hddm_lb_ctx* ctx, int64_t* requests, int64_t* batches)
+{
+	if (!ctx || !requests || !batches) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "host_call_stats: NULL argument");
+	std::lock_guard<std::mutex> lk(ctx->comb_mu);
+	*requests = ctx->comb_requests;
+	*batches = ctx->comb_batches;
 	return HDDM_LB_SUCCESS;
 }
 

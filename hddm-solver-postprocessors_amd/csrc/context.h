@@ -3,7 +3,9 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <condition_variable>
 #include <map>
+#include <mutex>
 #include <memory>
 #include <tuple>
 #include <vector>
@@ -53,6 +55,20 @@ struct Scratch
 	bool owned = true;   // false: the pointers lie in memory the scratch does not own
 	~Scratch();
 	void release();
+};
+
+// One host-pointer forward waiting to be coalesced with others (hddm_lb_forward_host).
+struct HostRequest
+{
+	int64_t n;
+	const double* x;
+	int64_t ldx;
+	const int32_t* sid;
+	double* y;
+	int64_t ldy;
+	int prec;
+	int rc;
+	bool done;
 };
 
 // Launch shape of the sweep, chosen by default or by the autotuner.
@@ -110,6 +126,15 @@ struct hddm_lb_ctx
 	double* h_ystage = nullptr; size_t cap_hy = 0;
 	int32_t* h_sidstage = nullptr; size_t cap_hsid = 0;
 	std::map<std::tuple<int64_t, int64_t, int64_t, int, int>, hddm_lb_plan*> small_plans;
+	// host-pointer calls from several threads: run_mu serializes the staging
+	// and launch; small calls queue in comb_queue and one of the waiting
+	// threads runs them as one batched forward
+	std::mutex run_mu, comb_mu;
+	std::condition_variable comb_cv;
+	std::vector<hddm_lb::HostRequest*> comb_queue;
+	bool comb_busy = false;
+	bool opt_coalesce = true;
+	int64_t comb_requests = 0, comb_batches = 0;
 	// Eager calls share `scratch`: each one records scratch_event on its
 	// stream, and the next call on another stream waits for it first.
 	hipEvent_t scratch_event = nullptr;

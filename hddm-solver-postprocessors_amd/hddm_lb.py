@@ -26,7 +26,7 @@ FP64, F16X2, BF16X3, F32 = 0, 1, 2, 3
 PRECISIONS = {"fp64": FP64, "f16x2": F16X2, "bf16x3": BF16X3, "f32": F32}
 
 OPT_AUTOTUNE, OPT_SEGMENT_NODES, OPT_WAVES_PER_WG, OPT_SORT_POINTS = 1, 2, 3, 4
-OPT_TIMING, OPT_SLAB_TILES, OPT_STATS, OPT_WALK, OPT_FORMULATION = 5, 6, 8, 9, 10
+OPT_TIMING, OPT_SLAB_TILES, OPT_STATS, OPT_WALK, OPT_FORMULATION, OPT_COALESCE = 5, 6, 8, 9, 10, 11
 STATUS_WAIT_LIMIT = 1
 
 SUCCESS = 0
@@ -86,6 +86,7 @@ _SIGS = {
     "hddm_lb_last_sweep_ms": (_dbl, [_vp]),
     "hddm_lb_last_stats": (_i32, [_vp, _vp]),
     "hddm_lb_last_config": (_i32, [_vp, _vp]),
+    "hddm_lb_host_call_stats": (_i32, [_vp, _c.POINTER(_i64), _c.POINTER(_i64)]),
     "hddm_lb_last_adjoint_stats": (_i32, [_vp, _vp, _i32]),
     "hddm_lb_last_stats_n": (_i32, [_vp, _vp, _i32]),
     "hddm_lb_sweep_time_summary": (_i32, [_vp, _vp, _vp, _vp, _vp]),
@@ -347,6 +348,12 @@ class Context:
         st = np.zeros(n, np.uint64)
         _check(lib.hddm_lb_last_adjoint_stats(self._h, _ptr(st), n))
         return tuple(int(v) for v in st)
+
+    def host_call_stats(self):
+        """(requests, batches) of host-pointer calls served by coalescing."""
+        r, b = _i64(), _i64()
+        _check(lib.hddm_lb_host_call_stats(self._h, _c.byref(r), _c.byref(b)))
+        return r.value, b.value
 
     def last_config(self):
         """Sweep configuration of the last forward: dict(formulation, waves, tiles, tuned, cache_hits)."""

@@ -65,8 +65,11 @@ typedef struct hddm_lb_grid hddm_lb_grid;
 /* option 7 is unused: MFMA partial sums are folded into FP64 after every chunk */
 #define HDDM_LB_OPT_STATS         8  /* 1: count walk ops, prunes, contracted chunks and items (hddm_lb_last_stats) */
 #define HDDM_LB_OPT_WALK          9  /* op stream source: 0 automatic, 1 scalar loads per wave, 2 LDS tiles per workgroup */
-#define HDDM_LB_OPT_FORMULATION  10  /* sweep: 0 automatic, 1 each wave walks and contracts its block,
-                                        2 a walker wave feeds MFMA consumer waves (one per 32-dof tile) */
+#define HDDM_LB_OPT_FORMULATION  10  /* forward and adjoint: 0 automatic, 1 each wave walks and contracts
+                                        its block (per-tile kernels), 2 a walker wave feeds MFMA consumer
+                                        waves (wave-specialized kernels) */
+#define HDDM_LB_OPT_COALESCE     11  /* 1 (default): hddm_lb_forward_host calls of at most 64 points made
+                                        at the same time from several threads run as one batched forward */
 
 /* bits of the device status word (hddm_lb_device_status) */
 #define HDDM_LB_STATUS_WAIT_LIMIT 1u /* a wave of a wave-specialized kernel stopped waiting for another
@@ -131,7 +134,11 @@ int hddm_lb_forward_noalloc(hddm_lb_ctx* ctx, int64_t npoints,
 	double* y, int64_t ldy, int precision, void* stream);
 
 /* Same contract with host pointers; synchronous. This is what the legacy
- * per-point Interpolator::interpolate maps onto. */
+ * per-point Interpolator::interpolate maps onto. Thread-safe on one context:
+ * x and y go through pinned staging buffers (one DMA each way); a call of at
+ * most 64 points replays a captured forward (one graph launch), and calls of
+ * at most 64 points made concurrently from several threads are queued and
+ * run as one batched forward (HDDM_LB_OPT_COALESCE). */
 int hddm_lb_forward_host(hddm_lb_ctx* ctx, int64_t npoints,
 	const double* x, int64_t ldx, const int32_t* state_ids,
 	double* y, int64_t ldy, int precision);
@@ -146,7 +153,10 @@ int hddm_lb_forward_host(hddm_lb_ctx* ctx, int64_t npoints,
  * rows whose state id is valid); either may be NULL. The basis is recomputed.
  * dx is summed in a fixed order (bitwise repeatable); dsurplus is summed with
  * FP64 atomics within each XCD, so its last bits may vary between runs.
- * precision: HDDM_LB_FP64. Stream-ordered. */
+ * precision: HDDM_LB_FP64, or HDDM_LB_F32 (held to 1e-6 normwise: on the
+ * wave-specialized kernel the products g.S and B^T g run in F32 on the MFMA
+ * units with FP64 accumulation; the per-tile kernels compute in FP64 for
+ * either precision). Stream-ordered. */
 int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 	const double* x, int64_t ldx, const int32_t* state_ids,
 	const double* gbar, int64_t ldg, double* dx, int64_t lddx,
@@ -197,6 +207,10 @@ int hddm_lb_last_stats_n(hddm_lb_ctx* ctx, uint64_t* stats, int n);
  * [5] consumers computing dS, [6] consumers idle, [7] work items, [8] chunks.
  * The first n (1 to 9). Waits for the device. */
 int hddm_lb_last_adjoint_stats(hddm_lb_ctx* ctx, uint64_t* stats, int n);
+
+/* Host-pointer calls served by coalescing (requests) and the batched
+ * forwards that served them (batches), since the context was created. */
+int hddm_lb_host_call_stats(hddm_lb_ctx* ctx, int64_t* requests, int64_t* batches);
 
 /* Sweep configuration of the last hddm_lb_forward: config[0] formulation
  * (1 each wave walks and contracts, 2 wave-specialized), [1] waves per

@@ -1,9 +1,10 @@
 """GPU parity of the adjoint (dx, dsurplus) through the C ABI.
 
-Two kernels compute it: the wave-specialized one (default; a walker wave and
-MFMA consumer waves, FP64 or F32) and the per-tile pair of FP64 kernels
-(HDDM_LB_OPT_FORMULATION = 1). F32 is held to 1e-6 normwise, FP64 to the
-bounds below. Every call must leave the device status word clear.
+Two kernels compute it: the wave-specialized one (a walker wave and MFMA
+consumer waves, FP64 or F32; HDDM_LB_OPT_FORMULATION = 2) and the per-tile
+pair of FP64 kernels (= 1, FP64 arithmetic for either precision). F32 is held
+to 1e-6 normwise, FP64 to the bounds below. Every call must leave the device
+status word clear.
 
 Checks, in FP64:
   * against the oracle's analytic adjoint (gold basis, the product of the
@@ -39,8 +40,9 @@ def say(*a):
 
 # (dx, dS) normwise bounds per precision
 TOL_ADJ = {hddm_lb.FP64: (1e-10, 1e-12), hddm_lb.F32: (1e-6, 1e-6)}
-# (precision, formulation): FP64 on both kernels, F32 on the wave-specialized one
-ADJ_PATHS = [(hddm_lb.FP64, 2), (hddm_lb.FP64, 1), (hddm_lb.F32, 2)]
+# (precision, formulation): FP64 on both kernels, F32 on the wave-specialized
+# one and on the automatic choice
+ADJ_PATHS = [(hddm_lb.FP64, 2), (hddm_lb.FP64, 1), (hddm_lb.F32, 2), (hddm_lb.F32, 0)]
 
 
 def run_adjoint(torch, ctx, X, G, sid=None, prec=hddm_lb.FP64):

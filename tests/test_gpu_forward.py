@@ -378,3 +378,57 @@ def test_autotuner_picks_and_caches(ctx):
     assert cfg["formulation"] in (1, 2)
     assert c.device_status() == 0
     c.close()
+
+
+def test_forward_host_threads_coalesce(ctx):
+    """Eight host threads each make 60 one-point calls on one context (two
+    states of different dofs, some invalid ids, a padded y): every result
+    matches the oracle and invalid rows stay as passed; with coalescing on,
+    the calls are served by fewer batched forwards than calls, and with it off
+    they still give the same results."""
+    import threading
+    ii, jj, lv, _ = hddm_lb.generate_grid(7, 2, 5, 0.4, 81, 2000)
+    S0 = hddm_lb.synthetic_surplus(lv, 12, 82)
+    S1 = hddm_lb.synthetic_surplus(lv, 20, 83)
+    c = hddm_lb.Context(0)
+    c.upload_state(0, ii, jj, S0)
+    c.upload_state(1, ii, jj, S1)
+    nthreads, calls = 8, 60
+    X = hddm_lb.synthetic_points(nthreads * calls, 7, 84)
+    sid = np.random.default_rng(85).integers(-1, 2, size=X.shape[0]).astype(np.int32)
+    ref = np.full((X.shape[0], 24), np.nan)
+    for s, S in enumerate((S0, S1)):
+        rows = sid == s
+        ref[rows, :S.shape[1]] = oracle.forward(ii, jj, S, X[rows])
+    for coalesce in (1, 0):
+        c.set_option(hddm_lb.OPT_COALESCE, coalesce)
+        out = np.full((X.shape[0], 24), np.nan)
+        errors = []
+
+        def worker(t):
+            try:
+                for k in range(calls):
+                    p = t * calls + k
+                    y = np.full((1, 24), np.nan)
+                    c.forward_host(X[p:p + 1], y, sid[p:p + 1], hddm_lb.FP64)
+                    out[p] = y[0]
+            except Exception as e:  # reported below
+                errors.append(e)
+        before = c.host_call_stats()
+        th = [threading.Thread(target=worker, args=(t,)) for t in range(nthreads)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        after = c.host_call_stats()
+        assert not errors, errors
+        ok = ~np.isnan(ref)
+        assert np.array_equal(np.isnan(out), np.isnan(ref))
+        assert np.abs(out[ok] - ref[ok]).max() <= 1e-12 * np.abs(ref[ok]).max()
+        reqs, batches = after[0] - before[0], after[1] - before[1]
+        say("coalesce", coalesce, "requests", reqs, "batches", batches)
+        if coalesce:
+            assert reqs == nthreads * calls and 0 < batches <= reqs
+        else:
+            assert reqs == 0
+    c.close()
