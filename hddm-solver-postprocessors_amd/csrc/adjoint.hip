@@ -582,8 +582,14 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 	// against 110 ms wave-specialized, F32 105 ms). The per-tile kernels compute
 	// in FP64 whatever the precision asked for, which meets the F32 bound too.
 	const bool wide = xbanks_for(ctx->dim) == 4;
+	// nodes of more than 8 factors next to four coordinate banks exceed the
+	// wave-specialized kernel's registers
+	const bool ws_fits = !(wide && ctx->fmax_all > 8);
+	if (ctx->opt_form == 2 && !ws_fits)
+		return set_error(HDDM_LB_ERROR_UNSUPPORTED,
+			"adjoint: the wave-specialized kernel takes nodes of at most 8 factors when dim > 32");
 	const bool ws = ctx->opt_form == 2
-		|| (ctx->opt_form == 0 && wide && (precision == HDDM_LB_F32 || ctx->dt_needed >= 2));
+		|| (ctx->opt_form == 0 && wide && ws_fits && (precision == HDDM_LB_F32 || ctx->dt_needed >= 2));
 	if (ws && ctx->opt_stats)
 	{
 		if (!ctx->d_adj_stats)
@@ -597,8 +603,8 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 	}
 	if (ws)
 	{
-		// one walker wave and four consumer waves per workgroup, four dof
-		// tiles per launch
+		// one walker wave, four consumer waves and one dx wave per
+		// workgroup, four dof tiles per launch
 		const void* kern = adjoint_ws_kernel_ptr(precision, xbanks_for(ctx->dim), ctx->fmax_all);
 		if (!kern) return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: dim %d is not supported", ctx->dim);
 		const size_t lds = adjoint_ws_lds_bytes(precision, ctx->fmax_all, ctx->dim);
@@ -606,7 +612,7 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 			return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: LDS of one workgroup (%zu B) exceeds the CU", lds);
 		HIP_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
 		int per_cu = 0;
-		HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 5 * kWave, lds));
+		HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 6 * kWave, lds));
 		if (per_cu < 1) return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: kernel does not fit a CU");
 		HIP_TRY(launch_fill_u32(sc.hist, nb, 0u, st));
 		HIP_TRY(launch_bucket_key(npoints, x, ldx, state_ids, ctx->d_state_ok, ns, ctx->dim, bb, sc.bkey, sc.hist, st));
@@ -622,7 +628,7 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 			a.dt_base = 4 * ps;
 			if (ps > 0) HIP_TRY(launch_reset_counter(sc.hdr, st));
 			void* params[] = { (void*)&a };
-			HIP_TRY(hipLaunchKernel(kern, dim3(per_cu * ctx->num_cus), dim3(5 * kWave), params, lds, st));
+			HIP_TRY(hipLaunchKernel(kern, dim3(per_cu * ctx->num_cus), dim3(6 * kWave), params, lds, st));
 		}
 		if (ctx->opt_timing)
 		{

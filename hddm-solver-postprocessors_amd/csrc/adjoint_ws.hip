@@ -1,36 +1,36 @@
-// Wave-specialized adjoint (gfx950): one walker wave and NC = 4 consumer waves
-// per workgroup. No counterpart in the reference.
+// Wave-specialized adjoint (gfx950): one walker wave, NC = 4 consumer waves
+// and one dx wave per workgroup. No counterpart in the reference.
 //
 // For y_p = sum_k B_k(x_p) S_k and an upstream gradient g_p:
 //   dS_k  = sum_p B_k(x_p) g_p                    (dsurplus)
 //   dx_pd = sum_k w_pk dB_k/dx_pd,  w_pk = g_p . S_k (dx)
 // Work items are the forward's: one 64-point block of one state against one
-// node segment. The walker wave runs two walks of the segment's trie, one
-// node chunk apart:
-//   A  the forward's assembly walk (walk.h): surviving nodes' basis values go
-//      into chunks of an LDS ring (32 nodes in FP32, 16 in FP64) that are
-//      published to the consumers;
-//   B  the edge-sum walk of the dx kernel (adjoint.hip): every trie edge e
-//      keeps R_e = sum of w_k B_k over the surviving nodes below it, and
-//      leaving e adds R_e phi'_e / phi_e to dx[dim(e)] and R_e to its parent.
-//      Walk B reaches the nodes of chunk c only after the consumers have
-//      computed w for chunk c, which they do as soon as walk A publishes it.
-// Both walks evaluate the same FP64 operations in the same order (fma,
-// 1 - |u|, max, product, all-zero test), so they survive and prune the same
-// nodes.
+// node segment. The walker wave runs the forward's assembly walk (walk.h):
+// surviving nodes' basis values go into chunks of an LDS ring (32 nodes in
+// FP32, 16 in FP64) that are published to the other waves.
 // Consumer c takes one 32-dof tile of the surplus (or, when a state has at
 // most two tiles in the pass, one of the two products of a tile) and, per
 // published chunk, on MFMA:
 //   dS   D[node][dof] = sum over the block's points of B[point][node] g[point][dof],
 //        added to one copy of dS per XCD with atomics (copies summed after);
 //   w    W[point][node] = sum over the tile's dofs of g[point][dof] S[node][dof],
-//        written to an LDS buffer of its own; the walker sums the tiles'
-//        buffers in consumer order (so dx is bitwise repeatable).
+//        written to an LDS buffer of its own (two buffers, by chunk parity).
+// The dx wave takes each chunk once every consumer is done with it. For each
+// surviving node k of the chunk it sums the tiles' w in consumer order, reads
+// k's factors phi_f (dim d_f) from the per-node factor table, recomputes them
+// at the block's points with the walk's own FP64 operations, and adds
+//   w_pk (prod_{g != f} phi_g) phi_f'       (prefix and suffix products)
+// to dx[d_f] in LDS rows of its own: no division, and a factor that is zero
+// contributes exactly zero. One wave adds in a fixed order, so dx is bitwise
+// repeatable; it is written out when the next item (or the exit) arrives.
 // FP64: v_mfma_f64_16x16x4f64 throughout. F32: v_mfma_f32_32x32x2f32 with
 // FP32 operands; dS partials of one chunk (64 points) are added in FP64, w is
-// summed over tiles and carried in FP64 by walk B.
+// summed over tiles in FP64.
 // Synchronization is the forward WS sweep's: an LDS message ring with
-// release/acquire counters and bounded waits (sweep_ws.hip).
+// release/acquire counters and bounded waits (sweep_ws.hip). The walker
+// reuses a ring slot once all five reader waves are done with it; a consumer
+// rewrites its w buffer of one parity only after the dx wave is done with the
+// chunk that last used it.
 #include "sweep_common.h"
 #include "kernels.h"
 #include "walk.h"
@@ -50,7 +50,7 @@ enum : int { kRoleDs = 1, kRoleW = 2 };
 struct AMsg
 {
 	int type, a, b, np;
-	int64_t start, lpos;
+	int64_t start, lpos, count;
 };
 
 struct ASync
@@ -66,12 +66,12 @@ template <int PREC> using adj_t = typename std::conditional<PREC == HDDM_LB_FP64
 template <int PREC> constexpr size_t adj_buf_bytes() { return ((size_t)1 << adj_chunk_bits<PREC>()) * kARow * sizeof(adj_t<PREC>); }
 
 // LDS layout of the workgroup (bytes): kARing chunk slots, 2 x kANC w buffers,
-// messages, counters, walk A's path stack [PM][64], walk B's P and R stacks
-// [PM][64], its edge words [PM], dx rows [dim rounded up to even][64].
+// messages, counters, the walk's path stack [PM][64], the dx wave's rows
+// [dim rounded up to even][64].
 template <int PREC>
 struct ALayout
 {
-	size_t slots, wbuf, msgs, sync, pa, pb, rb, edges, dx, total;
+	size_t slots, wbuf, msgs, sync, pa, dx, total;
 	__host__ __device__ ALayout(int pm, int dim)
 	{
 		slots = 0;
@@ -79,10 +79,7 @@ struct ALayout
 		msgs = wbuf + 2 * kANC * adj_buf_bytes<PREC>();
 		sync = msgs + kARing * sizeof(AMsg);
 		pa = sync + 64;
-		pb = pa + (size_t)pm * kWave * 8;
-		rb = pb + (size_t)pm * kWave * 8;
-		edges = rb + (size_t)pm * kWave * 8;
-		dx = edges + (size_t)pm * 16;
+		dx = pa + (size_t)pm * kWave * 8;
 		total = dx + (size_t)((dim + 1) & ~1) * kWave * 8;
 	}
 };
@@ -133,167 +130,109 @@ __device__ __forceinline__ double xpin(const d16_t& xv0, const d16_t& xv1, const
 	return v;
 }
 
-// phi'/phi of a factor: -i sign(u) / phi inside the support (u = x i - j,
-// phi = 1 - |u|), 0 outside and at the peak; the hardware reciprocal refined
-// by two Newton steps.
-__device__ __forceinline__ double dphi_over_phi(double u, double iv)
+// Four coordinates at once (twice the dims in i0..i3), m0 saved and
+// restored once.
+template <int XB>
+__device__ __forceinline__ void xpin4(const d16_t& xv0, const d16_t& xv1, const d16_t& xv2, const d16_t& xv3,
+	uint32_t i0, uint32_t i1, uint32_t i2, uint32_t i3, double& x0, double& x1, double& x2, double& x3)
 {
-	const double phi = 1.0 - fabs(u);
-	const double ph = phi > 0.0 ? phi : 1.0;
-	double rc = __builtin_amdgcn_rcp(ph);
-	rc = fma(rc, fma(-ph, rc, 1.0), rc);
-	rc = fma(rc, fma(-ph, rc, 1.0), rc);
-	return phi > 0.0 ? (u > 0.0 ? -iv : (u < 0.0 ? iv : 0.0)) * rc : 0.0;
+	uint32_t sv;
+#define HDDM_XP1(k) "s_set_gpr_idx_on %[i" #k "], gpr_idx(SRC0)\n" "v_mov_b64 %[v" #k "], v[64:65]\n" "s_set_gpr_idx_off\n"
+#define HDDM_XPIN4 "s_mov_b32 %[sv], m0\n" HDDM_XP1(0) HDDM_XP1(1) HDDM_XP1(2) HDDM_XP1(3) "s_mov_b32 m0, %[sv]\n"
+#define HDDM_XPOUT [v0] "=&v"(x0), [v1] "=&v"(x1), [v2] "=&v"(x2), [v3] "=&v"(x3), [sv] "=&s"(sv)
+#define HDDM_XPIDX [i0] "s"(i0), [i1] "s"(i1), [i2] "s"(i2), [i3] "s"(i3)
+	if constexpr (XB == 1)
+		asm volatile(HDDM_XPIN4 : HDDM_XPOUT : HDDM_XPIDX, "{v[64:95]}"(xv0));
+	else if constexpr (XB == 2)
+		asm volatile(HDDM_XPIN4 : HDDM_XPOUT : HDDM_XPIDX, "{v[64:95]}"(xv0), "{v[96:127]}"(xv1));
+	else
+		asm volatile(HDDM_XPIN4 : HDDM_XPOUT : HDDM_XPIDX, "{v[64:95]}"(xv0), "{v[96:127]}"(xv1),
+			"{v[128:159]}"(xv2), "{v[160:191]}"(xv3));
+#undef HDDM_XPIDX
+#undef HDDM_XPOUT
+#undef HDDM_XPIN4
+#undef HDDM_XP1
+	(void)xv1; (void)xv2; (void)xv3;
 }
 
-// Walk B: the dx edge-sum walk, paused before emitting a node whose chunk's
-// w is not computed yet.
-template <int XB, int CB>
-struct DxWalk
+// phi' of a factor at u = x i - j: -i sign(u) inside the support (phi =
+// 1 - |u| > 0), 0 outside it and at the peak. Selects only (no branch).
+__device__ __forceinline__ double dphi(double u, double iv, double phi)
 {
-	double* P;       // [PM][64] prefix products
-	double* R;       // [PM][64] edge sums
-	double* DX;      // [dim][64]
-	u4_t* edges;     // [PM] op words {2 dim, i, -j, 0} of the path's edges
-	const Op* gops;
-	Op nop;          // op q + 1, loaded one step ahead
-	int q, q1, top;
-	bool top_leaf;
-	double r_top, q_top, cur, base_prev;
-	uint32_t dim_top;
-	Op op;
-	bool pending;
-	int pend_n, pend_t;
-	double pend_p;
+	const double s = u > 0.0 ? -iv : iv;
+	return ((phi > 0.0) & (u != 0.0)) ? s : 0.0;
+}
 
-	__device__ __forceinline__ void init(int q0, int qend, const Op* ops, int lane)
+// dx terms of the surviving nodes of one chunk at the block's points. The
+// chunk's rows of the factor table (1 << CB nodes x FS slots, FS the state's
+// stride) are read with one vector load per lane and broadcast with
+// v_readlane, so the node loop issues no scalar-memory load. For a node with
+// factors phi_f (dims d_f) and weight w,
+//   w * (prod_{g < f} phi_g) * (prod_{g > f} phi_g) * phi_f'
+// is added to row d_f. The code has no branch: padding slots (i = j = 0)
+// evaluate to phi = 1 and phi' = 0 and add an exact zero to row 0, so two
+// nodes at a time form one block of independent instructions.
+template <int XB, int FS, int CB>
+struct DxNodes
+{
+	static constexpr int kC = 1 << CB;
+	static constexpr int kEPL = kC * FS / kWave;   // table entries per lane
+	static_assert(kEPL >= 1, "chunk block smaller than a wave");
+	static_assert(FS % 4 == 0, "factor slots come in fours");
+	u4_t fv[kEPL];
+
+	__device__ __forceinline__ void load(const u4_t* ftab, int chunk, int lane)
 	{
-		gops = ops;
-		q = q0;
-		q1 = qend;
-		top = -1;
-		top_leaf = false;
-		r_top = q_top = 0.0;
-		cur = base_prev = 1.0;
-		dim_top = 0;
-		pending = false;
-		pend_n = pend_t = 0;
-		pend_p = 0.0;
-		P[lane] = 1.0;
-		op = ldc_op(gops + q);
-		nop = ldc_op(gops + q + 1);
+		const u4_t* src = ftab + ((size_t)chunk << CB) * FS + (size_t)lane * kEPL;
+#pragma unroll
+		for (int e = 0; e < kEPL; e++) fv[e] = src[e];
 	}
 
-	__device__ __forceinline__ void leave(const u4_t& e, double r, int tt, int lane,
+	__device__ __forceinline__ void node(int nl, double w, double* DX, int lane,
 		const d16_t& xv0, const d16_t& xv1, const d16_t& xv2, const d16_t& xv3)
 	{
-		const uint32_t d2 = __builtin_amdgcn_readfirstlane(e[0]);
-		const double iv = f64_from_high(__builtin_amdgcn_readfirstlane(e[1]));
-		const double u = fma(xpin<XB>(xv0, xv1, xv2, xv3, d2), iv, f64_from_high(__builtin_amdgcn_readfirstlane(e[2])));
-		lds_add(DX + (d2 >> 1) * kWave + lane, r * dphi_over_phi(u, iv));
-		if (tt > 0) lds_add(R + (tt - 1) * kWave + lane, r);
-	}
-
-	// Leaves every edge at depth >= t.
-	__device__ __forceinline__ void leave_from(int t, int lane,
-		const d16_t& xv0, const d16_t& xv1, const d16_t& xv2, const d16_t& xv3)
-	{
-		int tt = top;
-		if (top_leaf && tt >= t)
+		uint32_t d2[FS];
+		double phi[FS], dp[FS];
+#pragma unroll
+		for (int f0 = 0; f0 < FS; f0 += 4)
 		{
-			if (wave_any(r_top != 0.0))
+			uint32_t ih[4], jh[4];
+#pragma unroll
+			for (int k = 0; k < 4; k++)
 			{
-				lds_add(DX + dim_top * kWave + lane, r_top * q_top);
-				if (tt > 0) lds_add(R + (tt - 1) * kWave + lane, r_top);
+				const int f = f0 + k;
+				const int src = nl * (kWave / kC) + f / kEPL;
+				const u4_t& v = fv[f % kEPL];
+				d2[f] = __builtin_amdgcn_readlane(v[0], src);
+				ih[k] = __builtin_amdgcn_readlane(v[1], src);
+				jh[k] = __builtin_amdgcn_readlane(v[2], src);
 			}
-			tt--;
-		}
-		for (; tt >= t; tt--)
-		{
-			const double r = R[tt * kWave + lane];
-			if (wave_any(r != 0.0)) leave(edges[tt], r, tt, lane, xv0, xv1, xv2, xv3);
-		}
-	}
-
-	__device__ __forceinline__ void emit(double w, double p, int t, int lane)
-	{
-		if (top_leaf) r_top = w * p;
-		else R[t * kWave + lane] = w * p;
-	}
-
-	// Walks until the segment ends (true) or until the next node to emit
-	// lies in a chunk above `ready` (false, the emission kept pending).
-	template <typename WF>
-	__device__ __forceinline__ bool advance(int ready, WF wof, int lane,
-		const d16_t& xv0, const d16_t& xv1, const d16_t& xv2, const d16_t& xv3)
-	{
-		if (pending)
-		{
-			if ((pend_n >> CB) > ready) return false;
-			emit(wof(pend_n), pend_p, pend_t, lane);
-			pending = false;
-		}
-		while (q < q1)
-		{
-			// the stream ends with kOpPad sentinels, so q + 2 is in bounds;
-			// op q + 2 is loaded now and waited for a step later
-			const Op nnop = ldc_op(gops + q + 2);
-			__builtin_amdgcn_sched_barrier(0);
-			const uint32_t dimw = op.dim, ctrl = op.ctrl;
-			const uint32_t d2 = dimw & kOpDimMask;
-			const int t = (int)(ctrl & 15u);
-			leave_from(t, lane, xv0, xv1, xv2, xv3);
-			top = t;
-			top_leaf = !(ctrl & kOpKids);
-			const uint32_t rel = (ctrl >> kOpRelShift) & 3u;
-			double bt;
-			if (rel == 0u) bt = cur;
-			else if (rel == 1u) bt = base_prev;
-			else bt = P[t * kWave + lane];
-			const double iv = f64_from_high(op.ihi);
-			const double u = fma(xpin<XB>(xv0, xv1, xv2, xv3, d2), iv, f64_from_high(op.njhi));
-			const double p = bt * fmax(1.0 - fabs(u), 0.0);
-			if (top_leaf)
+			double xs[4];
+			xpin4<XB>(xv0, xv1, xv2, xv3, d2[f0], d2[f0 + 1], d2[f0 + 2], d2[f0 + 3], xs[0], xs[1], xs[2], xs[3]);
+#pragma unroll
+			for (int k = 0; k < 4; k++)
 			{
-				r_top = 0.0;
-				dim_top = d2 >> 1;
-			}
-			else
-			{
-				P[(t + 1) * kWave + lane] = p;
-				R[t * kWave + lane] = 0.0;
-				if (lane == 0) edges[t] = u4_t{ d2, op.ihi, op.njhi, 0u };
-			}
-			base_prev = bt;
-			cur = p;
-			if (!wave_any(p != 0.0))
-			{
-				const int rs = (int)(dimw >> kOpSkipShift);
-				q += rs;
-				if (rs == 1) { op = nop; nop = nnop; }
-				else if (rs == 2) { op = nnop; nop = ldc_op(gops + q + 1); }
-				else { op = ldc_op(gops + q); nop = ldc_op(gops + q + 1); }
-				continue;
-			}
-			q++;
-			op = nop;
-			nop = nnop;
-			if (top_leaf) q_top = dphi_over_phi(u, iv);
-			if (ctrl & kOpLast)
-			{
-				const int n = (int)(ctrl >> kOpNodeShift);
-				if ((n >> CB) > ready)
-				{
-					pending = true;
-					pend_n = n;
-					pend_t = t;
-					pend_p = p;
-					return false;
-				}
-				emit(wof(n), p, t, lane);
+				const double iv = f64_from_high(ih[k]);
+				const double u = fma(xs[k], iv, f64_from_high(jh[k]));
+				phi[f0 + k] = fmax(1.0 - fabs(u), 0.0);
+				dp[f0 + k] = dphi(u, iv, phi[f0 + k]);
 			}
 		}
-		return true;
+		double suf[FS];
+		double acc = w;
+#pragma unroll
+		for (int f = FS - 1; f >= 0; f--)
+		{
+			suf[f] = acc;
+			acc *= phi[f];
+		}
+		double pre = 1.0;
+#pragma unroll
+		for (int f = 0; f < FS; f++)
+		{
+			lds_add(DX + (d2[f] >> 1) * kWave + lane, pre * suf[f] * dp[f]);
+			pre *= phi[f];
+		}
 	}
 };
 
@@ -474,13 +413,31 @@ struct AdjCons<HDDM_LB_F32>
 
 } // namespace
 
+// Loads the block's coordinates into the four 16-dim banks (the pinned
+// registers the walk and xpin read).
+template <int XB>
+__device__ __forceinline__ void load_xbanks(const double* xrow, int dim, double (&xb0)[16], double (&xb1)[16],
+	double (&xb2)[16], double (&xb3)[16])
+{
+#pragma unroll
+	for (int d = 0; d < 16; d++)
+	{
+		xb0[d] = xrow[d < dim ? d : 0];
+		xb1[d] = XB > 1 ? xrow[16 + d < dim ? 16 + d : 0] : 0.0;
+		xb2[d] = XB > 2 ? xrow[32 + d < dim ? 32 + d : 0] : 0.0;
+		xb3[d] = XB > 2 ? xrow[48 + d < dim ? 48 + d : 0] : 0.0;
+	}
+}
+
 template <int PREC, int XB, int PM>
-__global__ void __launch_bounds__(64 * (kANC + 1)) adjoint_ws_kernel(AdjointArgs a)
+__global__ void __launch_bounds__(64 * (kANC + 2)) adjoint_ws_kernel(AdjointArgs a)
 {
 	extern __shared__ __attribute__((aligned(16))) double lds[];
 	typedef adj_t<PREC> T;
 	constexpr int CB = adj_chunk_bits<PREC>();
 	constexpr size_t kBuf = adj_buf_bytes<PREC>();
+	constexpr int kDxWave = kANC + 1;
+	constexpr int kReaders = kANC + 1;   // consumers and the dx wave
 	const ALayout<PREC> L(PM, a.dim);
 	char* const lbase = (char*)lds;
 	AMsg* const msgs = (AMsg*)(lbase + L.msgs);
@@ -489,6 +446,8 @@ __global__ void __launch_bounds__(64 * (kANC + 1)) adjoint_ws_kernel(AdjointArgs
 	const int wave = uniform((int)(threadIdx.x >> 6));
 	auto slot = [&](int n) { return (T*)(lbase + L.slots + (size_t)(n & (kARing - 1)) * kBuf); };
 	auto wbuf = [&](int par, int c) { return (T*)(lbase + L.wbuf + ((size_t)(par & 1) * kANC + c) * kBuf); };
+	const bool st_on = a.stats != nullptr;
+	auto clk = [&]() -> uint64_t { return st_on ? __builtin_amdgcn_s_memtime() : 0; };
 
 	if (threadIdx.x == 0)
 	{
@@ -503,47 +462,30 @@ __global__ void __launch_bounds__(64 * (kANC + 1)) adjoint_ws_kernel(AdjointArgs
 		// ---- walker ----
 		const int64_t nitems = ldc(&a.hdr->nitems);
 		Walker<XB, T, kWalkSmem, PM, PREC == HDDM_LB_FP64, CB, kARow> wa;
-		DxWalk<XB, CB> wb;
-		wb.P = (double*)(lbase + L.pb);
-		wb.R = (double*)(lbase + L.rb);
-		wb.edges = (u4_t*)(lbase + L.edges);
-		wb.DX = (double*)(lbase + L.dx);
-		int npub = 0, nchunks = 0;
+		int npub = 0;
 		bool ok = true;
-		// statistics mode: shader-clock cycles of walk A, walk B, waits for w
-		// and waits for ring slots; items and chunks
-		const bool st_on = a.stats != nullptr;
-		uint64_t cyc_a = 0, cyc_b = 0, cyc_w = 0, cyc_s = 0, st_items = 0, st_chunks = 0;
-		auto clk = [&]() -> uint64_t { return st_on ? __builtin_amdgcn_s_memtime() : 0; };
+		// statistics mode: shader-clock cycles of the walk and of waits for
+		// ring slots; items and chunks
+		uint64_t cyc_a = 0, cyc_s = 0, st_items = 0, st_chunks = 0;
 		auto acquire = [&]() -> bool {
 			if (npub < kARing) return true;
 			const uint64_t t0 = clk();
 			bool r = true;
 #pragma unroll 1
-			for (int c = 0; c < kANC; c++)
+			for (int c = 0; c < kReaders; c++)
 				if (!a_wait(&sy->done[c], npub - kARing + 1, sy, a.status)) { r = false; break; }
 			cyc_s += clk() - t0;
 			return r;
 		};
-		auto publish = [&](int type, int a0, int b0, int np, int64_t start, int64_t lpos) {
+		auto publish = [&](int type, int a0, int b0, int np, int64_t start, int64_t lpos, int64_t count) {
 			AMsg* m = msgs + (npub & (kARing - 1));
 			if (lane == 0)
 			{
 				m->type = type; m->a = a0; m->b = b0; m->np = np;
-				m->start = start; m->lpos = lpos;
+				m->start = start; m->lpos = lpos; m->count = count;
 			}
 			a_release(&sy->produced, npub + 1);
 			npub++;
-		};
-		// waits until every consumer has handled message m
-		auto wait_msg = [&](int m) -> bool {
-			const uint64_t t0 = clk();
-			bool r = true;
-#pragma unroll 1
-			for (int c = 0; c < kANC; c++)
-				if (!a_wait(&sy->done[c], m + 1, sy, a.status)) { r = false; break; }
-			cyc_w += clk() - t0;
-			return r;
 		};
 		for (;;)
 		{
@@ -567,51 +509,11 @@ __global__ void __launch_bounds__(64 * (kANC + 1)) adjoint_ws_kernel(AdjointArgs
 			if (np == 0) continue;
 			const int myp = lane < np ? lane : np - 1;
 			const uint32_t orig = a.perm[it.start + blk * kWave + myp];
-			const double* xrow = a.x + (int64_t)orig * a.ldx;
 			double xb0[16], xb1[16], xb2[16], xb3[16];
-#pragma unroll
-			for (int d = 0; d < 16; d++)
-			{
-				xb0[d] = xrow[d < a.dim ? d : 0];
-				xb1[d] = XB > 1 ? xrow[16 + d < a.dim ? 16 + d : 0] : 0.0;
-				xb2[d] = XB > 2 ? xrow[32 + d < a.dim ? 32 + d : 0] : 0.0;
-				xb3[d] = XB > 2 ? xrow[48 + d < a.dim ? 48 + d : 0] : 0.0;
-			}
-			d16_t xv0, xv1, xv2, xv3;
-#pragma unroll
-			for (int i = 0; i < 16; i++) { xv0[i] = xb0[i]; xv1[i] = xb1[i]; xv2[i] = xb2[i]; xv3[i] = xb3[i]; }
-			// w of node n: the tiles' buffers of the chunk, summed in consumer order
-			const bool split = tpass <= kANC / 2;
-			int wpar = 0;
-			auto wof = [&](int n) -> double {
-				const int nl = n & ((1 << CB) - 1);
-				double s = 0.0;
-#pragma unroll
-				for (int c = 0; c < kANC; c++)
-				{
-					const bool wc = split ? ((c & 1) && (c >> 1) < tpass) : c < tpass;
-					if (wc) s += (double)wbuf(wpar, c)[nl * kARow + lane];
-				}
-				return s;
-			};
+			load_xbanks<XB>(a.x + (int64_t)orig * a.ldx, a.dim, xb0, xb1, xb2, xb3);
 			if (!(ok = acquire())) break;
-			publish(kAMsgItem, it.s, it.seg, np, it.start, blk * kWave);
-#pragma unroll
-			for (int d = 0; d < ((a.dim + 1) & ~1); d++) wb.DX[d * kWave + lane] = 0.0;
+			publish(kAMsgItem, it.s, it.seg, np, it.start, blk * kWave, it.count);
 			wa.init(q0, q1, (double*)(lbase + L.pa), slot(npub), lane, gops, gskip);
-			wb.init(q0, q1, gops, lane);
-			int prev_chunk = -1, prev_msg = -1, prev_par = 0;
-			const bool want_dx = a.dx != nullptr;
-			// walk B through the previously published chunk, once its w is ready
-			auto catch_up = [&]() -> bool {
-				if (prev_chunk < 0 || !want_dx) return true;
-				if (!wait_msg(prev_msg)) return false;
-				wpar = prev_par;
-				const uint64_t t0 = clk();
-				wb.advance(prev_chunk, wof, lane, xv0, xv1, xv2, xv3);
-				cyc_b += clk() - t0;
-				return true;
-			};
 			for (;;)
 			{
 				int node = 0;
@@ -622,13 +524,7 @@ __global__ void __launch_bounds__(64 * (kANC + 1)) adjoint_ws_kernel(AdjointArgs
 				if (wa.chunk >= 0)
 				{
 					st_chunks++;
-					const int m = npub, par = nchunks & 1;
-					publish(kAMsgChunk, wa.chunk, (int)wa.mask, 0, 0, 0);
-					nchunks++;
-					if (!(ok = catch_up())) break;
-					prev_chunk = wa.chunk;
-					prev_msg = m;
-					prev_par = par;
+					publish(kAMsgChunk, wa.chunk, (int)wa.mask, 0, 0, 0, 0);
 					wa.chunk = -1;
 				}
 				if (!more) break;
@@ -637,39 +533,167 @@ __global__ void __launch_bounds__(64 * (kANC + 1)) adjoint_ws_kernel(AdjointArgs
 				wa.start_chunk(node, val, lane);
 			}
 			if (!ok) break;
-			if (!want_dx) continue;
-			if (prev_chunk >= 0 && !(ok = wait_msg(prev_msg))) break;
-			// walk B ends with walk A's last chunk: a node left to emit beyond
-			// it means the two walks disagreed
-			wpar = prev_par;
-			const uint64_t tb = clk();
-			if (!wb.advance(prev_chunk, wof, lane, xv0, xv1, xv2, xv3))
-				if (lane == 0) atomicOr(a.status, (uint32_t)HDDM_LB_STATUS_WALK_MISMATCH);
-			wb.leave_from(0, lane, xv0, xv1, xv2, xv3);
-			cyc_b += clk() - tb;
 			st_items++;
-			if (lane < np)
-			{
-				double* out = ldc(&sd->nseg) > 1
-					? a.dxpart + ldc(&a.plans[it.s].dx_base) + ((int64_t)it.seg * it.count + blk * kWave + lane) * a.dim
-					: a.dx + (int64_t)orig * a.lddx;
-				for (int d = 0; d < a.dim; d++)
-				{
-					const double v = wb.DX[d * kWave + lane];
-					out[d] = a.dt_base == 0 ? v : out[d] + v;
-				}
-			}
 		}
-		if (ok && acquire()) publish(kAMsgExit, 0, 0, 0, 0, 0);
+		if (ok && acquire()) publish(kAMsgExit, 0, 0, 0, 0, 0, 0);
 		else a_release(&sy->abort, 1);
 		if (st_on && lane == 0)
 		{
 			atomicAdd(&a.stats[0], (unsigned long long)cyc_a);
-			atomicAdd(&a.stats[1], (unsigned long long)cyc_b);
-			atomicAdd(&a.stats[2], (unsigned long long)cyc_w);
 			atomicAdd(&a.stats[3], (unsigned long long)cyc_s);
 			atomicAdd(&a.stats[7], (unsigned long long)st_items);
 			atomicAdd(&a.stats[8], (unsigned long long)st_chunks);
+		}
+		return;
+	}
+
+	if (wave == kDxWave)
+	{
+		// ---- dx wave ----
+		double* const DX = (double*)(lbase + L.dx);
+		const bool want_dx = a.dx != nullptr;
+		uint64_t cyc_dx = 0, cyc_wait = 0;
+		int nchunks = 0;
+		// the item whose rows DX holds
+		bool have = false;
+		int np = 0, seg = 0, tpass = 0, fs = 4;
+		int64_t count = 0, lpos = 0;
+		uint32_t orig = 0;
+		const StateDesc* sd = nullptr;
+		const u4_t* ftab = nullptr;
+		bool split = false;
+		d16_t xv0, xv1, xv2, xv3;
+		// w of node nl: the tiles' buffers summed in consumer order
+		auto wsum = [&](int par, int nl) -> double {
+			double w = 0.0;
+#pragma unroll
+			for (int c = 0; c < kANC; c++)
+			{
+				const bool wc = split ? ((c & 1) && (c >> 1) < tpass) : c < tpass;
+				if (wc) w += (double)wbuf(par, c)[nl * kARow + lane];
+			}
+			return w;
+		};
+		// One chunk (message n): its factor rows load while the consumers
+		// finish it; then its nodes in order, each node's w read one node
+		// ahead. False when a wait gave up.
+		auto run_chunk = [&](auto&& dn, int chunk, uint32_t mask, int par, int n) -> bool {
+			dn.load(ftab, chunk, lane);
+			const uint64_t t1 = clk();
+#pragma unroll 1
+			for (int c = 0; c < kANC; c++)
+				if (!a_wait(&sy->done[c], n + 1, sy, a.status)) return false;
+			const uint64_t t2 = clk();
+			cyc_wait += t2 - t1;
+			// two nodes at a time (a lone last node is paired with itself at
+			// w = 0, which adds zeros); the next pair's w is read first
+			auto take = [&](int& n0, int& n1, double& w0, double& w1) {
+				n0 = __builtin_ctz(mask);
+				mask &= mask - 1u;
+				const bool two = mask != 0u;
+				n1 = two ? __builtin_ctz(mask) : n0;
+				mask &= mask - 1u;
+				w0 = wsum(par, n0);
+				w1 = two ? wsum(par, n1) : 0.0;
+			};
+			if (!mask) return true;
+			int n0, n1;
+			double w0, w1;
+			take(n0, n1, w0, w1);
+			for (;;)
+			{
+				const bool more = mask != 0u;
+				int m0 = n0, m1 = n1;
+				double v0 = 0.0, v1 = 0.0;
+				if (more) take(m0, m1, v0, v1);
+				dn.node(n0, w0, DX, lane, xv0, xv1, xv2, xv3);
+				dn.node(n1, w1, DX, lane, xv0, xv1, xv2, xv3);
+				if (!more) break;
+				n0 = m0; n1 = m1; w0 = v0; w1 = v1;
+			}
+			cyc_dx += clk() - t2;
+			return true;
+		};
+		auto flush = [&]() {
+			if (!have || !want_dx) return;
+			if (lane < np)
+			{
+				double* out = ldc(&sd->nseg) > 1
+					? a.dxpart + ldc(&a.plans[(int)(sd - a.states)].dx_base) + ((int64_t)seg * count + lpos + lane) * a.dim
+					: a.dx + (int64_t)orig * a.lddx;
+				for (int d = 0; d < a.dim; d++)
+				{
+					const double v = DX[d * kWave + lane];
+					out[d] = a.dt_base == 0 ? v : out[d] + v;
+				}
+			}
+			have = false;
+		};
+		for (int n = 0;; n++)
+		{
+			const uint64_t t0 = clk();
+			const bool got = a_wait(&sy->produced, n + 1, sy, a.status);
+			cyc_wait += clk() - t0;
+			if (!got) break;
+			const AMsg* m = msgs + (n & (kARing - 1));
+			const int type = uniform(m->type);
+			if (type == kAMsgChunk)
+			{
+				const int par = nchunks & 1;
+				nchunks++;
+				if (have && want_dx)
+				{
+					const int chunk = uniform(m->a);
+					const uint32_t mask = (uint32_t)uniform(m->b);
+					bool ok = true;
+					if (fs == 4) ok = run_chunk(DxNodes<XB, 4, CB>(), chunk, mask, par, n);
+					else if (fs == 8) ok = run_chunk(DxNodes<XB, 8, CB>(), chunk, mask, par, n);
+					// 16 factor slots next to four coordinate banks would spill:
+					// the host runs such states on the per-tile kernels
+					else if constexpr (XB < 4) ok = run_chunk(DxNodes<XB, 16, CB>(), chunk, mask, par, n);
+					if (!ok) break;
+				}
+			}
+			else if (type == kAMsgItem)
+			{
+				flush();
+				const int s = uniform(m->a);
+				seg = uniform(m->b);
+				np = uniform(m->np);
+				const int64_t start = (int64_t)uniform64((uint64_t)m->start);
+				lpos = (int64_t)uniform64((uint64_t)m->lpos);
+				count = (int64_t)uniform64((uint64_t)m->count);
+				sd = a.states + s;
+				const int DP = ldc(&sd->dofs_pad);
+				tpass = min(kANC, (DP >> 5) - a.dt_base);
+				split = tpass <= kANC / 2;
+				ftab = (const u4_t*)ldc(&sd->ftab);
+				fs = (int)ldc(&sd->fstride);
+				have = true;
+				if (want_dx)
+				{
+					const int myp = lane < np ? lane : np - 1;
+					orig = a.perm[start + lpos + myp];
+					double xb0[16], xb1[16], xb2[16], xb3[16];
+					load_xbanks<XB>(a.x + (int64_t)orig * a.ldx, a.dim, xb0, xb1, xb2, xb3);
+#pragma unroll
+					for (int i = 0; i < 16; i++) { xv0[i] = xb0[i]; xv1[i] = xb1[i]; xv2[i] = xb2[i]; xv3[i] = xb3[i]; }
+					for (int d = 0; d < ((a.dim + 1) & ~1); d++) DX[d * kWave + lane] = 0.0;
+
+				}
+			}
+			else
+			{
+				flush();
+				a_release(&sy->done[kDxWave - 1], n + 1);
+				break;
+			}
+			a_release(&sy->done[kDxWave - 1], n + 1);
+		}
+		if (st_on && lane == 0)
+		{
+			atomicAdd(&a.stats[1], (unsigned long long)cyc_dx);
+			atomicAdd(&a.stats[2], (unsigned long long)cyc_wait);
 		}
 		return;
 	}
@@ -681,10 +705,9 @@ __global__ void __launch_bounds__(64 * (kANC + 1)) adjoint_ws_kernel(AdjointArgs
 	bool active = false;
 	int role = 0, tile = 0, dofs = 0, DP = 32;
 	int nchunks = 0;
+	int wmsg[2] = { -1, -1 };   // message of the last chunk whose w went to each buffer
 	double* dsb = nullptr;
-	const bool st_on = a.stats != nullptr;
 	uint64_t cyc_ds = 0, cyc_wc = 0, cyc_idle = 0;
-	auto clk = [&]() -> uint64_t { return st_on ? __builtin_amdgcn_s_memtime() : 0; };
 	for (int n = 0;; n++)
 	{
 		const uint64_t t0 = clk();
@@ -705,7 +728,13 @@ __global__ void __launch_bounds__(64 * (kANC + 1)) adjoint_ws_kernel(AdjointArgs
 				const uint64_t t1 = clk();
 				if ((role & kRoleDs) && a.ds) cons.ds(a, sd, dsb, Bc, chunk, mask, tile, dofs, lane);
 				const uint64_t t2 = clk();
-				if (role & kRoleW) cons.w(sd, wbuf(par, c), chunk, mask, tile, DP, lane);
+				if (role & kRoleW)
+				{
+					// the dx wave must be done with this buffer's previous chunk
+					if (wmsg[par] >= 0 && !a_wait(&sy->done[kDxWave - 1], wmsg[par] + 1, sy, a.status)) break;
+					wmsg[par] = n;
+					cons.w(sd, wbuf(par, c), chunk, mask, tile, DP, lane);
+				}
 				cyc_ds += t2 - t1;
 				cyc_wc += clk() - t2;
 			}

@@ -265,6 +265,8 @@ extern "C" int hddm_lb_upload_state(hddm_lb_ctx* ctx, int istate, int dim, int n
 	d.sbf = (const uint16_t*)up(L.sbf.data(), L.sbf.size() * sizeof(uint16_t));
 	d.s32 = (const float*)up(L.s32.data(), L.s32.size() * sizeof(float));
 	d.s32r = (const float*)up(L.s32r.data(), L.s32r.size() * sizeof(float));
+	d.ftab = (const uint32_t*)up(L.ftab.data(), L.ftab.size() * sizeof(uint32_t));
+	d.fstride = L.fstride;
 	d.iscale = (const double*)up(L.iscale.data(), L.iscale.size() * sizeof(double));
 	d.order = (const int32_t*)up(L.order.data(), L.order.size() * sizeof(int32_t));
 	d.lomask = (const uint64_t*)up(L.lomask.data(), L.lomask.size() * sizeof(uint64_t));

@@ -65,6 +65,11 @@ struct StateDesc
 	const uint16_t* sbf;            // [nchunks][3][dofs_pad][16] bf16 terms
 	const float* s32;               // [nchunks][dofs_pad][2][8] f32
 	const float* s32r;              // [nno * dofs_pad] f32 rows, trie order
+	// Factors of every node (trie order), fstride slots of 16 bytes each:
+	// {twice the dim, high word of i, high word of -j, 1} as in an op, then
+	// {0, 0, 0, 0} padding (phi == 1 at u = 0, no derivative).
+	const uint32_t* ftab;           // [nno rounded up to 32][fstride][4]
+	int64_t fstride;
 	const double* iscale;           // [dofs_pad] 2^-sigma_c undoing the f16 scaling
 	const int32_t* order;           // trie position -> uploaded node index
 	const uint64_t* lomask;         // [nno] dims whose support lies in [0, 1/2]

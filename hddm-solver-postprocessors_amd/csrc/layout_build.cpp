@@ -226,6 +226,23 @@ int build_layout(int dim, int nno, int dofs,
 		o.dim |= (L.skip[q] - (uint32_t)q) << kOpSkipShift;
 	}
 
+	// Per-node factor table (the adjoint's dx terms), slots padded to 4, 8 or
+	// 16, nodes padded to a multiple of 32 (the adjoint reads whole chunks).
+	L.fstride = FM <= 4 ? 4 : (FM <= 8 ? 8 : 16);
+	L.ftab.assign((size_t)((nno + 31) & ~31) * L.fstride * 4, 0u);
+	for (int n = 0; n < nno; n++)
+	{
+		const auto& cur = seq[L.order[n]];
+		uint32_t* e = &L.ftab[(size_t)n * L.fstride * 4];
+		for (size_t t = 0; t < cur.size(); t++, e += 4)
+		{
+			e[0] = (uint32_t)cur[t].d << 1;
+			f64_high_word(cur[t].i, e[1]);
+			f64_high_word(-(int64_t)cur[t].j, e[2]);
+			e[3] = 1u;
+		}
+	}
+
 	// Support masks: bit d of lomask (himask) when the node's support in dim d
 	// lies in [0, 1/2] ([1/2, 1]); dims >= 64 are left out (conservative).
 	L.lomask.assign(nno, 0);

@@ -20,6 +20,8 @@ struct HostLayout
 	std::vector<uint16_t> sbf;        // [nchunks][3][dofs_pad][16]
 	std::vector<float> s32;           // [nchunks][dofs_pad][2][8]
 	std::vector<float> s32r;          // [nno * dofs_pad] FP32 rows (adjoint's g . S products)
+	int fstride = 0;                  // factor slots per node in ftab: 4, 8 or 16
+	std::vector<uint32_t> ftab;       // [nno][fstride][4] factors of each node, trie order
 	std::vector<double> iscale;       // [dofs_pad]
 	std::vector<int32_t> order;       // trie position -> uploaded node index
 	std::vector<uint64_t> lomask, himask;  // [nno]

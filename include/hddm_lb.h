@@ -202,8 +202,8 @@ int hddm_lb_last_stats_n(hddm_lb_ctx* ctx, uint64_t* stats, int n);
 
 /* With HDDM_LB_OPT_STATS set, counters of the last wave-specialized adjoint,
  * summed over workgroups (shader-clock cycles from s_memtime): [0] walker in
- * the forward walk, [1] walker in the dx edge-sum walk, [2] walker waiting
- * for g.S, [3] walker waiting for ring slots, [4] consumers computing g.S,
+ * the walk, [1] dx wave adding dx terms, [2] dx wave waiting for messages and
+ * g.S, [3] walker waiting for ring slots, [4] consumers computing g.S,
  * [5] consumers computing dS, [6] consumers idle, [7] work items, [8] chunks.
  * The first n (1 to 9). Waits for the device. */
 int hddm_lb_last_adjoint_stats(hddm_lb_ctx* ctx, uint64_t* stats, int n);

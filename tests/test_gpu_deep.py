@@ -82,9 +82,13 @@ def test_deep_nodes_forward(deep_case, prec):
     assert err <= TOL[prec]
 
 
-def test_deep_nodes_adjoint(deep_case):
+@pytest.mark.parametrize("form", [0, 2])
+def test_deep_nodes_adjoint(deep_case, form):
+    """Automatic choice (the per-tile kernels at D=16) and the
+    wave-specialized kernel (16 factor slots per node in its dx wave)."""
     import torch
     ctx, ii, jj, S, X = deep_case
+    ctx.set_option(hddm_lb.OPT_FORMULATION, form)
     G = np.random.default_rng(11).standard_normal((X.shape[0], S.shape[1]))
     x = torch.from_numpy(X).cuda()
     g = torch.from_numpy(G).cuda()
@@ -93,8 +97,10 @@ def test_deep_nodes_adjoint(deep_case):
     ds = torch.full((total,), np.nan, dtype=torch.float64, device="cuda")
     ctx.adjoint(x, g, dx, ds, None, hddm_lb.FP64)
     torch.cuda.synchronize()
+    ctx.set_option(hddm_lb.OPT_FORMULATION, 0)
+    assert ctx.device_status() == 0
     dxo, dso = oracle.adjoint(ii, jj, S, X, G)
     ex = normwise_rel_err(dx.cpu().numpy(), dxo)
     es = normwise_rel_err(ds.cpu().numpy().reshape(dso.shape), dso)
-    say("deep nodes adjoint dx err", ex, "ds err", es)
+    say("deep nodes adjoint, formulation", form, "dx err", ex, "ds err", es)
     assert ex <= 1e-10 and es <= 1e-12
