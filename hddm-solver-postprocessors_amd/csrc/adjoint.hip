@@ -603,16 +603,18 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 	}
 	if (ws)
 	{
-		// one walker wave, four consumer waves and one dx wave per
-		// workgroup, four dof tiles per launch
+		// one walker wave, four consumer waves and two dx waves per
+		// workgroup, four dof tiles per launch; the second dx wave works
+		// when its rows fit the LDS
 		const void* kern = adjoint_ws_kernel_ptr(precision, xbanks_for(ctx->dim), ctx->fmax_all);
 		if (!kern) return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: dim %d is not supported", ctx->dim);
-		const size_t lds = adjoint_ws_lds_bytes(precision, ctx->fmax_all, ctx->dim);
+		a.ndx = adjoint_ws_lds_bytes(precision, ctx->fmax_all, ctx->dim, 2) <= ctx->max_lds ? 2 : 1;
+		const size_t lds = adjoint_ws_lds_bytes(precision, ctx->fmax_all, ctx->dim, a.ndx);
 		if (lds > ctx->max_lds)
 			return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: LDS of one workgroup (%zu B) exceeds the CU", lds);
 		HIP_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
 		int per_cu = 0;
-		HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 6 * kWave, lds));
+		HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 7 * kWave, lds));
 		if (per_cu < 1) return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: kernel does not fit a CU");
 		HIP_TRY(launch_fill_u32(sc.hist, nb, 0u, st));
 		HIP_TRY(launch_bucket_key(npoints, x, ldx, state_ids, ctx->d_state_ok, ns, ctx->dim, bb, sc.bkey, sc.hist, st));
@@ -628,7 +630,7 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 			a.dt_base = 4 * ps;
 			if (ps > 0) HIP_TRY(launch_reset_counter(sc.hdr, st));
 			void* params[] = { (void*)&a };
-			HIP_TRY(hipLaunchKernel(kern, dim3(per_cu * ctx->num_cus), dim3(6 * kWave), params, lds, st));
+			HIP_TRY(hipLaunchKernel(kern, dim3(per_cu * ctx->num_cus), dim3(7 * kWave), params, lds, st));
 		}
 		if (ctx->opt_timing)
 		{

@@ -62,25 +62,30 @@ struct ASync
 
 template <int PREC> constexpr int adj_chunk_bits() { return PREC == HDDM_LB_FP64 ? 4 : 5; }
 template <int PREC> using adj_t = typename std::conditional<PREC == HDDM_LB_FP64, double, float>::type;
-// one chunk buffer (and one consumer's w buffer): nodes x padded row
+// one chunk buffer: nodes x padded row
 template <int PREC> constexpr size_t adj_buf_bytes() { return ((size_t)1 << adj_chunk_bits<PREC>()) * kARow * sizeof(adj_t<PREC>); }
+// one consumer's w buffer: nodes x 64 points, point p of node n at n * 64 +
+// (p ^ n) (the swizzle spreads the MFMA layout's column writes over banks
+// without padding)
+template <int PREC> constexpr size_t adj_wbuf_bytes() { return ((size_t)1 << adj_chunk_bits<PREC>()) * kWave * sizeof(adj_t<PREC>); }
+__device__ __forceinline__ int wpos(int node, int point) { return node * kWave + (point ^ node); }
 
 // LDS layout of the workgroup (bytes): kARing chunk slots, 2 x kANC w buffers,
-// messages, counters, the walk's path stack [PM][64], the dx wave's rows
-// [dim rounded up to even][64].
+// messages, counters, the walk's path stack [PM][64], ndx sets of dx rows
+// [dim][64] (one per dx wave).
 template <int PREC>
 struct ALayout
 {
 	size_t slots, wbuf, msgs, sync, pa, dx, total;
-	__host__ __device__ ALayout(int pm, int dim)
+	__host__ __device__ ALayout(int pm, int dim, int ndx)
 	{
 		slots = 0;
 		wbuf = slots + kARing * adj_buf_bytes<PREC>();
-		msgs = wbuf + 2 * kANC * adj_buf_bytes<PREC>();
+		msgs = wbuf + 2 * kANC * adj_wbuf_bytes<PREC>();
 		sync = msgs + kARing * sizeof(AMsg);
 		pa = sync + 64;
 		dx = pa + (size_t)pm * kWave * 8;
-		total = dx + (size_t)((dim + 1) & ~1) * kWave * 8;
+		total = dx + (size_t)ndx * dim * kWave * 8;
 	}
 };
 
@@ -325,7 +330,7 @@ struct AdjCons<HDDM_LB_FP64>
 #pragma unroll
 		for (int pb = 0; pb < 4; pb++)
 #pragma unroll
-			for (int e = 0; e < 4; e++) Wb[col * kARow + 16 * pb + 4 * e + kq] = acc[pb][e];
+			for (int e = 0; e < 4; e++) Wb[wpos(col, 16 * pb + 4 * e + kq)] = acc[pb][e];
 	}
 };
 
@@ -407,7 +412,7 @@ struct AdjCons<HDDM_LB_F32>
 #pragma unroll
 		for (int h = 0; h < 2; h++)
 #pragma unroll
-			for (int r = 0; r < 16; r++) Wb[col * kARow + 32 * h + 8 * (r >> 2) + 4 * kh + (r & 3)] = acc[h][r];
+			for (int r = 0; r < 16; r++) Wb[wpos(col, 32 * h + 8 * (r >> 2) + 4 * kh + (r & 3))] = acc[h][r];
 	}
 };
 
@@ -430,22 +435,26 @@ __device__ __forceinline__ void load_xbanks(const double* xrow, int dim, double 
 }
 
 template <int PREC, int XB, int PM>
-__global__ void __launch_bounds__(64 * (kANC + 2)) adjoint_ws_kernel(AdjointArgs a)
+__global__ void __launch_bounds__(64 * (kANC + 3)) adjoint_ws_kernel(AdjointArgs a)
 {
 	extern __shared__ __attribute__((aligned(16))) double lds[];
 	typedef adj_t<PREC> T;
 	constexpr int CB = adj_chunk_bits<PREC>();
 	constexpr size_t kBuf = adj_buf_bytes<PREC>();
-	constexpr int kDxWave = kANC + 1;
-	constexpr int kReaders = kANC + 1;   // consumers and the dx wave
-	const ALayout<PREC> L(PM, a.dim);
+	constexpr size_t kWBuf = adj_wbuf_bytes<PREC>();
+	constexpr int kDxWave = kANC + 1;    // first of the two dx waves
+	constexpr int kReaders = kANC + 2;   // consumers and dx waves
+	const int ndx = a.ndx;               // dx waves with work: 2, or 1 when two sets of rows exceed the LDS
+	const ALayout<PREC> L(PM, a.dim, ndx);
 	char* const lbase = (char*)lds;
 	AMsg* const msgs = (AMsg*)(lbase + L.msgs);
 	ASync* const sy = (ASync*)(lbase + L.sync);
 	const int lane = threadIdx.x & 63;
 	const int wave = uniform((int)(threadIdx.x >> 6));
 	auto slot = [&](int n) { return (T*)(lbase + L.slots + (size_t)(n & (kARing - 1)) * kBuf); };
-	auto wbuf = [&](int par, int c) { return (T*)(lbase + L.wbuf + ((size_t)(par & 1) * kANC + c) * kBuf); };
+	auto wbuf = [&](int par, int c) { return (T*)(lbase + L.wbuf + ((size_t)(par & 1) * kANC + c) * kWBuf); };
+	// the dx wave that takes the chunks of one parity, and its counter
+	auto dxw = [&](int par) { return ndx == 2 ? (par & 1) : 0; };
 	const bool st_on = a.stats != nullptr;
 	auto clk = [&]() -> uint64_t { return st_on ? __builtin_amdgcn_s_memtime() : 0; };
 
@@ -547,11 +556,18 @@ __global__ void __launch_bounds__(64 * (kANC + 2)) adjoint_ws_kernel(AdjointArgs
 		return;
 	}
 
-	if (wave == kDxWave)
+	if (wave >= kDxWave)
 	{
-		// ---- dx wave ----
-		double* const DX = (double*)(lbase + L.dx);
-		const bool want_dx = a.dx != nullptr;
+		// ---- dx waves ----
+		// Wave k takes the chunks of parity k (all chunks when ndx == 1)
+		// into rows of its own; at the end of an item wave 0 writes out the
+		// sum of both waves' rows (its own first), once wave 1 is done with
+		// the item, and wave 1 clears its rows only after that.
+		const int k = wave - kDxWave;
+		double* const DX = (double*)(lbase + L.dx) + (size_t)(ndx == 2 ? k : 0) * a.dim * kWave;
+		double* const DX1 = (double*)(lbase + L.dx) + (size_t)a.dim * kWave;
+		int* const mydone = &sy->done[kANC + k];
+		const bool want_dx = a.dx != nullptr && k < ndx;
 		uint64_t cyc_dx = 0, cyc_wait = 0;
 		int nchunks = 0;
 		// the item whose rows DX holds
@@ -570,13 +586,14 @@ __global__ void __launch_bounds__(64 * (kANC + 2)) adjoint_ws_kernel(AdjointArgs
 			for (int c = 0; c < kANC; c++)
 			{
 				const bool wc = split ? ((c & 1) && (c >> 1) < tpass) : c < tpass;
-				if (wc) w += (double)wbuf(par, c)[nl * kARow + lane];
+				if (wc) w += (double)wbuf(par, c)[wpos(nl, lane)];
 			}
 			return w;
 		};
 		// One chunk (message n): its factor rows load while the consumers
-		// finish it; then its nodes in order, each node's w read one node
-		// ahead. False when a wait gave up.
+		// finish it; then its nodes in order, two at a time (a lone last node
+		// is paired with itself at w = 0, which adds zeros), the next pair's
+		// w read first. False when a wait gave up.
 		auto run_chunk = [&](auto&& dn, int chunk, uint32_t mask, int par, int n) -> bool {
 			dn.load(ftab, chunk, lane);
 			const uint64_t t1 = clk();
@@ -585,8 +602,6 @@ __global__ void __launch_bounds__(64 * (kANC + 2)) adjoint_ws_kernel(AdjointArgs
 				if (!a_wait(&sy->done[c], n + 1, sy, a.status)) return false;
 			const uint64_t t2 = clk();
 			cyc_wait += t2 - t1;
-			// two nodes at a time (a lone last node is paired with itself at
-			// w = 0, which adds zeros); the next pair's w is read first
 			auto take = [&](int& n0, int& n1, double& w0, double& w1) {
 				n0 = __builtin_ctz(mask);
 				mask &= mask - 1u;
@@ -614,20 +629,42 @@ __global__ void __launch_bounds__(64 * (kANC + 2)) adjoint_ws_kernel(AdjointArgs
 			cyc_dx += clk() - t2;
 			return true;
 		};
-		auto flush = [&]() {
-			if (!have || !want_dx) return;
-			if (lane < np)
+		// End of an item (message n): wave 0 writes out the rows. False when
+		// a wait gave up.
+		auto item_end = [&](int n) -> bool {
+			if (!have || a.dx == nullptr) { have = false; return true; }
+			have = false;
+			if (k == 0)
 			{
-				double* out = ldc(&sd->nseg) > 1
-					? a.dxpart + ldc(&a.plans[(int)(sd - a.states)].dx_base) + ((int64_t)seg * count + lpos + lane) * a.dim
-					: a.dx + (int64_t)orig * a.lddx;
-				for (int d = 0; d < a.dim; d++)
+				if (ndx == 2)
 				{
-					const double v = DX[d * kWave + lane];
-					out[d] = a.dt_base == 0 ? v : out[d] + v;
+					const uint64_t t0 = clk();
+					const bool got = a_wait(&sy->done[kANC + 1], n, sy, a.status);
+					cyc_wait += clk() - t0;
+					if (!got) return false;
+				}
+				if (lane < np)
+				{
+					double* out = ldc(&sd->nseg) > 1
+						? a.dxpart + ldc(&a.plans[(int)(sd - a.states)].dx_base) + ((int64_t)seg * count + lpos + lane) * a.dim
+						: a.dx + (int64_t)orig * a.lddx;
+					for (int d = 0; d < a.dim; d++)
+					{
+						double v = DX[d * kWave + lane];
+						if (ndx == 2) v += DX1[d * kWave + lane];
+						out[d] = a.dt_base == 0 ? v : out[d] + v;
+					}
 				}
 			}
-			have = false;
+			else if (ndx == 2)
+			{
+				// wave 0 reads these rows when it writes the item out
+				const uint64_t t0 = clk();
+				const bool got = a_wait(&sy->done[kANC], n + 1, sy, a.status);
+				cyc_wait += clk() - t0;
+				if (!got) return false;
+			}
+			return true;
 		};
 		for (int n = 0;; n++)
 		{
@@ -641,7 +678,7 @@ __global__ void __launch_bounds__(64 * (kANC + 2)) adjoint_ws_kernel(AdjointArgs
 			{
 				const int par = nchunks & 1;
 				nchunks++;
-				if (have && want_dx)
+				if (have && want_dx && dxw(par) == k)
 				{
 					const int chunk = uniform(m->a);
 					const uint32_t mask = (uint32_t)uniform(m->b);
@@ -656,7 +693,7 @@ __global__ void __launch_bounds__(64 * (kANC + 2)) adjoint_ws_kernel(AdjointArgs
 			}
 			else if (type == kAMsgItem)
 			{
-				flush();
+				if (!item_end(n)) break;
 				const int s = uniform(m->a);
 				seg = uniform(m->b);
 				np = uniform(m->np);
@@ -678,17 +715,16 @@ __global__ void __launch_bounds__(64 * (kANC + 2)) adjoint_ws_kernel(AdjointArgs
 					load_xbanks<XB>(a.x + (int64_t)orig * a.ldx, a.dim, xb0, xb1, xb2, xb3);
 #pragma unroll
 					for (int i = 0; i < 16; i++) { xv0[i] = xb0[i]; xv1[i] = xb1[i]; xv2[i] = xb2[i]; xv3[i] = xb3[i]; }
-					for (int d = 0; d < ((a.dim + 1) & ~1); d++) DX[d * kWave + lane] = 0.0;
-
+					for (int d = 0; d < a.dim; d++) DX[d * kWave + lane] = 0.0;
 				}
 			}
 			else
 			{
-				flush();
-				a_release(&sy->done[kDxWave - 1], n + 1);
+				item_end(n);
+				a_release(mydone, n + 1);
 				break;
 			}
-			a_release(&sy->done[kDxWave - 1], n + 1);
+			a_release(mydone, n + 1);
 		}
 		if (st_on && lane == 0)
 		{
@@ -730,8 +766,9 @@ __global__ void __launch_bounds__(64 * (kANC + 2)) adjoint_ws_kernel(AdjointArgs
 				const uint64_t t2 = clk();
 				if (role & kRoleW)
 				{
-					// the dx wave must be done with this buffer's previous chunk
-					if (wmsg[par] >= 0 && !a_wait(&sy->done[kDxWave - 1], wmsg[par] + 1, sy, a.status)) break;
+					// the dx wave of this parity must be done with the buffer's
+					// previous chunk
+					if (wmsg[par] >= 0 && !a_wait(&sy->done[kANC + dxw(par)], wmsg[par] + 1, sy, a.status)) break;
 					wmsg[par] = n;
 					cons.w(sd, wbuf(par, c), chunk, mask, tile, DP, lane);
 				}
@@ -787,10 +824,10 @@ __global__ void __launch_bounds__(64 * (kANC + 2)) adjoint_ws_kernel(AdjointArgs
 	}
 }
 
-size_t adjoint_ws_lds_bytes(int prec, int depth, int dim)
+size_t adjoint_ws_lds_bytes(int prec, int depth, int dim, int ndx)
 {
 	const int pm = depth < 8 ? 8 : 16;
-	return prec == HDDM_LB_FP64 ? ALayout<HDDM_LB_FP64>(pm, dim).total : ALayout<HDDM_LB_F32>(pm, dim).total;
+	return prec == HDDM_LB_FP64 ? ALayout<HDDM_LB_FP64>(pm, dim, ndx).total : ALayout<HDDM_LB_F32>(pm, dim, ndx).total;
 }
 
 template <int PREC, int XB>

@@ -50,6 +50,7 @@ struct AdjointArgs
 	int dt_base;       // dof slab of 32 handled by this launch
 	uint32_t* status;  // device status bits (HDDM_LB_STATUS_*)
 	unsigned long long* stats;  // optional cycle and work counters of the wave-specialized adjoint
+	int ndx;           // wave-specialized adjoint: dx waves with work (1 or 2)
 };
 
 // How a sweep reads the op stream: each wave through scalar loads, waves
@@ -70,7 +71,7 @@ size_t sweep_ws_lds_bytes(int prec, int depth);
 // Wave-specialized adjoint (adjoint_ws.hip): a walker wave and 4 consumer
 // waves; FP64 or F32; 4 dof tiles per launch. NULL when unsupported.
 const void* adjoint_ws_kernel_ptr(int prec, int xbanks, int depth);
-size_t adjoint_ws_lds_bytes(int prec, int depth, int dim);
+size_t adjoint_ws_lds_bytes(int prec, int depth, int dim, int ndx);
 // Register banks needed for dim (0: dim above 64 is unsupported).
 int xbanks_for(int dim);
 
