@@ -8,9 +8,11 @@ Headline workload: high-dimensional multistate forward + adjoint.
 A step is one forward (y) and one adjoint (dx and dsurplus) over a batch of
 262,144 of those points (all four states; the workload's 2^20 points are four
 steps); one step over all 2^20 points is timed after the steps as a check
-that the rate does not depend on the batch. The mixed step runs the forward on f16 MFMA with split hi/lo operands
-(HDDM_LB_F16X2) and the adjoint on FP32 MFMA (HDDM_LB_F32); the plain FP64
-step runs both in FP64 with the same work queue, trie walk and zero-skip.
+that the rate does not depend on the batch. The mixed step runs the forward
+and the adjoint on FP32 MFMA (HDDM_LB_F32; the split f16 forward, HDDM_LB_F16X2,
+misses 1e-6 on this workload and is reported beside it, not in the value);
+the plain FP64 step runs both in FP64 with the same work queue, trie walk and
+zero-skip.
 value = points x dofs / mixed step time (point.dof evaluations per second,
 each with its gradient). The sweep formulation and launch shape of every
 forward are the autotuner's choice, reported in config.
@@ -133,8 +135,26 @@ def headline(args):
     dx = torch.empty((n, 59), dtype=torch.float64, device="cuda")
     _, total = ctx.surplus_offset(-1)
     ds = torch.empty((total,), dtype=torch.float64, device="cuda")
-    legs = {"mixed": (hddm_lb.F16X2, hddm_lb.F32), "fp64": (hddm_lb.FP64, hddm_lb.FP64)}
+    legs = {"mixed": (hddm_lb.F32, hddm_lb.F32), "fp64": (hddm_lb.FP64, hddm_lb.FP64)}
     res = {}
+    # oracle references on a seeded subset: y and dx at 16 points per state
+    refs = []
+    if not args.no_errors:
+        oracle = oracle_mod()
+        rng = np.random.default_rng(95)
+        t0 = time.perf_counter()
+        for s, (ii, jj, S) in enumerate(states):
+            pick = rng.choice(np.flatnonzero(sid == s), size=16, replace=False)
+            dxo, _ = oracle.adjoint(ii, jj, S, X[pick], G[pick])
+            refs.append((pick, oracle.forward(ii, jj, S, X[pick]), dxo))
+        say("bench: oracle references in %.1f s" % (time.perf_counter() - t0))
+
+    def errors():
+        Y = y.cpu().numpy()
+        DX = dx.cpu().numpy()
+        return {"y": max(normwise_err(Y[p], yo) for p, yo, _ in refs),
+                "dx": max(normwise_err(DX[p], dxo) for p, _, dxo in refs)}
+
     for name, (pf, pa) in legs.items():
         def step(pf=pf, pa=pa):
             ctx.forward(x, y, sd, pf)
@@ -151,21 +171,12 @@ def headline(args):
         say("bench: D=59 %s step %.1f ms (forward sweep %.1f, adjoint %.1f), config %s"
             % (name, ms, kern[0]["ms_per_step"], kern[1]["ms_per_step"], cfg))
         assert status == 0, "device status %d after the %s leg" % (status, name)
-        if args.no_errors:
-            res[name]["max_rel_err"] = None
-            continue
-        # errors against the oracle on a seeded subset: y and dx at 4 points per state
-        Y = y.cpu().numpy()
-        DX = dx.cpu().numpy()
-        oracle = oracle_mod()
-        rng = np.random.default_rng(95)
-        ey, edx = 0.0, 0.0
-        for s, (ii, jj, S) in enumerate(states):
-            pick = rng.choice(np.flatnonzero(sid == s), size=4, replace=False)
-            ey = max(ey, normwise_err(Y[pick], oracle.forward(ii, jj, S, X[pick])))
-            dxo, _ = oracle.adjoint(ii, jj, S, X[pick], G[pick])
-            edx = max(edx, normwise_err(DX[pick], dxo))
-        res[name]["max_rel_err"] = {"y": ey, "dx": edx}
+        res[name]["max_rel_err"] = None if args.no_errors else errors()
+    # the split f16 forward alone (not in the value: it misses 1e-6 here)
+    ms16, k16 = timed_steps(lambda: ctx.forward(x, y, sd, hddm_lb.F16X2), max(2, args.steps // 4), 1, ctx)
+    res["forward_f16x2"] = {"ms_per_forward": ms16, "sweep_ms": k16[0]["ms_per_step"], "config": ctx.last_config(),
+                            "max_rel_err_y": None if args.no_errors else errors()["y"]}
+    say("bench: D=59 f16x2 forward %.1f ms" % ms16)
     # dS error on a 128-point slice (the oracle's dS cost grows with the points)
     m = 128
     for name, (pf, pa) in (() if args.no_errors else legs.items()):
@@ -197,7 +208,8 @@ def headline(args):
         def fstep():
             ctx.forward(xf, yf, sdf, pf)
             ctx.adjoint(xf, gf, dxf, ds, sdf, pa)
-        fms, _ = timed_steps(fstep, 1, 0, None, label="D=59 mixed, all 2^20 points")
+        # one warm-up step: the first forward of a new size class is tuned
+        fms, _ = timed_steps(fstep, 1, 1, None, label="D=59 mixed, all 2^20 points")
         full = {"points": nf, "ms_per_step": fms, "value": nf * 128 / (fms * 1e-3)}
         del xf, sdf, gf, yf, dxf
     res["full_batch_check"] = full
@@ -444,17 +456,17 @@ def roofline_of(head, work, npoints):
     # executed MFMA work of the adjoint kernel: per 32-node chunk and 32-dof
     # tile, 32 MFMAs for dS and 32 for g.S, 4,096 FLOP each
     exec_flops = float(work["chunks32"]) * 4 * 64 * 4096
-    dominant = "adjoint_ws_kernel (f32)" if adj_ms >= fwd_ms else "forward sweep (f16x2)"
+    dominant = "adjoint_ws_kernel (f32)" if adj_ms >= fwd_ms else "forward sweep (f32)"
     if adj_ms >= fwd_ms:
         t, flops, peak = adj_ms, adj_flops, PEAK_TFLOPS["f32"]
     else:
-        t, flops, peak = fwd_ms, fwd_flops, PEAK_TFLOPS["f16x2"]
+        t, flops, peak = fwd_ms, fwd_flops, PEAK_TFLOPS["f32"]
     achieved = flops / (t * 1e-3) / 1e12
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("adjoint_ws_f32_d59x4" if adj_ms >= fwd_ms else "sweep_f16x2_d59x4")
+            traffic = json.load(f).get("adjoint_ws_f32_d59x4" if adj_ms >= fwd_ms else "sweep_f32_d59x4")
     return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
             "traffic": traffic, "kernel": dominant, "kernel_ms": t, "flops_per_launch": flops,
             "flops_basis": "non-zero (point, node) pairs: %d; per pair and dof 2 FLOP (forward), 4 (adjoint)" % int(nnz),
@@ -514,14 +526,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f16",
+        "dtype": "f32",
         "data": "synthetic (seeded regular grids, surpluses U[-1,1]*2^-sum(l), points U[0,1)^59, gbar N(0,1))",
         "config": {"workload": "multistate forward + adjoint, 4 states, D=59, regular grids sum(l-1) <= 3 "
                                "(%d nodes each), dofs=128, 2^20 points in steps of %d" % (work["nno"], n),
                    "states": 4, "dim": 59, "nno_per_state": work["nno"], "dofs": 128, "points": n,
                    "points_total": 1 << 20,
-                   "forward_precision": "f16x2 (f16 hi/lo MFMA, FP32 accumulate, FP64 fold per chunk)",
-                   "adjoint_precision": "f32 (FP32 MFMA; w summed and carried in FP64)",
+                   "forward_precision": "f32 (FP32 MFMA, FP64 fold per chunk)",
+                   "adjoint_precision": "f32 (FP32 MFMA; w summed over tiles in FP64)",
                    "forward_config": head["mixed"]["forward_config"]},
         "roofline": roofline_of(head, work, n),
         "fp64_plain": {"value": n * 128 / (fp64_ms * 1e-3), "ms_per_step": fp64_ms, "steps": head["fp64"]["steps"],
@@ -532,6 +544,7 @@ def main():
         "max_rel_err": head["mixed"]["max_rel_err"],
         "executed": work,
         "full_batch_check": head["full_batch_check"],
+        "forward_f16x2": head["forward_f16x2"],
     }
     say("bench: headline done")
     if not args.no_secondary:

@@ -679,6 +679,9 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 		ctx->dim, st));
 	HIP_TRY(launch_scatter(npoints, sc.bkey, sc.cursor, sc.perm, st));
 
+	auto& ring = ctx->tring[1];
+	const int tslot = (int)(ring.calls % hddm_lb_ctx::kTimingRing);
+	if (ctx->opt_timing) HIP_TRY(hipEventRecord(ring.e0[tslot], st));
 	bool first = true;
 	for (int m = 0; m < 2 && kerns[m]; m++)
 		for (int sl = 0; sl < ctx->dt_needed; sl++)
@@ -689,5 +692,10 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 			void* params[] = { (void*)&a };
 			HIP_TRY(hipLaunchKernel(kerns[m], dim3(grids[m]), dim3(groups[m] * kWave), params, ldss[m], st));
 		}
+	if (ctx->opt_timing)
+	{
+		HIP_TRY(hipEventRecord(ring.e1[tslot], st));
+		ring.calls++;
+	}
 	return adjoint_tail(ctx, sc, dx, lddx, dsurplus, ds_total, xcd_copies, ns, st);
 }

@@ -1,5 +1,6 @@
 // Autotuner of the sweep's formulation and launch shape, keyed by (precision,
-// D, total nodes, max dofs, log2 of the point count). On the first forward of
+// D, total nodes, max dofs, log2 of the point count) and by the options that
+// restrict the candidates (formulation, waves, slab tiles). On the first forward of
 // a key it times every candidate on the caller's data with hipEvents and
 // keeps the fastest:
 //   * the sweep where every wave walks and contracts its own block, with 2
@@ -32,7 +33,7 @@ int autotune(hddm_lb_ctx* ctx, int64_t npoints, const double* x, int64_t ldx, co
 		if (s && s->loaded) { nodes += s->desc.nno; maxdofs = std::max(maxdofs, s->desc.dofs); }
 	int lg = 0;
 	while (((int64_t)1 << (lg + 1)) <= npoints) lg++;
-	auto key = std::make_tuple(prec, ctx->dim, nodes, maxdofs, lg);
+	auto key = std::make_tuple(prec, ctx->dim, nodes, maxdofs, lg, ctx->opt_form, ctx->opt_waves, ctx->opt_slab);
 	auto hit = ctx->tuned.find(key);
 	if (hit != ctx->tuned.end()) { cfg = hit->second; ctx->tune_hits++; return HDDM_LB_SUCCESS; }
 

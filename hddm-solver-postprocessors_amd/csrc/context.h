@@ -171,7 +171,9 @@ struct hddm_lb_ctx
 	bool last_timed = false;
 
 	// autotuner cache: (prec, dim, total nodes, max dofs, log2 npoints) -> config
-	std::map<std::tuple<int, int, int64_t, int, int>, hddm_lb::TuneConfig> tuned;
+	// (precision, D, total nodes, max dofs, log2 points, the options that
+	// restrict the candidates: formulation, waves, slab tiles)
+	std::map<std::tuple<int, int, int64_t, int, int, int, int, int>, hddm_lb::TuneConfig> tuned;
 	int64_t tune_runs = 0, tune_hits = 0;   // keys tuned, forwards served from the cache
 	hddm_lb::TuneConfig last_cfg;           // configuration of the last eager forward
 

@@ -184,7 +184,8 @@ double hddm_lb_last_sweep_ms(hddm_lb_ctx* ctx);
 int hddm_lb_sweep_time_summary(hddm_lb_ctx* ctx, int64_t* ncalls, double* total_ms,
 	double* min_ms, double* max_ms);
 /* The same for one kind of kernel: 0 the sweep kernels of forwards (as
- * above), 1 the wave-specialized adjoint kernels of hddm_lb_adjoint. */
+ * above), 1 the adjoint kernels of hddm_lb_adjoint (from before the first to
+ * after the last adjoint kernel, the final reductions excluded). */
 int hddm_lb_kernel_time_summary(hddm_lb_ctx* ctx, int kind, int64_t* ncalls, double* total_ms,
 	double* min_ms, double* max_ms);
 /* With HDDM_LB_OPT_STATS set: counts of the last forward, summed over all
