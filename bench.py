@@ -213,16 +213,20 @@ def headline(args):
         full = {"points": nf, "ms_per_step": fms, "value": nf * 128 / (fms * 1e-3)}
         del xf, sdf, gf, yf, dxf
     res["full_batch_check"] = full
-    # executed work: nnz pairs and contraction chunks of one forward on each
-    # formulation's statistics pass (outside the timed steps)
+    # executed work, on statistics passes outside the timed steps: non-zero
+    # (point, node) pairs and walk ops of the wave-specialized F32 forward,
+    # 32-node chunks of the mixed adjoint
     ctx.set_option(hddm_lb.OPT_STATS, 1)
     ctx.set_option(hddm_lb.OPT_FORMULATION, 2)
     ctx.forward(x, y, sd, hddm_lb.F32)
     st = ctx.last_stats(6)
-    ctx.set_option(hddm_lb.OPT_STATS, 0)
     ctx.set_option(hddm_lb.OPT_FORMULATION, 0)
+    ctx.adjoint(x, g, dx, ds, sd, hddm_lb.F32)
+    ast = ctx.last_adjoint_stats(9)
+    ctx.set_option(hddm_lb.OPT_STATS, 0)
     ctx.close()
-    return res, {"nnz": st[5], "walk_ops": st[0], "chunks32": st[2], "items": st[3], "nno": states[0][0].shape[0]}
+    return res, {"nnz": st[5], "walk_ops": st[0], "adjoint_chunks32": ast[8], "adjoint_items": ast[7],
+                 "nno": states[0][0].shape[0]}
 
 
 def single_state_d16(args):
@@ -454,19 +458,24 @@ def roofline_of(head, work, npoints):
     adj_flops = 4.0 * nnz * dofs
     fwd_flops = 2.0 * nnz * dofs
     # executed MFMA work of the adjoint kernel: per 32-node chunk and 32-dof
-    # tile, 32 MFMAs for dS and 32 for g.S, 4,096 FLOP each
-    exec_flops = float(work["chunks32"]) * 4 * 64 * 4096
+    # tile (4 tiles), 32 v_mfma_f32_32x32x2f32 for dS and 32 for g.S, 4,096 FLOP each
+    exec_flops = float(work["adjoint_chunks32"]) * 4 * 64 * 4096
     dominant = "adjoint_ws_kernel (f32)" if adj_ms >= fwd_ms else "forward sweep (f32)"
     if adj_ms >= fwd_ms:
         t, flops, peak = adj_ms, adj_flops, PEAK_TFLOPS["f32"]
     else:
         t, flops, peak = fwd_ms, fwd_flops, PEAK_TFLOPS["f32"]
     achieved = flops / (t * 1e-3) / 1e12
+    # HBM bytes per launch: the rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+    # (calibrated on a copy kernel) of the same kernel on the same workload,
+    # per point, times this launch's points
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("adjoint_ws_f32_d59x4" if adj_ms >= fwd_ms else "sweep_f32_d59x4")
+            tk = json.load(f).get("kernels", {}).get("adjoint_f32_d59x4" if adj_ms >= fwd_ms else "sweep_f32_d59x4")
+        if tk:
+            traffic = tk["bytes_per_point"] * npoints
     return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
             "traffic": traffic, "kernel": dominant, "kernel_ms": t, "flops_per_launch": flops,
             "flops_basis": "non-zero (point, node) pairs: %d; per pair and dof 2 FLOP (forward), 4 (adjoint)" % int(nnz),
@@ -494,6 +503,8 @@ def main():
     ap.add_argument("--latency-calls", type=int, default=2000)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device 3 times (calibrates profiler byte counters)")
+    ap.add_argument("--only", choices=["single_state_d16", "multistate_d20x12", "forward_adjoint_d20"],
+                    help="run one secondary workload alone and print its JSON (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the library has no multi-GPU path (a forward shards by points; "
@@ -510,6 +521,12 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+
+    if args.only:
+        fn = {"single_state_d16": lambda: single_state_d16(args)[0], "multistate_d20x12": lambda: multistate_d20(args),
+              "forward_adjoint_d20": lambda: forward_adjoint_d20(args)}[args.only]
+        print(json.dumps({args.only: fn()}), flush=True)
+        return 0
 
     head, work = headline(args)
     n = args.points
