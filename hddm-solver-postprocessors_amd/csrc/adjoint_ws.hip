@@ -177,13 +177,13 @@ __device__ __forceinline__ double dphi(double u, double iv, double phi)
 // is added to row d_f. The code has no branch: padding slots (i = j = 0)
 // evaluate to phi = 1 and phi' = 0 and add an exact zero to row 0, so two
 // nodes at a time form one block of independent instructions.
-template <int XB, int FS, int CB>
+template <int XB, int FS, int NS, int CB>
 struct DxNodes
 {
 	static constexpr int kC = 1 << CB;
 	static constexpr int kEPL = kC * FS / kWave;   // table entries per lane
 	static_assert(kEPL >= 1, "chunk block smaller than a wave");
-	static_assert(FS % 4 == 0, "factor slots come in fours");
+	static_assert(FS % 4 == 0 && NS <= FS, "factor slots come in fours; NS of them are read");
 	u4_t fv[kEPL];
 
 	__device__ __forceinline__ void load(const u4_t* ftab, int chunk, int lane)
@@ -193,47 +193,57 @@ struct DxNodes
 		for (int e = 0; e < kEPL; e++) fv[e] = src[e];
 	}
 
+	// Node nl: its first NS slots (the state's most factors per node).
 	__device__ __forceinline__ void node(int nl, double w, double* DX, int lane,
 		const d16_t& xv0, const d16_t& xv1, const d16_t& xv2, const d16_t& xv3)
 	{
-		uint32_t d2[FS];
-		double phi[FS], dp[FS];
+		uint32_t d2[NS];
+		double phi[NS], dp[NS];
 #pragma unroll
-		for (int f0 = 0; f0 < FS; f0 += 4)
+		for (int f0 = 0; f0 < NS; f0 += 4)
 		{
-			uint32_t ih[4], jh[4];
+			uint32_t ih[4], jh[4], di[4];
 #pragma unroll
 			for (int k = 0; k < 4; k++)
 			{
 				const int f = f0 + k;
-				const int src = nl * (kWave / kC) + f / kEPL;
-				const u4_t& v = fv[f % kEPL];
-				d2[f] = __builtin_amdgcn_readlane(v[0], src);
-				ih[k] = __builtin_amdgcn_readlane(v[1], src);
-				jh[k] = __builtin_amdgcn_readlane(v[2], src);
+				if (f < NS)
+				{
+					const int src = nl * (kWave / kC) + f / kEPL;
+					const u4_t& v = fv[f % kEPL];
+					d2[f] = di[k] = __builtin_amdgcn_readlane(v[0], src);
+					ih[k] = __builtin_amdgcn_readlane(v[1], src);
+					jh[k] = __builtin_amdgcn_readlane(v[2], src);
+				}
+				else
+				{
+					di[k] = di[0];
+					ih[k] = jh[k] = 0u;
+				}
 			}
 			double xs[4];
-			xpin4<XB>(xv0, xv1, xv2, xv3, d2[f0], d2[f0 + 1], d2[f0 + 2], d2[f0 + 3], xs[0], xs[1], xs[2], xs[3]);
+			xpin4<XB>(xv0, xv1, xv2, xv3, di[0], di[1], di[2], di[3], xs[0], xs[1], xs[2], xs[3]);
 #pragma unroll
 			for (int k = 0; k < 4; k++)
 			{
+				if (f0 + k >= NS) continue;
 				const double iv = f64_from_high(ih[k]);
 				const double u = fma(xs[k], iv, f64_from_high(jh[k]));
 				phi[f0 + k] = fmax(1.0 - fabs(u), 0.0);
 				dp[f0 + k] = dphi(u, iv, phi[f0 + k]);
 			}
 		}
-		double suf[FS];
+		double suf[NS];
 		double acc = w;
 #pragma unroll
-		for (int f = FS - 1; f >= 0; f--)
+		for (int f = NS - 1; f >= 0; f--)
 		{
 			suf[f] = acc;
 			acc *= phi[f];
 		}
 		double pre = 1.0;
 #pragma unroll
-		for (int f = 0; f < FS; f++)
+		for (int f = 0; f < NS; f++)
 		{
 			lds_add(DX + (d2[f] >> 1) * kWave + lane, pre * suf[f] * dp[f]);
 			pre *= phi[f];
@@ -572,7 +582,7 @@ __global__ void __launch_bounds__(64 * (kANC + 3)) adjoint_ws_kernel(AdjointArgs
 		int nchunks = 0;
 		// the item whose rows DX holds
 		bool have = false;
-		int np = 0, seg = 0, tpass = 0, fs = 4;
+		int np = 0, seg = 0, tpass = 0, fs = 4, nfac = 4;
 		int64_t count = 0, lpos = 0;
 		uint32_t orig = 0;
 		const StateDesc* sd = nullptr;
@@ -682,12 +692,19 @@ __global__ void __launch_bounds__(64 * (kANC + 3)) adjoint_ws_kernel(AdjointArgs
 				{
 					const int chunk = uniform(m->a);
 					const uint32_t mask = (uint32_t)uniform(m->b);
+					// the state's table stride (4, 8, 16 slots) and, below 4,
+					// the slots its nodes use
 					bool ok = true;
-					if (fs == 4) ok = run_chunk(DxNodes<XB, 4, CB>(), chunk, mask, par, n);
-					else if (fs == 8) ok = run_chunk(DxNodes<XB, 8, CB>(), chunk, mask, par, n);
+					if (fs == 4)
+					{
+						if (nfac <= 2) ok = run_chunk(DxNodes<XB, 4, 2, CB>(), chunk, mask, par, n);
+						else if (nfac == 3) ok = run_chunk(DxNodes<XB, 4, 3, CB>(), chunk, mask, par, n);
+						else ok = run_chunk(DxNodes<XB, 4, 4, CB>(), chunk, mask, par, n);
+					}
+					else if (fs == 8) ok = run_chunk(DxNodes<XB, 8, 8, CB>(), chunk, mask, par, n);
 					// 16 factor slots next to four coordinate banks would spill:
 					// the host runs such states on the per-tile kernels
-					else if constexpr (XB < 4) ok = run_chunk(DxNodes<XB, 16, CB>(), chunk, mask, par, n);
+					else if constexpr (XB < 4) ok = run_chunk(DxNodes<XB, 16, 16, CB>(), chunk, mask, par, n);
 					if (!ok) break;
 				}
 			}
@@ -706,6 +723,7 @@ __global__ void __launch_bounds__(64 * (kANC + 3)) adjoint_ws_kernel(AdjointArgs
 				split = tpass <= kANC / 2;
 				ftab = (const u4_t*)ldc(&sd->ftab);
 				fs = (int)ldc(&sd->fstride);
+				nfac = ldc(&sd->max_depth);
 				have = true;
 				if (want_dx)
 				{
