@@ -362,6 +362,14 @@ extern "C" int hddm_lb_layout_order(const hddm_lb_layout* l, int32_t* order)
 	return HDDM_LB_SUCCESS;
 }
 
+extern "C" int hddm_lb_layout_factors(const hddm_lb_layout* l, uint32_t* ftab, int* fstride)
+{
+	if (!l) return hddm_lb::set_error(HDDM_LB_ERROR_INVALID_VALUE, "layout_factors: NULL layout");
+	if (fstride) *fstride = l->L.fstride;
+	if (ftab) std::memcpy(ftab, l->L.ftab.data(), l->L.ftab.size() * sizeof(uint32_t));
+	return HDDM_LB_SUCCESS;
+}
+
 extern "C" int hddm_lb_layout_free(hddm_lb_layout* l)
 {
 	delete l;

@@ -105,6 +105,7 @@ _SIGS = {
     "hddm_lb_layout_info": (_i32, [_vp] + [_c.POINTER(_i32)] * 4),
     "hddm_lb_layout_ops": (_i32, [_vp, _vp, _vp, _vp]),
     "hddm_lb_layout_order": (_i32, [_vp, _vp]),
+    "hddm_lb_layout_factors": (_i32, [_vp, _vp, _c.POINTER(_i32)]),
     "hddm_lb_layout_free": (_i32, [_vp]),
 }
 for _name, (_res, _args) in _SIGS.items():
@@ -250,6 +251,12 @@ class Layout:
         self.op_negj = (ops[:, 3].astype(np.uint64) << np.uint64(32)).view(np.float64)
         self.order = np.zeros(nno, np.int32)
         _check(lib.hddm_lb_layout_order(h, _ptr(self.order)))
+        fs = _i32()
+        _check(lib.hddm_lb_layout_factors(h, None, _c.byref(fs)))
+        self.fstride = fs.value
+        ft = np.zeros(((nno + 31) // 32 * 32, self.fstride, 4), np.uint32)
+        _check(lib.hddm_lb_layout_factors(h, _ptr(ft), None))
+        self.ftab = ft
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -239,6 +239,11 @@ int hddm_lb_layout_info(const hddm_lb_layout* layout, int* max_factors, int* nop
 int hddm_lb_layout_ops(const hddm_lb_layout* layout, uint32_t* ops, uint32_t* skip, int32_t* opstart);
 /* order[n]: uploaded index of the node at trie position n */
 int hddm_lb_layout_order(const hddm_lb_layout* layout, int32_t* order);
+/* The per-node factor table the adjoint's dx terms read: fstride (4, 8 or 16)
+ * slots of 4 words per node in trie order, nodes rounded up to 32; a slot is
+ * {2 * dim, high word of i, high word of -j, 1}, padding slots are zero.
+ * ftab may be NULL (stride only). */
+int hddm_lb_layout_factors(const hddm_lb_layout* layout, uint32_t* ftab, int* fstride);
 int hddm_lb_layout_free(hddm_lb_layout* layout);
 
 /* ---- .plt surplus files (LB/gold/src/Data.cpp:13-595) ---- */

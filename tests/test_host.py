@@ -313,6 +313,53 @@ def test_adjoint_edge_sums_match_oracle(dim, start, level, target, seed):
     np.testing.assert_allclose(ds, dso, rtol=0, atol=1e-13 * np.abs(dso).max())
 
 
+def flat_adjoint_dx(L, X, G, S_sorted):
+    """dx as the wave-specialized adjoint's dx waves form it: for every node
+    (trie order) its factors from the factor table, w = g . S, and
+    w * (product of the other factors) * phi' added to each factor's dim."""
+    nno = L.nno
+    ft = L.ftab[:nno]
+    d = (ft[:, :, 0] >> 1).astype(np.int64)
+    iv = (ft[:, :, 1].astype(np.uint64) << np.uint64(32)).view(np.float64)
+    nj = (ft[:, :, 2].astype(np.uint64) << np.uint64(32)).view(np.float64)
+    on = ft[:, :, 3] != 0
+    u = X[:, d] * iv[None] + nj[None]                     # [points, nodes, slots]
+    phi = np.maximum(1.0 - np.abs(u), 0.0)
+    dp = np.where((phi > 0) & (u != 0), np.where(u > 0, -iv[None], iv[None]), 0.0)
+    phi = np.where(on[None], phi, 1.0)
+    dp = np.where(on[None], dp, 0.0)
+    w = G @ S_sorted.T                                     # [points, nodes]
+    dx = np.zeros_like(X)
+    for f in range(L.fstride):
+        other = np.prod(np.delete(phi, f, axis=2), axis=2)
+        term = w * other * dp[:, :, f]
+        for p in range(X.shape[0]):
+            np.add.at(dx[p], d[:, f], term[p])
+    return dx, np.prod(phi, axis=2)
+
+
+@pytest.mark.parametrize("dim,start,level,target,seed", [(3, 3, 4, 0, 14), (7, 2, 5, 1500, 15), (59, 3, 4, 0, 16)])
+def test_factor_table_dx_matches_oracle(dim, start, level, target, seed):
+    """The factor table holds each node's factors in the op encoding (the
+    basis it gives equals the oracle's), and the leave-one-out dx of the dx
+    waves equals the oracle's adjoint."""
+    ii, jj, lv, _ = hddm_lb.generate_grid(dim, start, level, 0.4, seed, target)
+    S = hddm_lb.synthetic_surplus(lv, 4, seed)
+    L = hddm_lb.Layout(ii, jj, S)
+    nact = ((ii != 0) | (jj != 0)).sum(axis=1).max()
+    assert L.fstride == (4 if nact <= 4 else 8 if nact <= 8 else 16)
+    assert L.ftab.shape[0] % 32 == 0 and not L.ftab[L.nno:].any()
+    n = 16 if dim > 20 else 48
+    X = hddm_lb.synthetic_points(n, dim, seed, 0.01, 0.99, kink_level=level, kink_gap=1e-3)
+    G = np.random.default_rng(seed).standard_normal((n, 4))
+    dx, B = flat_adjoint_dx(L, X, G, S[L.order])
+    y = B @ S[L.order]
+    yo = oracle.forward(ii, jj, S, X)
+    np.testing.assert_allclose(y, yo, rtol=0, atol=1e-13 * np.abs(yo).max())
+    dxo, _ = oracle.adjoint(ii, jj, S, X, G)
+    np.testing.assert_allclose(dx, dxo, rtol=0, atol=1e-11 * np.abs(dxo).max())
+
+
 def test_layout_rejects_bad_input():
     with pytest.raises(hddm_lb.HddmLbError) as e:
         hddm_lb.Layout(np.zeros((0, 3), np.int32), np.zeros((0, 3), np.int32), np.zeros((0, 2)))
