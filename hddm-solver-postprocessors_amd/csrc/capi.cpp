@@ -205,6 +205,7 @@ extern "C" int hddm_lb_set_option(hddm_lb_ctx* ctx, int option, int64_t value)
 		if (value < 0 || value > 2) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "set_option: walk must be 0, 1 or 2");
 		ctx->opt_walk = (int)value; break;
 	case HDDM_LB_OPT_COALESCE: ctx->opt_coalesce = value != 0; break;
+	case HDDM_LB_OPT_DEVICE_LAYOUT: ctx->opt_device_layout = value != 0; break;
 	case HDDM_LB_OPT_FORMULATION:
 		if (value < 0 || value > 2) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "set_option: formulation must be 0, 1 or 2");
 		ctx->opt_form = (int)value; break;
@@ -213,6 +214,8 @@ extern "C" int hddm_lb_set_option(hddm_lb_ctx* ctx, int option, int64_t value)
 	}
 	return HDDM_LB_SUCCESS;
 }
+
+static int finish_upload(hddm_lb_ctx* ctx, int istate, int dim, std::unique_ptr<StateSlot> slot);
 
 extern "C" int hddm_lb_upload_state(hddm_lb_ctx* ctx, int istate, int dim, int nno, int dofs,
 	const int32_t* index_i, const int32_t* index_j, int64_t ld_index,
@@ -230,12 +233,48 @@ extern "C" int hddm_lb_upload_state(hddm_lb_ctx* ctx, int istate, int dim, int n
 	if (xbanks_for(dim) == 0)
 		return set_error(HDDM_LB_ERROR_UNSUPPORTED, "upload_state: dim %d exceeds 64 (x is held in registers)", dim);
 
-	HostLayout L;
-	int rc = build_layout(dim, nno, dofs, index_i, index_j, ld_index, surplus, ld_surplus, L);
-	if (rc) return rc;
-
 	DeviceGuard g(ctx->device);
 	std::unique_ptr<StateSlot> slot(new StateSlot);
+	int rc = HDDM_LB_SUCCESS;
+	StateDesc& d = slot->desc;
+	std::memset(&d, 0, sizeof(d));
+	d.dim = dim; d.nno = nno; d.dofs = dofs;
+	d.nseg = 1;
+	// The layout is built on the device unless HDDM_LB_OPT_DEVICE_LAYOUT is 0;
+	// inputs the device build does not take go to the host build, which builds
+	// them or reports the error.
+	if (ctx->opt_device_layout)
+	{
+		DeviceLayout DL;
+		auto alloc = [&](size_t bytes, int& arc) -> void* {
+			if (arc) return nullptr;
+			void* p = dev_alloc(bytes, arc);
+			if (p) slot->bufs.push_back(p);
+			return p;
+		};
+		rc = build_layout_device(dim, nno, dofs, index_i, index_j, ld_index, surplus, ld_surplus, alloc, DL);
+		if (rc != kLayoutHostCheck)
+		{
+			if (rc) return rc;
+			d.dofs_pad = DL.dofs_pad; d.nops = DL.nops; d.nchunks = DL.nchunks; d.max_depth = DL.fmax;
+			d.ops = DL.ops; d.skip = DL.skip; d.s64 = DL.s64; d.s16 = DL.s16; d.sbf = DL.sbf; d.s32 = DL.s32;
+			d.s32r = DL.s32r; d.ftab = DL.ftab; d.fstride = DL.fstride; d.iscale = DL.iscale; d.order = DL.order;
+			d.lomask = DL.lomask; d.himask = DL.himask;
+			slot->shared = std::move(DL.shared);
+			slot->opstart = std::move(DL.opstart);
+			slot->d_seg_op = (int32_t*)alloc(sizeof(int32_t) * ((size_t)nno + 1), rc);
+			d.seg_op = slot->d_seg_op;
+			if (rc) return rc;
+			return finish_upload(ctx, istate, dim, std::move(slot));
+		}
+		for (void* p : slot->bufs) dev_free(p);
+		slot->bufs.clear();
+		rc = HDDM_LB_SUCCESS;
+	}
+
+	HostLayout L;
+	rc = build_layout(dim, nno, dofs, index_i, index_j, ld_index, surplus, ld_surplus, L);
+	if (rc) return rc;
 	auto up = [&](const void* src, size_t bytes) -> void* {
 		if (rc) return nullptr;
 		void* p = dev_alloc(bytes, rc);
@@ -248,11 +287,8 @@ extern "C" int hddm_lb_upload_state(hddm_lb_ctx* ctx, int istate, int dim, int n
 		}
 		return p;
 	};
-	StateDesc& d = slot->desc;
-	std::memset(&d, 0, sizeof(d));
-	d.dim = dim; d.nno = nno; d.dofs = dofs; d.dofs_pad = L.dofs_pad;
+	d.dofs_pad = L.dofs_pad;
 	d.nops = (int)L.skip.size() - 1; d.nchunks = L.nchunks; d.max_depth = L.fmax;
-	d.nseg = 1;
 	// device copies carry kOpPad sentinels so that read-ahead stays in bounds
 	std::vector<Op> ops_pad(L.ops);
 	std::vector<uint32_t> skip_pad(L.skip);
@@ -276,8 +312,13 @@ extern "C" int hddm_lb_upload_state(hddm_lb_ctx* ctx, int istate, int dim, int n
 	slot->shared = std::move(L.shared);
 	slot->opstart = std::move(L.opstart);
 	if (rc) return rc;
-	slot->loaded = true;
+	return finish_upload(ctx, istate, dim, std::move(slot));
+}
 
+// Installs a built state in its slot and rebuilds the descriptor tables.
+static int finish_upload(hddm_lb_ctx* ctx, int istate, int dim, std::unique_ptr<StateSlot> slot)
+{
+	slot->loaded = true;
 	if ((size_t)istate >= ctx->states.size()) ctx->states.resize(istate + 1);
 	ctx->states[istate] = std::move(slot);
 	ctx->dim = dim;
@@ -287,6 +328,44 @@ extern "C" int hddm_lb_upload_state(hddm_lb_ctx* ctx, int istate, int dim, int n
 	// the descriptor tables are built now (not at the next forward), so that
 	// the allocation-free forward finds them ready
 	return prepare_tables(ctx);
+}
+
+extern "C" int hddm_lb_state_array(hddm_lb_ctx* ctx, int istate, int array, void* dst, uint64_t* bytes)
+{
+	if (!ctx || !bytes) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "state_array: NULL argument");
+	if (istate < 0 || (size_t)istate >= ctx->states.size() || !ctx->states[istate] || !ctx->states[istate]->loaded)
+		return set_error(HDDM_LB_ERROR_STATE, "state_array: state %d is not loaded", istate);
+	const StateDesc& d = ctx->states[istate]->desc;
+	const size_t nno = (size_t)d.nno, DP = (size_t)d.dofs_pad, NC = (size_t)d.nchunks, nops = (size_t)d.nops + kOpPad;
+	const void* src = nullptr;
+	size_t n = 0;
+	switch (array)
+	{
+	case 0: src = d.ops; n = nops * sizeof(Op); break;
+	case 1: src = d.skip; n = nops * sizeof(uint32_t); break;
+	case 2: src = d.order; n = nno * sizeof(int32_t); break;
+	case 3: src = d.s64; n = nno * DP * sizeof(double); break;
+	case 4: src = d.s16; n = NC * 2 * DP * kChunk * sizeof(uint16_t); break;
+	case 5: src = d.sbf; n = NC * 3 * DP * kChunk * sizeof(uint16_t); break;
+	case 6: src = d.s32; n = NC * DP * kChunk * sizeof(float); break;
+	case 7: src = d.s32r; n = nno * DP * sizeof(float); break;
+	case 8: src = d.iscale; n = DP * sizeof(double); break;
+	case 9: src = d.lomask; n = nno * sizeof(uint64_t); break;
+	case 10: src = d.himask; n = nno * sizeof(uint64_t); break;
+	case 11: src = d.ftab; n = ((nno + 31) & ~(size_t)31) * (size_t)d.fstride * 4 * sizeof(uint32_t); break;
+	default: return set_error(HDDM_LB_ERROR_INVALID_VALUE, "state_array: array %d unknown (0 to 11)", array);
+	}
+	if (dst)
+	{
+		if (*bytes < n)
+			return set_error(HDDM_LB_ERROR_INVALID_VALUE, "state_array: %zu bytes needed, %llu given", n,
+				(unsigned long long)*bytes);
+		DeviceGuard g(ctx->device);
+		HIP_TRY(hipDeviceSynchronize());
+		HIP_TRY(hipMemcpy(dst, src, n, hipMemcpyDeviceToHost));
+	}
+	*bytes = n;
+	return HDDM_LB_SUCCESS;
 }
 
 extern "C" int hddm_lb_state_info(hddm_lb_ctx* ctx, int istate, int* dim, int* nno, int* dofs, int* max_factors)

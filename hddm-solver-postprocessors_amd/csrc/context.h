@@ -134,6 +134,7 @@ struct hddm_lb_ctx
 	std::vector<hddm_lb::HostRequest*> comb_queue;
 	bool comb_busy = false;
 	bool opt_coalesce = true;
+	bool opt_device_layout = true;   // build uploaded layouts on the device
 	int64_t comb_requests = 0, comb_batches = 0;
 	// Eager calls share `scratch`: each one records scratch_event on its
 	// stream, and the next call on another stream waits for it first.

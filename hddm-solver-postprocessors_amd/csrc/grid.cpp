@@ -140,7 +140,9 @@ extern "C" int hddm_lb_grid_generate(int dim, int start_sum, int max_level, doub
 
 	SplitMix64 rng(seed);
 	const int limit = max_level - 1;
-	bool done = target_nno > 0 && (int64_t)g->nodes.size() >= target_nno;
+	// With p = 0 no node is ever refined: the grid is the regular start grid
+	// (and the passes below would never run out of candidates).
+	bool done = p_refine <= 0.0 || (target_nno > 0 && (int64_t)g->nodes.size() >= target_nno);
 	while (!done)
 	{
 		bool any = false;

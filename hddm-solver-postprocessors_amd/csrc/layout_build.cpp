@@ -128,7 +128,7 @@ int build_layout(int dim, int nno, int dofs,
 			uint32_t h;
 			if (!f64_high_word(i, h) || !f64_high_word(-(int64_t)j, h))
 				return set_error(HDDM_LB_ERROR_UNSUPPORTED,
-					"upload_state: node %d dim %d has (i, j) = (%d, %d); |i|, |j| < 2^21 are supported", k, d, i, j);
+					"upload_state: node %d dim %d has (i, j) = (%d, %d); supported are values whose FP64 low word is zero (every |v| < 2^21)", k, d, i, j);
 			seq[k].push_back(FactorKey{ d, i, j });
 		}
 		fmax = std::max(fmax, (int)seq[k].size());

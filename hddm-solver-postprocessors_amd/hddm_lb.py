@@ -27,6 +27,9 @@ PRECISIONS = {"fp64": FP64, "f16x2": F16X2, "bf16x3": BF16X3, "f32": F32}
 
 OPT_AUTOTUNE, OPT_SEGMENT_NODES, OPT_WAVES_PER_WG, OPT_SORT_POINTS = 1, 2, 3, 4
 OPT_TIMING, OPT_SLAB_TILES, OPT_STATS, OPT_WALK, OPT_FORMULATION, OPT_COALESCE = 5, 6, 8, 9, 10, 11
+OPT_DEVICE_LAYOUT = 12
+# state arrays readable with Context.state_array (hddm_lb_state_array)
+STATE_ARRAYS = ("ops", "skip", "order", "s64", "s16", "sbf", "s32", "s32r", "iscale", "lomask", "himask", "ftab")
 STATUS_WAIT_LIMIT = 1
 
 SUCCESS = 0
@@ -75,6 +78,7 @@ _SIGS = {
     "hddm_lb_set_option": (_i32, [_vp, _i32, _i64]),
     "hddm_lb_upload_state": (_i32, [_vp, _i32, _i32, _i32, _i32, _vp, _vp, _i64, _vp, _i64]),
     "hddm_lb_state_info": (_i32, [_vp, _i32, _c.POINTER(_i32), _c.POINTER(_i32), _c.POINTER(_i32), _c.POINTER(_i32)]),
+    "hddm_lb_state_array": (_i32, [_vp, _i32, _i32, _vp, _c.POINTER(_c.c_uint64)]),
     "hddm_lb_forward": (_i32, [_vp, _i64, _vp, _i64, _vp, _vp, _i64, _i32, _vp]),
     "hddm_lb_forward_host": (_i32, [_vp, _i64, _vp, _i64, _vp, _vp, _i64, _i32]),
     "hddm_lb_forward_noalloc": (_i32, [_vp, _i64, _vp, _i64, _vp, _vp, _i64, _i32, _vp]),
@@ -309,6 +313,15 @@ class Context:
         v = [_i32() for _ in range(4)]
         _check(lib.hddm_lb_state_info(self._h, istate, *[_c.byref(a) for a in v]))
         return dict(zip(("dim", "nno", "dofs", "max_factors"), [a.value for a in v]))
+
+    def state_array(self, istate, name):
+        """Bytes (uint8 array) of one device array of an uploaded state's layout."""
+        which = STATE_ARRAYS.index(name)
+        n = _c.c_uint64(0)
+        _check(lib.hddm_lb_state_array(self._h, istate, which, None, _c.byref(n)))
+        out = np.zeros(n.value, np.uint8)
+        _check(lib.hddm_lb_state_array(self._h, istate, which, _ptr(out), _c.byref(n)))
+        return out
 
     def forward(self, x, y, state_ids=None, precision=FP64, stream=None):
         """x: device FP64 [n, >=dim]; y: device FP64 [n, >=dofs]; state_ids: device int32 [n] or None."""

@@ -70,6 +70,9 @@ typedef struct hddm_lb_grid hddm_lb_grid;
                                         waves (wave-specialized kernels) */
 #define HDDM_LB_OPT_COALESCE     11  /* 1 (default): hddm_lb_forward_host calls of at most 64 points made
                                         at the same time from several threads run as one batched forward */
+#define HDDM_LB_OPT_DEVICE_LAYOUT 12 /* 1 (default): hddm_lb_upload_state builds the node layout (trie
+                                        order, op stream, surplus splits, factor table) on the device,
+                                        bit for bit as the host build; 0: on the host */
 
 /* bits of the device status word (hddm_lb_device_status) */
 #define HDDM_LB_STATUS_WAIT_LIMIT 1u /* a wave of a wave-specialized kernel stopped waiting for another
@@ -106,6 +109,13 @@ int hddm_lb_upload_state(hddm_lb_ctx* ctx, int istate, int dim, int nno, int dof
 	const int32_t* index_i, const int32_t* index_j, int64_t ld_index,
 	const double* surplus, int64_t ld_surplus);
 
+/* Copy of one device array of an uploaded state's layout (for checks): 0 op
+ * stream (with its sentinels), 1 skip, 2 trie order, 3 FP64 surplus rows,
+ * 4 f16 hi/lo chunks, 5 bf16 chunks, 6 f32 chunks, 7 FP32 surplus rows,
+ * 8 f16 column scales, 9 and 10 support masks, 11 factor table; in the layouts
+ * hddm_lb_layout_* describe. With dst NULL, *bytes receives the size; else
+ * *bytes (at least the size) bytes of dst are available and receive it. */
+int hddm_lb_state_array(hddm_lb_ctx* ctx, int istate, int array, void* dst, uint64_t* bytes);
 int hddm_lb_state_info(hddm_lb_ctx* ctx, int istate,
 	int* dim, int* nno, int* dofs, int* max_factors);
 

@@ -116,6 +116,15 @@ def test_regular_grid_counts():
     assert g[0].shape[0] == 281077
 
 
+def test_no_refinement_keeps_start_grid():
+    """p = 0 below the level cap returns the regular start grid: D=4, sum(l-1)
+    <= 2 has 1 + 4*2 + 4*2 + 6*4 = 41 nodes, whatever the cap and target."""
+    for maxl, target in ((4, 0), (6, 0), (6, 10 ** 6)):
+        ii, jj, lv, ix = hddm_lb.generate_grid(4, 2, maxl, 0.0, 41, target)
+        assert ii.shape == (41, 4)
+        assert int((lv - 1).sum(axis=1).max()) == 2
+
+
 def test_adaptive_grid_properties():
     a = hddm_lb.generate_grid(16, 2, 7, 0.3, 1, 60000)
     b = hddm_lb.generate_grid(16, 2, 7, 0.3, 1, 60000)
