@@ -24,6 +24,8 @@
 #include "context.h"
 #include "errors.h"
 
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
 #include <cstring>
 
@@ -54,14 +56,18 @@ constexpr int kXcdCopies = 8;
 // dS is accumulated in place.
 constexpr int64_t kXcdCopiesMaxElems = (int64_t)1 << 27;
 
-// out[i] = sum of the kXcdCopies copies of element i, in copy order.
-__global__ void sum_ds_copies_kernel(const double* __restrict__ part, int64_t n, double* __restrict__ out)
+// Owned units of the wave-specialized adjoint: the dS copies they may use,
+// all together at most this many bytes (and an eighth of device memory).
+constexpr int kOwnMaxCopies = 32;
+constexpr int64_t kOwnMaxBytes = (int64_t)16 << 30;
+
+// out[i] = sum of the ncopies copies of element i, in copy order.
+__global__ void sum_ds_copies_kernel(const double* __restrict__ part, int64_t n, int ncopies, double* __restrict__ out)
 {
 	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
 	{
 		double v = 0.0;
-#pragma unroll
-		for (int c = 0; c < kXcdCopies; c++) v += part[c * n + i];
+		for (int c = 0; c < ncopies; c++) v += part[c * n + i];
 		out[i] = v;
 	}
 }
@@ -473,26 +479,101 @@ using namespace hddm_lb;
 
 #define HIP_TRY(expr) do { hipError_t e_ = (expr); if (e_ != hipSuccess) return hip_fail(e_, #expr, __FILE__, __LINE__); } while (0)
 
-// After the adjoint kernels: dx of split states summed over node segments,
-// dS summed over the per-XCD copies, both in a fixed order.
-static int adjoint_tail(hddm_lb_ctx* ctx, Scratch& sc, double* dx, int64_t lddx, double* dsurplus,
-	int64_t ds_total, bool xcd_copies, int ns, hipStream_t st)
+// After the adjoint kernels: dx of split states summed over node segments
+// (of the descriptor table the kernels used), dS summed over its copies
+// (ncopies > 1), both in a fixed order.
+static int adjoint_tail(hddm_lb_ctx* ctx, Scratch& sc, const StateDesc* tables, int max_nseg, double* dx,
+	int64_t lddx, double* dsurplus, int64_t ds_total, int ncopies, int ns, hipStream_t st)
 {
-	if (dx && ctx->max_nseg > 1)
+	if (dx && max_nseg > 1)
 	{
 		hipLaunchKernelGGL(reduce_dx_partials_kernel, dim3(1024, ns), dim3(64, 4), 0, st,
-			(const StateDesc*)ctx->d_states, (const StatePlan*)sc.plans, (const uint32_t*)sc.perm,
+			tables, (const StatePlan*)sc.plans, (const uint32_t*)sc.perm,
 			(const double*)sc.dxpart, ctx->dim, dx, lddx);
 		HIP_TRY(hipGetLastError());
 	}
-	if (xcd_copies)
+	if (ncopies > 1)
 	{
 		int blocks = (int)std::min<int64_t>(4096, (ds_total + 255) / 256);
 		hipLaunchKernelGGL(sum_ds_copies_kernel, dim3(std::max(blocks, 1)), dim3(256), 0, st,
-			(const double*)sc.dspart, ds_total, dsurplus);
+			(const double*)sc.dspart, ds_total, ncopies, dsurplus);
 		HIP_TRY(hipGetLastError());
 	}
 	return scratch_mark(ctx, st);
+}
+
+// dS copies K (a power of two) for the owned units of the wave-specialized
+// adjoint: the fewest that give each workgroup of the grid about four units
+// ((state, segment, copy) triples), within the byte limits; K = 1 adds to
+// dsurplus itself. 0 (atomics) when the limits leave fewer than two units
+// per workgroup.
+static int owned_copies(const hddm_lb_ctx* ctx, int64_t ds_total, int grid)
+{
+	const int64_t segs = ctx->nseg_adj_total;
+	if (!ctx->opt_adj_owned || segs <= 0 || ds_total <= 0 || grid <= 0) return 0;
+	const int64_t cap = std::min<int64_t>(kOwnMaxBytes, (int64_t)(ctx->total_mem / 8));
+	const int64_t copy_bytes = ds_total * (int64_t)sizeof(double);
+	int K = 1;
+	while (K < kOwnMaxCopies && segs * K < 4 * (int64_t)grid && 2 * K * copy_bytes <= cap) K *= 2;
+	return segs * K >= 2 * (int64_t)grid ? K : 0;
+}
+
+__global__ void iota_u32_kernel(int64_t n, uint32_t* __restrict__ p)
+{
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+		p[i] = (uint32_t)i;
+}
+
+template <typename T>
+static int grow_buf(T*& p, size_t& cap, size_t need)
+{
+	if (need <= cap && p) return HDDM_LB_SUCCESS;
+	if (p) HIP_TRY(hipDeviceSynchronize());
+	dev_free(p);
+	p = nullptr;
+	cap = 0;
+	int rc = 0;
+	p = (T*)dev_alloc(sizeof(T) * std::max<size_t>(need, 1), rc);
+	if (rc) return rc;
+	cap = std::max<size_t>(need, 1);
+	return HDDM_LB_SUCCESS;
+}
+
+// sc.perm = the points ordered by bucket key and, within a bucket, by index:
+// a stable radix sort of the keys (the forward's atomic scatter orders a
+// bucket by arrival). Which points share a 64-point block then does not vary
+// between runs, so neither do the adjoint's per-block dS partials.
+static int stable_order(Scratch& sc, int64_t npoints, int64_t nbuckets, hipStream_t st)
+{
+	int bits = 1;
+	while (bits < 32 && ((int64_t)1 << bits) < nbuckets) bits++;
+	const int n = (int)npoints;
+	int rc;
+	if ((rc = grow_buf(sc.bkey2, sc.cap_bkey2, (size_t)npoints))) return rc;
+	if ((rc = grow_buf(sc.pidx, sc.cap_pidx, (size_t)npoints))) return rc;
+	size_t tmp = 0;
+	HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, sc.bkey, sc.bkey2, sc.pidx, sc.perm, n, 0, bits, st));
+	if ((rc = grow_buf(sc.sort_tmp, sc.cap_sort_tmp, tmp))) return rc;
+	const int blocks = (int)std::min<int64_t>(4096, (npoints + 255) / 256);
+	hipLaunchKernelGGL(iota_u32_kernel, dim3(std::max(blocks, 1)), dim3(256), 0, st, npoints, sc.pidx);
+	HIP_TRY(hipGetLastError());
+	HIP_TRY(hipcub::DeviceRadixSort::SortPairs(sc.sort_tmp, tmp, sc.bkey, sc.bkey2, sc.pidx, sc.perm, n, 0, bits, st));
+	return HDDM_LB_SUCCESS;
+}
+
+// Grows the dS copy scratch to `need` elements.
+static int grow_dspart(Scratch& scr, size_t need)
+{
+	if (need <= scr.cap_dspart) return HDDM_LB_SUCCESS;
+	if (scr.dspart) HIP_TRY(hipDeviceSynchronize());
+	dev_free(scr.dspart);
+	scr.dspart = nullptr;
+	scr.cap_dspart = 0;
+	int arc = 0;
+	scr.dspart = (double*)dev_alloc(need * sizeof(double), arc);
+	if (arc) return arc;
+	scr.cap_dspart = need;
+	return HDDM_LB_SUCCESS;
 }
 
 extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
@@ -520,35 +601,72 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 	if ((rc = scratch_wait(ctx, st))) return rc;
 	int64_t ds_total = 0;
 	if ((rc = hddm_lb_surplus_offset(ctx, -1, nullptr, &ds_total))) return rc;
+
+	// automatic: the wave-specialized kernel for wide states (D > 32), which
+	// walks each trie once instead of once per dof tile (measured: D=59, 128
+	// dofs, 2^16 points: FP64 1255 -> 838 ms; its F32 consumers take dS to the
+	// MFMA units); the per-tile kernels otherwise (D=20, 32 dofs: FP64 32 ms
+	// against 110 ms wave-specialized, F32 105 ms). The per-tile kernels compute
+	// in FP64 whatever the precision asked for, which meets the F32 bound too.
+	const bool wide = xbanks_for(ctx->dim) == 4;
+	// nodes of more than 8 factors next to four coordinate banks exceed the
+	// wave-specialized kernel's registers
+	const bool ws_fits = !(wide && ctx->fmax_all > 8);
+	if (ctx->opt_form == 2 && !ws_fits)
+		return set_error(HDDM_LB_ERROR_UNSUPPORTED,
+			"adjoint: the wave-specialized kernel takes nodes of at most 8 factors when dim > 32");
+	const bool ws = ctx->opt_form == 2
+		|| (ctx->opt_form == 0 && wide && ws_fits && (precision == HDDM_LB_F32 || ctx->dt_needed >= 2));
+	// the wave-specialized kernel's launch shape: one walker wave, four
+	// consumer waves and two dx waves per workgroup, four dof tiles per
+	// launch; the second dx wave works when its rows fit the LDS
+	const void* ws_kern = nullptr;
+	size_t ws_lds = 0;
+	int ws_grid = 0, ws_ndx = 1;
+	if (ws)
+	{
+		ws_kern = adjoint_ws_kernel_ptr(precision, xbanks_for(ctx->dim), ctx->fmax_all);
+		if (!ws_kern) return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: dim %d is not supported", ctx->dim);
+		ws_ndx = adjoint_ws_lds_bytes(precision, ctx->fmax_all, ctx->dim, 2) <= ctx->max_lds ? 2 : 1;
+		ws_lds = adjoint_ws_lds_bytes(precision, ctx->fmax_all, ctx->dim, ws_ndx);
+		if (ws_lds > ctx->max_lds)
+			return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: LDS of one workgroup (%zu B) exceeds the CU", ws_lds);
+		HIP_TRY(hipFuncSetAttribute(ws_kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ws_lds));
+		int per_cu = 0;
+		HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ws_kern, 7 * kWave, ws_lds));
+		if (per_cu < 1) return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: kernel does not fit a CU");
+		ws_grid = per_cu * ctx->num_cus;
+	}
+	// the wave-specialized kernel splits states into finer node segments
+	const StateDesc* tables = ws ? ctx->d_states_adj : ctx->d_states;
+	const int max_nseg = ws ? ctx->max_nseg_adj : ctx->max_nseg;
+
 	// dS is accumulated: clear it with a kernel (never a memset, so the
-	// sequence stays capturable), through per-XCD copies when they fit the
-	// scratch limit
-	const bool xcd_copies = npoints > 0 && dsurplus && ds_total && ds_total * kXcdCopies <= kXcdCopiesMaxElems;
-	if (xcd_copies)
+	// sequence stays capturable). The wave-specialized kernel adds through
+	// owned units into K copies (K = 1: dsurplus itself); otherwise atomics
+	// go to per-XCD copies when they fit the scratch limit, else to dsurplus.
+	const int own = ws && npoints > 0 && dsurplus && ds_total ? owned_copies(ctx, ds_total, ws_grid) : 0;
+	const bool xcd_copies = !own && npoints > 0 && dsurplus && ds_total && ds_total * kXcdCopies <= kXcdCopiesMaxElems;
+	const int ncopies = own > 1 ? own : (xcd_copies ? kXcdCopies : 1);
+	if (ncopies > 1)
 	{
 		Scratch& scr = ctx->scratch;
-		const size_t need = (size_t)ds_total * kXcdCopies;
-		if (need > scr.cap_dspart)
-		{
-			if (scr.dspart) HIP_TRY(hipDeviceSynchronize());
-			dev_free(scr.dspart);
-			scr.dspart = nullptr;
-			scr.cap_dspart = 0;
-			int arc = 0;
-			scr.dspart = (double*)dev_alloc(need * sizeof(double), arc);
-			if (arc) return arc;
-			scr.cap_dspart = need;
-		}
+		const size_t need = (size_t)ds_total * ncopies;
+		if ((rc = grow_dspart(scr, need))) return rc;
 		HIP_TRY(launch_fill_u32((uint32_t*)scr.dspart, 2 * (int64_t)need, 0u, st));
 	}
 	else if (dsurplus && ds_total) HIP_TRY(launch_fill_u32((uint32_t*)dsurplus, 2 * ds_total, 0u, st));
+	ctx->last_adj[0] = ws ? 2 : 1;
+	ctx->last_adj[1] = own;
+	ctx->last_adj[2] = ncopies;
+	ctx->last_adj[3] = max_nseg;
 	if (npoints == 0 || (!dx && !dsurplus)) return scratch_mark(ctx, st);
 	// dx is stored, not accumulated: no clear, and columns past dim of a
 	// padded dx stay as the caller left them
-	if (dx && ctx->max_nseg > 1)
+	if (dx && max_nseg > 1)
 	{
 		Scratch& scr = ctx->scratch;
-		const size_t need = (size_t)npoints * (size_t)ctx->max_nseg * (size_t)ctx->dim;
+		const size_t need = (size_t)npoints * (size_t)max_nseg * (size_t)ctx->dim;
 		if (need > scr.cap_dxpart)
 		{
 			if (scr.dxpart) HIP_TRY(hipDeviceSynchronize());
@@ -568,28 +686,14 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 	const int64_t nb = (int64_t)(ns + 1) << bb;
 	AdjointArgs a;
 	std::memset(&a, 0, sizeof(a));
-	a.states = ctx->d_states; a.cost_order = ctx->d_cost_order; a.hdr = sc.hdr;
+	a.states = tables; a.cost_order = ctx->d_cost_order; a.hdr = sc.hdr;
 	a.perm = sc.perm; a.x = x; a.ldx = ldx; a.g = gbar; a.ldg = ldg; a.dx = dx; a.lddx = lddx;
 	a.dxpart = sc.dxpart; a.plans = sc.plans;
-	a.ds = xcd_copies ? sc.dspart : dsurplus; a.ds_xcd = xcd_copies ? ds_total : 0;
+	a.ds = ncopies > 1 ? sc.dspart : dsurplus; a.ds_xcd = ncopies > 1 ? ds_total : 0;
+	a.ds_own = own;
 	a.nstates = ns; a.dim = ctx->dim;
 	a.status = ctx->d_status;
 
-	// automatic: the wave-specialized kernel for wide states (D > 32), which
-	// walks each trie once instead of once per dof tile (measured: D=59, 128
-	// dofs, 2^16 points: FP64 1255 -> 838 ms; its F32 consumers take dS to the
-	// MFMA units); the per-tile kernels otherwise (D=20, 32 dofs: FP64 32 ms
-	// against 110 ms wave-specialized, F32 105 ms). The per-tile kernels compute
-	// in FP64 whatever the precision asked for, which meets the F32 bound too.
-	const bool wide = xbanks_for(ctx->dim) == 4;
-	// nodes of more than 8 factors next to four coordinate banks exceed the
-	// wave-specialized kernel's registers
-	const bool ws_fits = !(wide && ctx->fmax_all > 8);
-	if (ctx->opt_form == 2 && !ws_fits)
-		return set_error(HDDM_LB_ERROR_UNSUPPORTED,
-			"adjoint: the wave-specialized kernel takes nodes of at most 8 factors when dim > 32");
-	const bool ws = ctx->opt_form == 2
-		|| (ctx->opt_form == 0 && wide && ws_fits && (precision == HDDM_LB_F32 || ctx->dt_needed >= 2));
 	if (ws && ctx->opt_stats)
 	{
 		if (!ctx->d_adj_stats)
@@ -603,24 +707,12 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 	}
 	if (ws)
 	{
-		// one walker wave, four consumer waves and two dx waves per
-		// workgroup, four dof tiles per launch; the second dx wave works
-		// when its rows fit the LDS
-		const void* kern = adjoint_ws_kernel_ptr(precision, xbanks_for(ctx->dim), ctx->fmax_all);
-		if (!kern) return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: dim %d is not supported", ctx->dim);
-		a.ndx = adjoint_ws_lds_bytes(precision, ctx->fmax_all, ctx->dim, 2) <= ctx->max_lds ? 2 : 1;
-		const size_t lds = adjoint_ws_lds_bytes(precision, ctx->fmax_all, ctx->dim, a.ndx);
-		if (lds > ctx->max_lds)
-			return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: LDS of one workgroup (%zu B) exceeds the CU", lds);
-		HIP_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-		int per_cu = 0;
-		HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 7 * kWave, lds));
-		if (per_cu < 1) return set_error(HDDM_LB_ERROR_UNSUPPORTED, "adjoint: kernel does not fit a CU");
+		a.ndx = ws_ndx;
 		HIP_TRY(launch_fill_u32(sc.hist, nb, 0u, st));
 		HIP_TRY(launch_bucket_key(npoints, x, ldx, state_ids, ctx->d_state_ok, ns, ctx->dim, bb, sc.bkey, sc.hist, st));
-		HIP_TRY(launch_scan_plan(sc.hist, sc.cursor, nb, bb, ns, ctx->d_states, ctx->d_cost_order, sc.plans, sc.hdr, 1,
+		HIP_TRY(launch_scan_plan(sc.hist, sc.cursor, nb, bb, ns, tables, ctx->d_cost_order, sc.plans, sc.hdr, 1,
 			ctx->dim, st));
-		HIP_TRY(launch_scatter(npoints, sc.bkey, sc.cursor, sc.perm, st));
+		if ((rc = stable_order(sc, npoints, nb, st))) return rc;
 		const int npass = (ctx->dt_needed + 3) / 4;
 		auto& ring = ctx->tring[1];
 		const int tslot = (int)(ring.calls % hddm_lb_ctx::kTimingRing);
@@ -630,14 +722,14 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 			a.dt_base = 4 * ps;
 			if (ps > 0) HIP_TRY(launch_reset_counter(sc.hdr, st));
 			void* params[] = { (void*)&a };
-			HIP_TRY(hipLaunchKernel(kern, dim3(per_cu * ctx->num_cus), dim3(7 * kWave), params, lds, st));
+			HIP_TRY(hipLaunchKernel(ws_kern, dim3(ws_grid), dim3(7 * kWave), params, ws_lds, st));
 		}
 		if (ctx->opt_timing)
 		{
 			HIP_TRY(hipEventRecord(ring.e1[tslot], st));
 			ring.calls++;
 		}
-		return adjoint_tail(ctx, sc, dx, lddx, dsurplus, ds_total, xcd_copies, ns, st);
+		return adjoint_tail(ctx, sc, tables, max_nseg, dx, lddx, dsurplus, ds_total, ncopies, ns, st);
 	}
 
 	// dx and dS together: the edge-sum walk for dx, then the forward's walk
@@ -675,9 +767,9 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 
 	HIP_TRY(launch_fill_u32(sc.hist, nb, 0u, st));
 	HIP_TRY(launch_bucket_key(npoints, x, ldx, state_ids, ctx->d_state_ok, ns, ctx->dim, bb, sc.bkey, sc.hist, st));
-	HIP_TRY(launch_scan_plan(sc.hist, sc.cursor, nb, bb, ns, ctx->d_states, ctx->d_cost_order, sc.plans, sc.hdr, 1,
+	HIP_TRY(launch_scan_plan(sc.hist, sc.cursor, nb, bb, ns, tables, ctx->d_cost_order, sc.plans, sc.hdr, 1,
 		ctx->dim, st));
-	HIP_TRY(launch_scatter(npoints, sc.bkey, sc.cursor, sc.perm, st));
+	if ((rc = stable_order(sc, npoints, nb, st))) return rc;
 
 	auto& ring = ctx->tring[1];
 	const int tslot = (int)(ring.calls % hddm_lb_ctx::kTimingRing);
@@ -697,5 +789,5 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 		HIP_TRY(hipEventRecord(ring.e1[tslot], st));
 		ring.calls++;
 	}
-	return adjoint_tail(ctx, sc, dx, lddx, dsurplus, ds_total, xcd_copies, ns, st);
+	return adjoint_tail(ctx, sc, tables, max_nseg, dx, lddx, dsurplus, ds_total, ncopies, ns, st);
 }

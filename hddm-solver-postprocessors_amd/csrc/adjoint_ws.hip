@@ -1,36 +1,45 @@
 // Wave-specialized adjoint (gfx950): one walker wave, NC = 4 consumer waves
-// and one dx wave per workgroup. No counterpart in the reference.
+// and two dx waves per workgroup. No counterpart in the reference.
 //
 // For y_p = sum_k B_k(x_p) S_k and an upstream gradient g_p:
 //   dS_k  = sum_p B_k(x_p) g_p                    (dsurplus)
 //   dx_pd = sum_k w_pk dB_k/dx_pd,  w_pk = g_p . S_k (dx)
-// Work items are the forward's: one 64-point block of one state against one
-// node segment. The walker wave runs the forward's assembly walk (walk.h):
-// surviving nodes' basis values go into chunks of an LDS ring (32 nodes in
-// FP32, 16 in FP64) that are published to the other waves.
+// A work item is one 64-point block of one state against one node segment.
+// Items come either one at a time from the work queue, or as owned units:
+// a unit (state, segment, copy class k) is all blocks k, k + K, ... of the
+// state against the segment, and its workgroup alone adds to those rows of
+// copy k of dS, so it adds with plain loads and stores instead of atomics
+// (the K copies are summed in a fixed order afterwards, which makes dS
+// bitwise repeatable). The walker wave runs the forward's assembly walk
+// (walk.h): surviving nodes' basis values go into chunks of an LDS ring (32
+// nodes in FP32, 16 in FP64) that are published to the other waves.
 // Consumer c takes one 32-dof tile of the surplus (or, when a state has at
 // most two tiles in the pass, one of the two products of a tile) and, per
 // published chunk, on MFMA:
 //   dS   D[node][dof] = sum over the block's points of B[point][node] g[point][dof],
-//        added to one copy of dS per XCD with atomics (copies summed after);
+//        added to the unit's copy of dS (owned units) or with atomics to one
+//        copy per XCD (copies summed after) or to dS itself;
 //   w    W[point][node] = sum over the tile's dofs of g[point][dof] S[node][dof],
-//        written to an LDS buffer of its own (two buffers, by chunk parity).
-// The dx wave takes each chunk once every consumer is done with it. For each
-// surviving node k of the chunk it sums the tiles' w in consumer order, reads
-// k's factors phi_f (dim d_f) from the per-node factor table, recomputes them
-// at the block's points with the walk's own FP64 operations, and adds
+//        written to an LDS buffer of its own (two buffers, by the chunk's
+//        parity within the item).
+// A dx wave takes each chunk of its parity (counted from the item's start)
+// once every consumer is done with it. For each surviving node k of the chunk it sums the tiles' w in
+// consumer order, reads k's factors phi_f (dim d_f) from the per-node factor
+// table, recomputes them at the block's points with the walk's own FP64
+// operations, and adds
 //   w_pk (prod_{g != f} phi_g) phi_f'       (prefix and suffix products)
 // to dx[d_f] in LDS rows of its own: no division, and a factor that is zero
-// contributes exactly zero. One wave adds in a fixed order, so dx is bitwise
-// repeatable; it is written out when the next item (or the exit) arrives.
+// contributes exactly zero. Each wave adds in a fixed order and the two
+// waves' rows are summed in a fixed order, so dx is bitwise repeatable; it is
+// written out when the next item (or the exit) arrives.
 // FP64: v_mfma_f64_16x16x4f64 throughout. F32: v_mfma_f32_32x32x2f32 with
 // FP32 operands; dS partials of one chunk (64 points) are added in FP64, w is
 // summed over tiles in FP64.
 // Synchronization is the forward WS sweep's: an LDS message ring with
 // release/acquire counters and bounded waits (sweep_ws.hip). The walker
-// reuses a ring slot once all five reader waves are done with it; a consumer
-// rewrites its w buffer of one parity only after the dx wave is done with the
-// chunk that last used it.
+// reuses a ring slot once all six reader waves are done with it; a consumer
+// rewrites its w buffer of one parity only after the dx wave of that parity
+// is done with the chunk that last used it.
 #include "sweep_common.h"
 #include "kernels.h"
 #include "walk.h"
@@ -49,7 +58,7 @@ enum : int { kRoleDs = 1, kRoleW = 2 };
 
 struct AMsg
 {
-	int type, a, b, np;
+	int type, a, b, np, copy, pad;
 	int64_t start, lpos, count;
 };
 
@@ -289,11 +298,31 @@ struct AdjCons<HDDM_LB_FP64>
 			}
 	}
 
+	// own: owned units, this wave is the only writer of the chunk's dS rows
+	// in its copy: it loads them (before the MFMAs), adds and stores.
+	// Otherwise FP64 atomics.
 	__device__ __forceinline__ void ds(const AdjointArgs& a, const StateDesc* sd, double* dsb, const double* Bc,
-		int chunk, uint32_t mask, int tile, int dofs, int lane)
+		int chunk, uint32_t mask, int tile, int dofs, int lane, bool own)
 	{
 		const int col = lane & 15, kq = lane >> 4;
 		const bool on = (mask >> col) & 1u;
+		const int32_t* order = ldc(&sd->order);
+		(void)a;
+		double* rowp[4];
+		double old[4][2];
+#pragma unroll
+		for (int e = 0; e < 4; e++)
+		{
+			const int nl = 4 * e + kq;
+			const bool live = (mask >> nl) & 1u;
+			rowp[e] = dsb + (int64_t)(live ? ldc(order + chunk * 16 + nl) : 0) * dofs;
+#pragma unroll
+			for (int db = 0; db < 2; db++)
+			{
+				const int dof = 32 * tile + 16 * db + col;
+				old[e][db] = own && live && dof < dofs ? rowp[e][dof] : 0.0;
+			}
+		}
 		d4_t acc[2] = { { 0.0, 0.0, 0.0, 0.0 }, { 0.0, 0.0, 0.0, 0.0 } };
 #pragma unroll
 		for (int s = 0; s < 16; s++)
@@ -302,19 +331,18 @@ struct AdjCons<HDDM_LB_FP64>
 			acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, gd[0][s], acc[0], 0, 0, 0);
 			acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, gd[1][s], acc[1], 0, 0, 0);
 		}
-		const int32_t* order = ldc(&sd->order);
-		(void)a;
 #pragma unroll
 		for (int e = 0; e < 4; e++)
 		{
 			const int nl = 4 * e + kq;
 			if (!((mask >> nl) & 1u)) continue;
-			double* row = dsb + (int64_t)ldc(order + chunk * 16 + nl) * dofs;
 #pragma unroll
 			for (int db = 0; db < 2; db++)
 			{
 				const int dof = 32 * tile + 16 * db + col;
-				if (dof < dofs) unsafeAtomicAdd(row + dof, acc[db][e]);
+				if (dof >= dofs) continue;
+				if (own) rowp[e][dof] = old[e][db] + acc[db][e];
+				else unsafeAtomicAdd(rowp[e] + dof, acc[db][e]);
 			}
 		}
 	}
@@ -374,11 +402,27 @@ struct AdjCons<HDDM_LB_F32>
 			}
 	}
 
+	// own: as in the FP64 consumer, the owned rows are loaded (before the
+	// MFMAs), added to and stored; otherwise FP64 atomics.
 	__device__ __forceinline__ void ds(const AdjointArgs& a, const StateDesc* sd, double* dsb, const float* Bc,
-		int chunk, uint32_t mask, int tile, int dofs, int lane)
+		int chunk, uint32_t mask, int tile, int dofs, int lane, bool own)
 	{
 		const int col = lane & 31, kh = lane >> 5;
 		const bool on = (mask >> col) & 1u;
+		const int32_t* order = ldc(&sd->order);
+		(void)a;
+		const int dof = 32 * tile + col;
+		double old[16];
+		if (own)
+		{
+#pragma unroll
+			for (int r = 0; r < 16; r++)
+			{
+				const int nl = 8 * (r >> 2) + 4 * kh + (r & 3);
+				const bool live = ((mask >> nl) & 1u) && dof < dofs;
+				old[r] = live ? dsb[(int64_t)ldc(order + chunk * 32 + nl) * dofs + dof] : 0.0;
+			}
+		}
 		f16_t acc;
 #pragma unroll
 		for (int r = 0; r < 16; r++) acc[r] = 0.0f;
@@ -388,15 +432,14 @@ struct AdjCons<HDDM_LB_F32>
 			const float bv = on ? Bc[col * kARow + 2 * s + kh] : 0.0f;
 			acc = __builtin_amdgcn_mfma_f32_32x32x2f32(bv, gd[s], acc, 0, 0, 0);
 		}
-		const int32_t* order = ldc(&sd->order);
-		(void)a;
-		const int dof = 32 * tile + col;
 #pragma unroll
 		for (int r = 0; r < 16; r++)
 		{
 			const int nl = 8 * (r >> 2) + 4 * kh + (r & 3);
 			if (!((mask >> nl) & 1u) || dof >= dofs) continue;
-			unsafeAtomicAdd(dsb + (int64_t)ldc(order + chunk * 32 + nl) * dofs + dof, (double)acc[r]);
+			double* p = dsb + (int64_t)ldc(order + chunk * 32 + nl) * dofs + dof;
+			if (own) *p = old[r] + (double)acc[r];
+			else unsafeAtomicAdd(p, (double)acc[r]);
 		}
 	}
 
@@ -427,6 +470,32 @@ struct AdjCons<HDDM_LB_F32>
 };
 
 } // namespace
+
+// Owned units: (state, node segment, copy class k) for the states with points,
+// in cost order, each state's units segment-major; unit (s, g, k) covers the
+// blocks k, k + K, k + 2K, ... of state s against segment g and adds dS to
+// copy k. No two units share a (copy, row), so one workgroup at a time adds
+// to each row of a copy. False past the last unit.
+__device__ __forceinline__ bool decode_unit(const AdjointArgs& a, int64_t unit, int K, int& s, int& seg, int& k)
+{
+	int64_t base = 0;
+	for (int i = 0; i < a.nstates; i++)
+	{
+		const int si = ldc(&a.cost_order[i]);
+		if (ldc(&a.plans[si].nblk) == 0) continue;
+		const int64_t nu = (int64_t)ldc(&a.states[si].nseg) * K;
+		if (unit < base + nu)
+		{
+			const int64_t local = unit - base;
+			s = si;
+			seg = (int)(local / K);
+			k = (int)(local - (int64_t)seg * K);
+			return true;
+		}
+		base += nu;
+	}
+	return false;
+}
 
 // Loads the block's coordinates into the four 16-dim banks (the pinned
 // registers the walk and xpin read).
@@ -496,42 +565,72 @@ __global__ void __launch_bounds__(64 * (kANC + 3)) adjoint_ws_kernel(AdjointArgs
 			cyc_s += clk() - t0;
 			return r;
 		};
-		auto publish = [&](int type, int a0, int b0, int np, int64_t start, int64_t lpos, int64_t count) {
+		auto publish = [&](int type, int a0, int b0, int np, int64_t start, int64_t lpos, int64_t count, int copy) {
 			AMsg* m = msgs + (npub & (kARing - 1));
 			if (lane == 0)
 			{
-				m->type = type; m->a = a0; m->b = b0; m->np = np;
+				m->type = type; m->a = a0; m->b = b0; m->np = np; m->copy = copy;
 				m->start = start; m->lpos = lpos; m->count = count;
 			}
 			a_release(&sy->produced, npub + 1);
 			npub++;
 		};
+		// owned units (K > 0): the unit in hand and its next block
+		const int K = a.ds_own;
+		int u_s = 0, u_seg = 0, u_k = 0;
+		int64_t u_blk = 0, u_nblk = 0, u_start = 0, u_count = 0;
 		for (;;)
 		{
-			uint64_t v = 0;
-			if (lane == 0) v = atomicAdd(&a.hdr->counter, 1ull);
-			const int64_t item = (int64_t)uniform64(v);
-			if (item >= nitems) break;
-			ItemInfo it;
-			decode_item(a, item, 1, it);
-			const StateDesc* sd = a.states + it.s;
+			int s, seg, copy;
+			int64_t blk, start, count;
+			if (K > 0)
+			{
+				if (u_blk >= u_nblk)
+				{
+					bool found = false;
+					for (;;)
+					{
+						uint64_t v = 0;
+						if (lane == 0) v = atomicAdd(&a.hdr->counter, 1ull);
+						if (!decode_unit(a, (int64_t)uniform64(v), K, u_s, u_seg, u_k)) break;
+						u_nblk = ldc(&a.plans[u_s].nblk);
+						if (u_k < u_nblk) { found = true; break; }
+					}
+					if (!found) break;
+					u_blk = u_k;
+					u_start = ldc(&a.plans[u_s].start);
+					u_count = ldc(&a.plans[u_s].count);
+				}
+				s = u_s; seg = u_seg; copy = u_k; blk = u_blk; start = u_start; count = u_count;
+				u_blk += K;
+			}
+			else
+			{
+				uint64_t v = 0;
+				if (lane == 0) v = atomicAdd(&a.hdr->counter, 1ull);
+				const int64_t item = (int64_t)uniform64(v);
+				if (item >= nitems) break;
+				ItemInfo it;
+				decode_item(a, item, 1, it);
+				s = it.s; seg = it.seg; copy = 0; blk = it.grp; start = it.start; count = it.count;
+			}
+			const StateDesc* sd = a.states + s;
 			const int DP = ldc(&sd->dofs_pad);
 			const int tpass = min(kANC, (DP >> 5) - a.dt_base);
 			if (tpass <= 0) continue;
 			const Op* const gops = ldc(&sd->ops);
 			const uint32_t* const gskip = ldc(&sd->skip);
 			const int32_t* const seg_op = ldc(&sd->seg_op);
-			const int q0 = ldc(seg_op + it.seg), q1 = ldc(seg_op + it.seg + 1);
-			const int64_t blk = it.grp;
-			const int64_t rem = it.count - blk * kWave;
+			const int q0 = ldc(seg_op + seg), q1 = ldc(seg_op + seg + 1);
+			const int64_t rem = count - blk * kWave;
 			const int np = rem <= 0 ? 0 : (rem < kWave ? (int)rem : kWave);
 			if (np == 0) continue;
 			const int myp = lane < np ? lane : np - 1;
-			const uint32_t orig = a.perm[it.start + blk * kWave + myp];
+			const uint32_t orig = a.perm[start + blk * kWave + myp];
 			double xb0[16], xb1[16], xb2[16], xb3[16];
 			load_xbanks<XB>(a.x + (int64_t)orig * a.ldx, a.dim, xb0, xb1, xb2, xb3);
 			if (!(ok = acquire())) break;
-			publish(kAMsgItem, it.s, it.seg, np, it.start, blk * kWave, it.count);
+			publish(kAMsgItem, s, seg, np, start, blk * kWave, count, copy);
 			wa.init(q0, q1, (double*)(lbase + L.pa), slot(npub), lane, gops, gskip);
 			for (;;)
 			{
@@ -543,7 +642,7 @@ __global__ void __launch_bounds__(64 * (kANC + 3)) adjoint_ws_kernel(AdjointArgs
 				if (wa.chunk >= 0)
 				{
 					st_chunks++;
-					publish(kAMsgChunk, wa.chunk, (int)wa.mask, 0, 0, 0, 0);
+					publish(kAMsgChunk, wa.chunk, (int)wa.mask, 0, 0, 0, 0, 0);
 					wa.chunk = -1;
 				}
 				if (!more) break;
@@ -554,7 +653,7 @@ __global__ void __launch_bounds__(64 * (kANC + 3)) adjoint_ws_kernel(AdjointArgs
 			if (!ok) break;
 			st_items++;
 		}
-		if (ok && acquire()) publish(kAMsgExit, 0, 0, 0, 0, 0, 0);
+		if (ok && acquire()) publish(kAMsgExit, 0, 0, 0, 0, 0, 0, 0);
 		else a_release(&sy->abort, 1);
 		if (st_on && lane == 0)
 		{
@@ -579,7 +678,7 @@ __global__ void __launch_bounds__(64 * (kANC + 3)) adjoint_ws_kernel(AdjointArgs
 		int* const mydone = &sy->done[kANC + k];
 		const bool want_dx = a.dx != nullptr && k < ndx;
 		uint64_t cyc_dx = 0, cyc_wait = 0;
-		int nchunks = 0;
+		int nitem_chunks = 0;
 		// the item whose rows DX holds
 		bool have = false;
 		int np = 0, seg = 0, tpass = 0, fs = 4, nfac = 4;
@@ -686,8 +785,13 @@ __global__ void __launch_bounds__(64 * (kANC + 3)) adjoint_ws_kernel(AdjointArgs
 			const int type = uniform(m->type);
 			if (type == kAMsgChunk)
 			{
-				const int par = nchunks & 1;
-				nchunks++;
+				// the dx wave (and w buffer) of a chunk alternates with the
+				// chunks of the item, counted from its start: the split of a
+				// point's terms between the two waves, and the rounding of
+				// their sum, depends only on the item's points (which the
+				// stable point order fixes)
+				const int par = nitem_chunks & 1;
+				nitem_chunks++;
 				if (have && want_dx && dxw(par) == k)
 				{
 					const int chunk = uniform(m->a);
@@ -711,6 +815,7 @@ __global__ void __launch_bounds__(64 * (kANC + 3)) adjoint_ws_kernel(AdjointArgs
 			else if (type == kAMsgItem)
 			{
 				if (!item_end(n)) break;
+				nitem_chunks = 0;
 				const int s = uniform(m->a);
 				seg = uniform(m->b);
 				np = uniform(m->np);
@@ -758,7 +863,7 @@ __global__ void __launch_bounds__(64 * (kANC + 3)) adjoint_ws_kernel(AdjointArgs
 	const StateDesc* sd = nullptr;
 	bool active = false;
 	int role = 0, tile = 0, dofs = 0, DP = 32;
-	int nchunks = 0;
+	int nitem_chunks = 0;
 	int wmsg[2] = { -1, -1 };   // message of the last chunk whose w went to each buffer
 	double* dsb = nullptr;
 	uint64_t cyc_ds = 0, cyc_wc = 0, cyc_idle = 0;
@@ -772,15 +877,15 @@ __global__ void __launch_bounds__(64 * (kANC + 3)) adjoint_ws_kernel(AdjointArgs
 		const int type = uniform(m->type);
 		if (type == kAMsgChunk)
 		{
-			const int par = nchunks & 1;
-			nchunks++;
+			const int par = nitem_chunks & 1;   // as in the dx waves
+			nitem_chunks++;
 			if (active)
 			{
 				const int chunk = uniform(m->a);
 				const uint32_t mask = (uint32_t)uniform(m->b);
 				const T* Bc = slot(n);
 				const uint64_t t1 = clk();
-				if ((role & kRoleDs) && a.ds) cons.ds(a, sd, dsb, Bc, chunk, mask, tile, dofs, lane);
+				if ((role & kRoleDs) && a.ds) cons.ds(a, sd, dsb, Bc, chunk, mask, tile, dofs, lane, a.ds_own > 0);
 				const uint64_t t2 = clk();
 				if (role & kRoleW)
 				{
@@ -796,6 +901,7 @@ __global__ void __launch_bounds__(64 * (kANC + 3)) adjoint_ws_kernel(AdjointArgs
 		}
 		else if (type == kAMsgItem)
 		{
+			nitem_chunks = 0;
 			const int s = uniform(m->a);
 			const int np = uniform(m->np);
 			const int64_t start = (int64_t)uniform64((uint64_t)m->start);
@@ -813,7 +919,8 @@ __global__ void __launch_bounds__(64 * (kANC + 3)) adjoint_ws_kernel(AdjointArgs
 			if (a.ds)
 			{
 				dsb = a.ds + ldc(&sd->ds_offset);
-				if (a.ds_xcd > 0)
+				if (a.ds_own > 0) dsb += (int64_t)uniform(m->copy) * a.ds_xcd;
+				else if (a.ds_xcd > 0)
 				{
 					uint32_t xcc;
 					asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc));

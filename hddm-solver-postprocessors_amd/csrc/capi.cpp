@@ -54,11 +54,14 @@ void Scratch::release()
 {
 	if (owned)
 		for (void* p : { (void*)bkey, (void*)perm, (void*)hist, (void*)cursor, (void*)partial, (void*)dspart,
-				(void*)dxpart, (void*)plans, (void*)hdr })
+				(void*)dxpart, (void*)plans, (void*)hdr, (void*)bkey2, (void*)pidx, (void*)sort_tmp })
 			dev_free(p);
 	bkey = perm = hist = cursor = nullptr;
 	partial = dspart = dxpart = nullptr;
+	bkey2 = pidx = nullptr;
+	sort_tmp = nullptr;
 	cap_dspart = cap_dxpart = 0;
+	cap_bkey2 = cap_pidx = cap_sort_tmp = 0;
 	plans = nullptr;
 	hdr = nullptr;
 	cap_bkey = cap_perm = cap_hist = cap_cursor = cap_partial = cap_plans = cap_hdr = 0;
@@ -98,7 +101,7 @@ hddm_lb_ctx::~hddm_lb_ctx()
 		if (p) (void)hipHostFree(p);
 	states.clear();
 	scratch.release();
-	for (void* p : { (void*)d_states, (void*)d_cost_order, (void*)d_state_ok,
+	for (void* p : { (void*)d_states, (void*)d_states_adj, (void*)d_cost_order, (void*)d_state_ok,
 		(void*)d_xstage, (void*)d_ystage, (void*)d_sidstage, (void*)d_wbuf, (void*)d_stats, (void*)d_status, (void*)d_adj_stats })
 		dev_free(p);
 	if (scratch_event) (void)hipEventDestroy(scratch_event);
@@ -149,6 +152,11 @@ extern "C" int hddm_lb_create(int device, hddm_lb_ctx** out)
 	HIP_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device));
 	ctx->num_cus = cus;
 	ctx->max_lds = (size_t)lds;
+	{
+		size_t mem = 0;
+		HIP_TRY(hipDeviceTotalMem(&mem, device));
+		ctx->total_mem = mem;
+	}
 	HIP_TRY(hipEventCreateWithFlags(&ctx->scratch_event, hipEventDisableTiming));
 	{
 		int rc = 0;
@@ -206,6 +214,16 @@ extern "C" int hddm_lb_set_option(hddm_lb_ctx* ctx, int option, int64_t value)
 		ctx->opt_walk = (int)value; break;
 	case HDDM_LB_OPT_COALESCE: ctx->opt_coalesce = value != 0; break;
 	case HDDM_LB_OPT_DEVICE_LAYOUT: ctx->opt_device_layout = value != 0; break;
+	case HDDM_LB_OPT_ADJOINT_OWNED:
+		if (ctx->opt_adj_owned != (value != 0))
+		{
+			// the adjoint's segment tables depend on it
+			ctx->opt_adj_owned = value != 0;
+			ctx->tables_dirty = true;
+			ctx->generation++;
+			if (ctx->dim > 0) return prepare_tables(ctx);
+		}
+		break;
 	case HDDM_LB_OPT_FORMULATION:
 		if (value < 0 || value > 2) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "set_option: formulation must be 0, 1 or 2");
 		ctx->opt_form = (int)value; break;
@@ -263,6 +281,7 @@ extern "C" int hddm_lb_upload_state(hddm_lb_ctx* ctx, int istate, int dim, int n
 			slot->shared = std::move(DL.shared);
 			slot->opstart = std::move(DL.opstart);
 			slot->d_seg_op = (int32_t*)alloc(sizeof(int32_t) * ((size_t)nno + 1), rc);
+			slot->d_seg_op_adj = (int32_t*)alloc(sizeof(int32_t) * ((size_t)nno + 1), rc);
 			d.seg_op = slot->d_seg_op;
 			if (rc) return rc;
 			return finish_upload(ctx, istate, dim, std::move(slot));
@@ -308,6 +327,7 @@ extern "C" int hddm_lb_upload_state(hddm_lb_ctx* ctx, int istate, int dim, int n
 	d.lomask = (const uint64_t*)up(L.lomask.data(), L.lomask.size() * sizeof(uint64_t));
 	d.himask = (const uint64_t*)up(L.himask.data(), L.himask.size() * sizeof(uint64_t));
 	slot->d_seg_op = (int32_t*)up(nullptr, sizeof(int32_t) * ((size_t)nno + 1));
+	slot->d_seg_op_adj = (int32_t*)up(nullptr, sizeof(int32_t) * ((size_t)nno + 1));
 	d.seg_op = slot->d_seg_op;
 	slot->shared = std::move(L.shared);
 	slot->opstart = std::move(L.opstart);
@@ -418,7 +438,26 @@ int scratch_mark(hddm_lb_ctx* ctx, hipStream_t st)
 	return HDDM_LB_SUCCESS;
 }
 
-// Uploads the per-state descriptor table when states or segmenting changed.
+// Writes a state's node-segment table for segments of at least `want` nodes
+// (0: one segment), boundaries only at top-level runs so each segment's
+// first op is at depth 0; returns the segment count.
+static int build_segments(const StateSlot& slot, int nno, int64_t want, int32_t* d_table, int& nseg)
+{
+	std::vector<int32_t> segop;
+	segop.push_back(0);
+	if (want > 0)
+	{
+		int64_t last = 0;
+		for (int n = 1; n < nno; n++)
+			if (n - last >= want && slot.shared[n] == 0) { segop.push_back(slot.opstart[n]); last = n; }
+	}
+	segop.push_back(slot.opstart[nno]);
+	HIP_TRY(hipMemcpy(d_table, segop.data(), sizeof(int32_t) * segop.size(), hipMemcpyHostToDevice));
+	nseg = (int)segop.size() - 1;
+	return HDDM_LB_SUCCESS;
+}
+
+// Uploads the per-state descriptor tables when states or segmenting changed.
 int prepare_tables(hddm_lb_ctx* ctx)
 {
 	const int ns = (int)ctx->states.size();
@@ -432,18 +471,25 @@ int prepare_tables(hddm_lb_ctx* ctx)
 	for (auto& s : ctx->states) if (s && s->loaded) nloaded++;
 	if (!nloaded) return set_error(HDDM_LB_ERROR_STATE, "no state is loaded");
 
-	std::vector<StateDesc> descs(ns);
+	std::vector<StateDesc> descs(ns), descs_adj(ns);
 	std::vector<uint8_t> ok(ns, 0);
 	std::vector<std::pair<double, int> > cost;
 	ctx->dt_needed = 1;
 	ctx->fmax_all = 1;
 	ctx->max_nseg_dp = 0;
 	ctx->max_nseg = 1;
+	ctx->max_nseg_adj = 1;
+	ctx->nseg_adj_total = 0;
 	int64_t ds_offset = 0;
 	for (int s = 0; s < ns; s++)
 	{
 		std::memset(&descs[s], 0, sizeof(StateDesc));
-		if (!ctx->states[s] || !ctx->states[s]->loaded) { descs[s].nseg = 1; descs[s].dofs_pad = 32; continue; }
+		if (!ctx->states[s] || !ctx->states[s]->loaded)
+		{
+			descs[s].nseg = 1; descs[s].dofs_pad = 32;
+			descs_adj[s] = descs[s];
+			continue;
+		}
 		StateSlot& slot = *ctx->states[s];
 		StateDesc d = slot.desc;
 		// automatic: eighths of states of at least 16k nodes. Segment-major
@@ -455,24 +501,31 @@ int prepare_tables(hddm_lb_ctx* ctx)
 			: (d.nno >= 16384 ? ((int64_t)d.nno + 7) / 8 : 0);
 		if (slot.seg_len != want)
 		{
-			// Segment boundaries at top-level runs, so each segment's first op is at depth 0.
-			std::vector<int32_t> segop;
-			segop.push_back(0);
-			if (want > 0)
-			{
-				int64_t last = 0;
-				for (int n = 1; n < d.nno; n++)
-					if (n - last >= want && slot.shared[n] == 0) { segop.push_back(slot.opstart[n]); last = n; }
-			}
-			segop.push_back(slot.opstart[d.nno]);
-			HIP_TRY(hipMemcpy(slot.d_seg_op, segop.data(), sizeof(int32_t) * segop.size(), hipMemcpyHostToDevice));
-			slot.desc.nseg = (int)segop.size() - 1;
+			int rc = build_segments(slot, d.nno, want, slot.d_seg_op, slot.desc.nseg);
+			if (rc) return rc;
 			slot.seg_len = want;
+		}
+		// the wave-specialized adjoint: with owned units, automatic segments of
+		// a 32nd of states of at least 16k nodes, so that (state, segment, dS
+		// copy) units give every workgroup several units to own; the
+		// forward's segments otherwise
+		const int64_t want_adj = ctx->opt_seg_nodes > 0 || !ctx->opt_adj_owned ? want
+			: (d.nno >= 16384 ? ((int64_t)d.nno + 31) / 32 : 0);
+		if (slot.seg_len_adj != want_adj)
+		{
+			int rc = build_segments(slot, d.nno, want_adj, slot.d_seg_op_adj, slot.nseg_adj);
+			if (rc) return rc;
+			slot.seg_len_adj = want_adj;
 		}
 		d = slot.desc;
 		d.ds_offset = ds_offset;
 		ds_offset += (int64_t)d.nno * d.dofs;
 		descs[s] = d;
+		descs_adj[s] = d;
+		descs_adj[s].nseg = slot.nseg_adj;
+		descs_adj[s].seg_op = slot.d_seg_op_adj;
+		ctx->max_nseg_adj = std::max(ctx->max_nseg_adj, slot.nseg_adj);
+		ctx->nseg_adj_total += slot.nseg_adj;
 		ok[s] = 1;
 		ctx->dt_needed = std::max(ctx->dt_needed, d.dofs_pad / 32);
 		ctx->fmax_all = std::max(ctx->fmax_all, d.max_depth);
@@ -487,15 +540,21 @@ int prepare_tables(hddm_lb_ctx* ctx)
 
 	if (ctx->tables_cap < ns)
 	{
-		for (void* p : { (void*)ctx->d_states, (void*)ctx->d_cost_order, (void*)ctx->d_state_ok })
+		for (void* p : { (void*)ctx->d_states, (void*)ctx->d_states_adj, (void*)ctx->d_cost_order, (void*)ctx->d_state_ok })
 			dev_free(p);
+		ctx->d_states = ctx->d_states_adj = nullptr;
+		ctx->d_cost_order = nullptr;
+		ctx->d_state_ok = nullptr;
+		ctx->tables_cap = 0;
 		int rc = 0;
 		ctx->d_states = (StateDesc*)dev_alloc(sizeof(StateDesc) * ns, rc); if (rc) return rc;
+		ctx->d_states_adj = (StateDesc*)dev_alloc(sizeof(StateDesc) * ns, rc); if (rc) return rc;
 		ctx->d_cost_order = (int32_t*)dev_alloc(sizeof(int32_t) * ns, rc); if (rc) return rc;
 		ctx->d_state_ok = (uint8_t*)dev_alloc(ns, rc); if (rc) return rc;
 		ctx->tables_cap = ns;
 	}
 	HIP_TRY(hipMemcpy(ctx->d_states, descs.data(), sizeof(StateDesc) * ns, hipMemcpyHostToDevice));
+	HIP_TRY(hipMemcpy(ctx->d_states_adj, descs_adj.data(), sizeof(StateDesc) * ns, hipMemcpyHostToDevice));
 	HIP_TRY(hipMemcpy(ctx->d_cost_order, order.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice));
 	HIP_TRY(hipMemcpy(ctx->d_state_ok, ok.data(), ns, hipMemcpyHostToDevice));
 	ctx->tables_n = ns;
@@ -723,6 +782,13 @@ extern "C" int hddm_lb_last_config(hddm_lb_ctx* ctx, int* config)
 	config[2] = c.ws ? (c.nc == 4 ? 4 : 1) : c.slab;
 	config[3] = (int)ctx->tune_runs;
 	config[4] = (int)ctx->tune_hits;
+	return HDDM_LB_SUCCESS;
+}
+
+extern "C" int hddm_lb_last_adjoint_config(hddm_lb_ctx* ctx, int* config)
+{
+	if (!ctx || !config) return set_error(HDDM_LB_ERROR_INVALID_VALUE, "last_adjoint_config: NULL argument");
+	for (int i = 0; i < 4; i++) config[i] = ctx->last_adj[i];
 	return HDDM_LB_SUCCESS;
 }
 

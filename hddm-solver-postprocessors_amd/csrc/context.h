@@ -35,6 +35,10 @@ struct StateSlot
 	std::vector<int32_t> opstart;// host copy: first op of each node
 	int32_t* d_seg_op = nullptr; // device segment table (capacity nno + 1)
 	int64_t seg_len = -1;        // segment length the table was built for
+	// the wave-specialized adjoint's own (finer) segment table
+	int32_t* d_seg_op_adj = nullptr;
+	int64_t seg_len_adj = -1;
+	int nseg_adj = 1;
 	~StateSlot();
 	void release();
 };
@@ -48,8 +52,12 @@ struct Scratch
 	uint32_t* hist = nullptr; size_t cap_hist = 0;
 	uint32_t* cursor = nullptr; size_t cap_cursor = 0;
 	double* partial = nullptr; size_t cap_partial = 0;
-	double* dspart = nullptr; size_t cap_dspart = 0;   // per-XCD copies of the adjoint's dS
+	double* dspart = nullptr; size_t cap_dspart = 0;   // copies of the adjoint's dS (per XCD or per owner class)
 	double* dxpart = nullptr; size_t cap_dxpart = 0;   // node-segment partials of the adjoint's dx
+	// the adjoint's stable point order: sorted keys, point indices, sort scratch
+	uint32_t* bkey2 = nullptr; size_t cap_bkey2 = 0;
+	uint32_t* pidx = nullptr; size_t cap_pidx = 0;
+	char* sort_tmp = nullptr; size_t cap_sort_tmp = 0;
 	StatePlan* plans = nullptr; size_t cap_plans = 0;
 	PlanHeader* hdr = nullptr; size_t cap_hdr = 0;
 	bool owned = true;   // false: the pointers lie in memory the scratch does not own
@@ -90,6 +98,7 @@ struct hddm_lb_ctx
 	int dim = -1;
 	int num_cus = 0;
 	size_t max_lds = 0;
+	size_t total_mem = 0;        // device memory, bytes
 	std::vector<std::unique_ptr<hddm_lb::StateSlot> > states;
 
 	// options
@@ -109,6 +118,10 @@ struct hddm_lb_ctx
 	int64_t max_nseg_dp = 0;
 	int max_nseg = 1;            // most node segments of a split state (1: none is split)
 	hddm_lb::StateDesc* d_states = nullptr;
+	// the same descriptors with the wave-specialized adjoint's segment tables
+	hddm_lb::StateDesc* d_states_adj = nullptr;
+	int max_nseg_adj = 1;
+	int64_t nseg_adj_total = 0;  // segments of all loaded states in that table
 	int32_t* d_cost_order = nullptr;
 	uint8_t* d_state_ok = nullptr;
 	uint64_t generation = 0;     // bumped whenever states or tables change
@@ -135,6 +148,7 @@ struct hddm_lb_ctx
 	bool comb_busy = false;
 	bool opt_coalesce = true;
 	bool opt_device_layout = true;   // build uploaded layouts on the device
+	bool opt_adj_owned = false;      // wave-specialized adjoint: dS through owned units (bitwise repeatable)
 	int64_t comb_requests = 0, comb_batches = 0;
 	// Eager calls share `scratch`: each one records scratch_event on its
 	// stream, and the next call on another stream waits for it first.
@@ -177,6 +191,7 @@ struct hddm_lb_ctx
 	std::map<std::tuple<int, int, int64_t, int, int, int, int, int>, hddm_lb::TuneConfig> tuned;
 	int64_t tune_runs = 0, tune_hits = 0;   // keys tuned, forwards served from the cache
 	hddm_lb::TuneConfig last_cfg;           // configuration of the last eager forward
+	int last_adj[4] = { 0, 0, 0, 0 };       // hddm_lb_last_adjoint_config
 
 	~hddm_lb_ctx();
 };

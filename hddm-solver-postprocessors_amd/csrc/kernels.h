@@ -44,7 +44,8 @@ struct AdjointArgs
 	int64_t lddx;
 	double* dxpart;    // node-segment partials of dx for split states ([nseg][count][dim] per state)
 	double* ds;        // may be null; state s starts at states[s].ds_offset
-	int64_t ds_xcd;    // > 0: ds holds one copy of this many elements per XCD
+	int64_t ds_xcd;    // > 0: ds holds copies of this many elements (per XCD, or per owner class)
+	int ds_own;        // wave-specialized adjoint, > 0: dS by owned units over this many copies
 	int nstates;
 	int dim;
 	int dt_base;       // dof slab of 32 handled by this launch

@@ -28,6 +28,7 @@ PRECISIONS = {"fp64": FP64, "f16x2": F16X2, "bf16x3": BF16X3, "f32": F32}
 OPT_AUTOTUNE, OPT_SEGMENT_NODES, OPT_WAVES_PER_WG, OPT_SORT_POINTS = 1, 2, 3, 4
 OPT_TIMING, OPT_SLAB_TILES, OPT_STATS, OPT_WALK, OPT_FORMULATION, OPT_COALESCE = 5, 6, 8, 9, 10, 11
 OPT_DEVICE_LAYOUT = 12
+OPT_ADJOINT_OWNED = 13
 # state arrays readable with Context.state_array (hddm_lb_state_array)
 STATE_ARRAYS = ("ops", "skip", "order", "s64", "s16", "sbf", "s32", "s32r", "iscale", "lomask", "himask", "ftab")
 STATUS_WAIT_LIMIT = 1
@@ -90,6 +91,7 @@ _SIGS = {
     "hddm_lb_last_sweep_ms": (_dbl, [_vp]),
     "hddm_lb_last_stats": (_i32, [_vp, _vp]),
     "hddm_lb_last_config": (_i32, [_vp, _vp]),
+    "hddm_lb_last_adjoint_config": (_i32, [_vp, _vp]),
     "hddm_lb_host_call_stats": (_i32, [_vp, _c.POINTER(_i64), _c.POINTER(_i64)]),
     "hddm_lb_last_adjoint_stats": (_i32, [_vp, _vp, _i32]),
     "hddm_lb_last_stats_n": (_i32, [_vp, _vp, _i32]),
@@ -380,6 +382,13 @@ class Context:
         v = np.zeros(5, np.int32)
         _check(lib.hddm_lb_last_config(self._h, _ptr(v)))
         return dict(zip(("formulation", "waves", "tiles", "tuned", "cache_hits"), (int(a) for a in v)))
+
+    def last_adjoint_config(self):
+        """Configuration of the last adjoint: dict(kernels (1 per-tile, 2 wave-specialized),
+        owned_copies (0: dS by atomics), ds_copies (summed after), max_segments)."""
+        v = np.zeros(4, np.int32)
+        _check(lib.hddm_lb_last_adjoint_config(self._h, _ptr(v)))
+        return dict(zip(("kernels", "owned_copies", "ds_copies", "max_segments"), (int(a) for a in v)))
 
     def device_status(self, clear=True):
         """Status bits raised by this context's kernels since the last clearing call (0: none)."""

@@ -73,6 +73,12 @@ typedef struct hddm_lb_grid hddm_lb_grid;
 #define HDDM_LB_OPT_DEVICE_LAYOUT 12 /* 1 (default): hddm_lb_upload_state builds the node layout (trie
                                         order, op stream, surplus splits, factor table) on the device,
                                         bit for bit as the host build; 0: on the host */
+#define HDDM_LB_OPT_ADJOINT_OWNED 13 /* 1: the wave-specialized adjoint adds dS through owned units
+                                        (one workgroup per row of one of K dS copies, plain loads and
+                                        stores, the copies summed in a fixed order): dS bitwise
+                                        repeatable, when the copies fit (at most 16 GiB and an eighth
+                                        of device memory); slower than atomics at D=59 (one 65,536-
+                                        point F32 call 551 against 342 ms). 0 (default): FP64 atomics */
 
 /* bits of the device status word (hddm_lb_device_status) */
 #define HDDM_LB_STATUS_WAIT_LIMIT 1u /* a wave of a wave-specialized kernel stopped waiting for another
@@ -161,8 +167,10 @@ int hddm_lb_forward_host(hddm_lb_ctx* ctx, int64_t npoints,
  *             the uploaded node order, row length dofs(s): dS_k = sum_p B_k(x_p) gbar_p
  * Both outputs are overwritten (dx only in its first dim columns, and only in
  * rows whose state id is valid); either may be NULL. The basis is recomputed.
- * dx is summed in a fixed order (bitwise repeatable); dsurplus is summed with
- * FP64 atomics within each XCD, so its last bits may vary between runs.
+ * dx is summed in a fixed order (bitwise repeatable). dsurplus is bitwise
+ * repeatable on the wave-specialized kernel with owned units
+ * (HDDM_LB_OPT_ADJOINT_OWNED); elsewhere it is summed with FP64 atomics and
+ * its last bits may vary between runs.
  * precision: HDDM_LB_FP64, or HDDM_LB_F32 (held to 1e-6 normwise: on the
  * wave-specialized kernel the products g.S and B^T g run in F32 on the MFMA
  * units with FP64 accumulation; the per-tile kernels compute in FP64 for
@@ -228,6 +236,12 @@ int hddm_lb_host_call_stats(hddm_lb_ctx* ctx, int64_t* requests, int64_t* batche
  * workgroup, [2] dof tiles of 32 per launch, [3] keys the autotuner has
  * tuned on this context, [4] forwards it served from its cache. */
 int hddm_lb_last_config(hddm_lb_ctx* ctx, int* config);
+
+/* Configuration of the last hddm_lb_adjoint: config[0] kernels (1 per-tile,
+ * 2 wave-specialized), [1] dS copies of owned units (0: dS added with
+ * atomics), [2] dS copies summed after the kernels (1: none), [3] most node
+ * segments of a state in the kernels' tables. */
+int hddm_lb_last_adjoint_config(hddm_lb_ctx* ctx, int* config);
 
 /* ---- host-side node layout, for inspection without a device ---- */
 typedef struct hddm_lb_layout hddm_lb_layout;

@@ -14,8 +14,10 @@ Checks, in FP64:
     each coordinate, so the difference quotient is exact up to rounding);
   * dS against B^T g written out from the oracle's basis on a tiny grid;
   * the torch.autograd.Function harness on the same inputs.
-The dsurplus sums use FP64 atomics; their order varies, hence 1e-12 and not
-bitwise.
+On the per-tile kernels and on the wave-specialized kernel without owned
+units the dsurplus sums use FP64 atomics; their order varies, hence 1e-12
+and not bitwise. With owned units (one workgroup adds to each row of one of
+K dS copies; HDDM_LB_OPT_ADJOINT_OWNED) dsurplus is bitwise repeatable.
 """
 import numpy as np
 import pytest
@@ -189,6 +191,84 @@ def test_adjoint_multistate_offsets(torch_mod):
     ctx.close()
 
 
+def owned_pair(torch, cases, X, G, sid, prec, seg_nodes):
+    """The wave-specialized adjoint twice on the same states: dS through
+    owned units, then through atomics. Returns ((dx, ds) owned, (dx, ds) a
+    second owned call, (dx, ds) atomics, the owned call's configuration)."""
+    out = []
+    for owned in (1, 0):
+        ctx = adj_ctx(2)
+        ctx.set_option(hddm_lb.OPT_SEGMENT_NODES, seg_nodes)
+        ctx.set_option(hddm_lb.OPT_ADJOINT_OWNED, owned)
+        for s, (ii, jj, S) in enumerate(cases):
+            ctx.upload_state(s, ii, jj, S)
+        out.append(run_adjoint(torch, ctx, X, G, sid, prec))
+        cfg = ctx.last_adjoint_config()
+        say("  owned option", owned, cfg)
+        assert cfg["kernels"] == 2
+        if owned:
+            assert cfg["owned_copies"] >= 2
+            owned_cfg = cfg
+            out.append(run_adjoint(torch, ctx, X, G, sid, prec))
+        else:
+            assert cfg["owned_copies"] == 0
+        ctx.close()
+    return out[0], out[1], out[2], owned_cfg
+
+
+@pytest.mark.parametrize("prec", [hddm_lb.FP64, hddm_lb.F32])
+def test_adjoint_owned_units(torch_mod, prec):
+    """dS through owned units (segments of ~100 nodes, so every workgroup
+    owns several (state, segment, copy) units): dx equal to the atomic
+    path's to the last bit (same kernel, same segments), dS equal to it to
+    rounding, both against the oracle, and dS bitwise repeatable."""
+    ii, jj, S, X, G = grid_case(40, 2, 5, 0.4, 88, 6000, 96, 2000, 5)
+    say("owned units D=40 nno", ii.shape[0], "dofs 96, 2000 points, precision", prec)
+    own, own2, atom, cfg = owned_pair(torch_mod, [(ii, jj, S)], X, G, None, prec, 100)
+    assert np.isfinite(own[0]).all() and np.isfinite(own[1]).all()
+    assert np.array_equal(own[0], atom[0])
+    assert np.array_equal(own[1], own2[1]) and np.array_equal(own[0], own2[0])
+    ds_o, ds_a = own[1].reshape(-1, 96), atom[1].reshape(-1, 96)
+    assert normwise_rel_err(ds_o, ds_a) <= 1e-14
+    dxo, dso = oracle.adjoint(ii, jj, S, X, G, nthreads=8)
+    say("  dx err", normwise_rel_err(own[0], dxo), "ds err", normwise_rel_err(ds_o, dso))
+    assert normwise_rel_err(own[0], dxo) <= TOL_ADJ[prec][0]
+    assert normwise_rel_err(ds_o, dso) <= TOL_ADJ[prec][1]
+
+
+def test_adjoint_owned_units_multistate(torch_mod):
+    """Owned units over three states of different node counts and dofs 24,
+    64 and 130 (five dof tiles: two launches, the second with one state
+    only), points in random state order: every state's dS rows against the
+    oracle and the atomic path; dS bitwise repeatable."""
+    dofs = (24, 64, 130)
+    cases, Xs = [], []
+    for s, d in enumerate(dofs):
+        ii, jj, S, X, _ = grid_case(36, 2, 5, 0.4, 120 + s, 1500 + 1500 * s, d, 400, 5)
+        cases.append((ii, jj, S))
+        Xs.append(X)
+    X = np.concatenate(Xs)
+    sid = np.repeat(np.arange(3), 400).astype(np.int32)
+    rng = np.random.default_rng(9)
+    perm = rng.permutation(X.shape[0])
+    X, sid = X[perm], sid[perm]
+    G = rng.standard_normal((X.shape[0], 130))
+    own, own2, atom, cfg = owned_pair(torch_mod, cases, X, G, sid, hddm_lb.FP64, 60)
+    assert np.array_equal(own[1], own2[1])
+    assert np.array_equal(own[0], atom[0])
+    for s, (ii, jj, S) in enumerate(cases):
+        sel = sid == s
+        d = S.shape[1]
+        dxo, dso = oracle.adjoint(ii, jj, S, X[sel], G[sel, :d])
+        off = sum(c[0].shape[0] * c[2].shape[1] for c in cases[:s])
+        dss = own[1][off:off + ii.shape[0] * d].reshape(ii.shape[0], d)
+        dsa = atom[1][off:off + ii.shape[0] * d].reshape(ii.shape[0], d)
+        say("  state", s, "dx err", normwise_rel_err(own[0][sel], dxo), "ds err", normwise_rel_err(dss, dso))
+        assert normwise_rel_err(dss, dsa) <= 1e-14
+        assert normwise_rel_err(own[0][sel], dxo) <= 1e-10
+        assert normwise_rel_err(dss, dso) <= 1e-12
+
+
 def test_autograd_harness(torch_mod):
     torch = torch_mod
     import hddm_lb_autograd
@@ -262,16 +342,22 @@ def test_adjoint_padded_dx_rows(torch_mod):
     ctx.close()
 
 
-def test_adjoint_dx_bitwise_repeatable(torch_mod):
+@pytest.mark.parametrize("dim,dofs,form,prec", [(20, 32, 0, hddm_lb.FP64), (40, 96, 2, hddm_lb.F32),
+                                                (40, 96, 2, hddm_lb.FP64)])
+def test_adjoint_dx_bitwise_repeatable(torch_mod, dim, dofs, form, prec):
     """Node segments split every state here; the per-segment dx partials are
-    summed in a fixed order, so two runs agree to the last bit."""
+    summed in a fixed order, so two runs agree to the last bit. On the
+    wave-specialized kernel (form 2) the points are put in a stable order and
+    each item's chunks alternate between the two dx waves from the item's
+    start, so its two waves' sum is the same in every run too."""
     torch = torch_mod
-    ii, jj, S, X, G = grid_case(20, 2, 5, 0.4, 55, 20000, 32, 4096, 5)
-    ctx = hddm_lb.Context(0)
+    ii, jj, S, X, G = grid_case(dim, 2, 5, 0.4, 55, 20000, dofs, 4096, 5)
+    ctx = adj_ctx(form)
     ctx.set_option(hddm_lb.OPT_SEGMENT_NODES, 2000)
     ctx.upload_state(0, ii, jj, S)
-    a, _ = run_adjoint(torch, ctx, X, G)
-    b, _ = run_adjoint(torch, ctx, X, G)
+    a, _ = run_adjoint(torch, ctx, X, G, prec=prec)
+    b, _ = run_adjoint(torch, ctx, X, G, prec=prec)
+    assert ctx.last_adjoint_config()["kernels"] == (2 if form == 2 else 1)
     assert np.isfinite(a).all()
     assert np.array_equal(a, b)
     ctx.close()
