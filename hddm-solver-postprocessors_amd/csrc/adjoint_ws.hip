@@ -306,7 +306,10 @@ struct AdjCons<HDDM_LB_FP64>
 	{
 		const int col = lane & 15, kq = lane >> 4;
 		const bool on = (mask >> col) & 1u;
+		// the chunk's uploaded node indices, one coalesced load (lane i holds
+		// node i % 16's), passed to the lanes that need them by shuffles
 		const int32_t* order = ldc(&sd->order);
+		const int ol = on ? order[chunk * 16 + col] : 0;
 		(void)a;
 		double* rowp[4];
 		double old[4][2];
@@ -315,7 +318,7 @@ struct AdjCons<HDDM_LB_FP64>
 		{
 			const int nl = 4 * e + kq;
 			const bool live = (mask >> nl) & 1u;
-			rowp[e] = dsb + (int64_t)(live ? ldc(order + chunk * 16 + nl) : 0) * dofs;
+			rowp[e] = dsb + (int64_t)__shfl(ol, nl, 64) * dofs;
 #pragma unroll
 			for (int db = 0; db < 2; db++)
 			{
@@ -409,7 +412,10 @@ struct AdjCons<HDDM_LB_F32>
 	{
 		const int col = lane & 31, kh = lane >> 5;
 		const bool on = (mask >> col) & 1u;
+		// the chunk's uploaded node indices, one coalesced load (lane i holds
+		// node i % 32's), passed to the lanes that need them by shuffles
 		const int32_t* order = ldc(&sd->order);
+		const int ol = on ? order[chunk * 32 + col] : 0;
 		(void)a;
 		const int dof = 32 * tile + col;
 		double old[16];
@@ -420,7 +426,8 @@ struct AdjCons<HDDM_LB_F32>
 			{
 				const int nl = 8 * (r >> 2) + 4 * kh + (r & 3);
 				const bool live = ((mask >> nl) & 1u) && dof < dofs;
-				old[r] = live ? dsb[(int64_t)ldc(order + chunk * 32 + nl) * dofs + dof] : 0.0;
+				const int o = __shfl(ol, nl, 64);
+				old[r] = live ? dsb[(int64_t)o * dofs + dof] : 0.0;
 			}
 		}
 		f16_t acc;
@@ -436,8 +443,9 @@ struct AdjCons<HDDM_LB_F32>
 		for (int r = 0; r < 16; r++)
 		{
 			const int nl = 8 * (r >> 2) + 4 * kh + (r & 3);
+			const int o = __shfl(ol, nl, 64);
 			if (!((mask >> nl) & 1u) || dof >= dofs) continue;
-			double* p = dsb + (int64_t)ldc(order + chunk * 32 + nl) * dofs + dof;
+			double* p = dsb + (int64_t)o * dofs + dof;
 			if (own) *p = old[r] + (double)acc[r];
 			else unsafeAtomicAdd(p, (double)acc[r]);
 		}
@@ -688,15 +696,19 @@ __global__ void __launch_bounds__(64 * (kANC + 3)) adjoint_ws_kernel(AdjointArgs
 		const u4_t* ftab = nullptr;
 		bool split = false;
 		d16_t xv0, xv1, xv2, xv3;
-		// w of node nl: the tiles' buffers summed in consumer order
+		// consumers whose buffers hold w for the current item (bit c)
+		uint32_t wcons = 0;
+		// w of node nl: the tiles' buffers summed in consumer order. Every
+		// buffer is read (one wait for all four reads) and an idle
+		// consumer's value selected away, which adds an exact zero.
 		auto wsum = [&](int par, int nl) -> double {
+			const T* b = wbuf(par, 0) + wpos(nl, lane);
+			T v[kANC];
+#pragma unroll
+			for (int c = 0; c < kANC; c++) v[c] = b[c * (int)(kWBuf / sizeof(T))];
 			double w = 0.0;
 #pragma unroll
-			for (int c = 0; c < kANC; c++)
-			{
-				const bool wc = split ? ((c & 1) && (c >> 1) < tpass) : c < tpass;
-				if (wc) w += (double)wbuf(par, c)[wpos(nl, lane)];
-			}
+			for (int c = 0; c < kANC; c++) w += ((wcons >> c) & 1u) ? (double)v[c] : 0.0;
 			return w;
 		};
 		// One chunk (message n): its factor rows load while the consumers
@@ -826,6 +838,9 @@ __global__ void __launch_bounds__(64 * (kANC + 3)) adjoint_ws_kernel(AdjointArgs
 				const int DP = ldc(&sd->dofs_pad);
 				tpass = min(kANC, (DP >> 5) - a.dt_base);
 				split = tpass <= kANC / 2;
+				wcons = 0;
+				for (int c = 0; c < kANC; c++)
+					if (split ? ((c & 1) && (c >> 1) < tpass) : c < tpass) wcons |= 1u << c;
 				ftab = (const u4_t*)ldc(&sd->ftab);
 				fs = (int)ldc(&sd->fstride);
 				nfac = ldc(&sd->max_depth);
