@@ -169,12 +169,39 @@ __device__ __forceinline__ void xpin4(const d16_t& xv0, const d16_t& xv1, const 
 	(void)xv1; (void)xv2; (void)xv3;
 }
 
+// Three coordinates (nodes of at most three factors).
+template <int XB>
+__device__ __forceinline__ void xpin3(const d16_t& xv0, const d16_t& xv1, const d16_t& xv2, const d16_t& xv3,
+	uint32_t i0, uint32_t i1, uint32_t i2, double& x0, double& x1, double& x2)
+{
+	uint32_t sv;
+#define HDDM_XP1(k) "s_set_gpr_idx_on %[i" #k "], gpr_idx(SRC0)\n" "v_mov_b64 %[v" #k "], v[64:65]\n" "s_set_gpr_idx_off\n"
+#define HDDM_XPIN3 "s_mov_b32 %[sv], m0\n" HDDM_XP1(0) HDDM_XP1(1) HDDM_XP1(2) "s_mov_b32 m0, %[sv]\n"
+#define HDDM_XPOUT [v0] "=&v"(x0), [v1] "=&v"(x1), [v2] "=&v"(x2), [sv] "=&s"(sv)
+#define HDDM_XPIDX [i0] "s"(i0), [i1] "s"(i1), [i2] "s"(i2)
+	if constexpr (XB == 1)
+		asm volatile(HDDM_XPIN3 : HDDM_XPOUT : HDDM_XPIDX, "{v[64:95]}"(xv0));
+	else if constexpr (XB == 2)
+		asm volatile(HDDM_XPIN3 : HDDM_XPOUT : HDDM_XPIDX, "{v[64:95]}"(xv0), "{v[96:127]}"(xv1));
+	else
+		asm volatile(HDDM_XPIN3 : HDDM_XPOUT : HDDM_XPIDX, "{v[64:95]}"(xv0), "{v[96:127]}"(xv1),
+			"{v[128:159]}"(xv2), "{v[160:191]}"(xv3));
+#undef HDDM_XPIDX
+#undef HDDM_XPOUT
+#undef HDDM_XPIN3
+#undef HDDM_XP1
+	(void)xv1; (void)xv2; (void)xv3;
+}
+
 // phi' of a factor at u = x i - j: -i sign(u) inside the support (phi =
 // 1 - |u| > 0), 0 outside it and at the peak. Selects only (no branch).
-__device__ __forceinline__ double dphi(double u, double iv, double phi)
+// ih is the high word of i (i's low word is zero), so -i sign(u) is i with
+// its sign bit flipped where u > 0: one xor on the high word, then the select.
+__device__ __forceinline__ double dphi(double u, uint32_t ih, double phi)
 {
-	const double s = u > 0.0 ? -iv : iv;
-	return ((phi > 0.0) & (u != 0.0)) ? s : 0.0;
+	const uint32_t flip = ~(uint32_t)__double2hiint(u) & 0x80000000u;
+	const bool on = (phi > 0.0) & (u != 0.0);
+	return __hiloint2double(on ? (int)(ih ^ flip) : 0, 0);
 }
 
 // dx terms of the surviving nodes of one chunk at the block's points. The
@@ -231,15 +258,19 @@ struct DxNodes
 				}
 			}
 			double xs[4];
-			xpin4<XB>(xv0, xv1, xv2, xv3, di[0], di[1], di[2], di[3], xs[0], xs[1], xs[2], xs[3]);
+			if constexpr (NS == 3)   // one group of three slots (f0 == 0)
+			{
+				xpin3<XB>(xv0, xv1, xv2, xv3, di[0], di[1], di[2], xs[0], xs[1], xs[2]);
+				xs[3] = xs[0];
+			}
+			else xpin4<XB>(xv0, xv1, xv2, xv3, di[0], di[1], di[2], di[3], xs[0], xs[1], xs[2], xs[3]);
 #pragma unroll
 			for (int k = 0; k < 4; k++)
 			{
 				if (f0 + k >= NS) continue;
-				const double iv = f64_from_high(ih[k]);
-				const double u = fma(xs[k], iv, f64_from_high(jh[k]));
+				const double u = fma(xs[k], f64_from_high(ih[k]), f64_from_high(jh[k]));
 				phi[f0 + k] = fmax(1.0 - fabs(u), 0.0);
-				dp[f0 + k] = dphi(u, iv, phi[f0 + k]);
+				dp[f0 + k] = dphi(u, ih[k], phi[f0 + k]);
 			}
 		}
 		double suf[NS];
