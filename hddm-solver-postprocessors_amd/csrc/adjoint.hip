@@ -602,12 +602,15 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 	int64_t ds_total = 0;
 	if ((rc = hddm_lb_surplus_offset(ctx, -1, nullptr, &ds_total))) return rc;
 
-	// automatic: the wave-specialized kernel for wide states (D > 32), which
-	// walks each trie once instead of once per dof tile (measured: D=59, 128
-	// dofs, 2^16 points: FP64 1255 -> 838 ms; its F32 consumers take dS to the
-	// MFMA units); the per-tile kernels otherwise (D=20, 32 dofs: FP64 32 ms
-	// against 110 ms wave-specialized, F32 105 ms). The per-tile kernels compute
-	// in FP64 whatever the precision asked for, which meets the F32 bound too.
+	// automatic: the wave-specialized kernel, which walks each trie once for
+	// four dof tiles instead of once per tile, for states of two or more dof
+	// tiles and for wide states (D > 32) in F32; the per-tile kernels for one
+	// tile at D <= 32, where their edge-sum walk beats the dx waves (65,536
+	// points, F32: D=24 / 12k nodes / 128 dofs 8.7 ms against 38.7 per-tile,
+	// D=16 / 60k / 128 dofs 31.5 against 56.3, D=32 / 40k / 64 dofs 24.5
+	// against 49.1, D=20 / 100k / 64 dofs 57.1 against 64.3; D=20 / 100k /
+	// 32 dofs 56.5 against 31.3). The per-tile kernels compute in FP64
+	// whatever the precision asked for, which meets the F32 bound too.
 	const bool wide = xbanks_for(ctx->dim) == 4;
 	// nodes of more than 8 factors next to four coordinate banks exceed the
 	// wave-specialized kernel's registers
@@ -616,7 +619,8 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 		return set_error(HDDM_LB_ERROR_UNSUPPORTED,
 			"adjoint: the wave-specialized kernel takes nodes of at most 8 factors when dim > 32");
 	const bool ws = ctx->opt_form == 2
-		|| (ctx->opt_form == 0 && wide && ws_fits && (precision == HDDM_LB_F32 || ctx->dt_needed >= 2));
+		|| (ctx->opt_form == 0 && ws_fits
+			&& (ctx->dt_needed >= 2 || (wide && precision == HDDM_LB_F32)));
 	// the wave-specialized kernel's launch shape: one walker wave, four
 	// consumer waves and two dx waves per workgroup, four dof tiles per
 	// launch; the second dx wave works when its rows fit the LDS
