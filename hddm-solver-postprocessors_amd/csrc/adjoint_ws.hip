@@ -169,7 +169,31 @@ __device__ __forceinline__ void xpin4(const d16_t& xv0, const d16_t& xv1, const 
 	(void)xv1; (void)xv2; (void)xv3;
 }
 
-// Three coordinates (nodes of at most three factors).
+// Two coordinates.
+template <int XB>
+__device__ __forceinline__ void xpin2(const d16_t& xv0, const d16_t& xv1, const d16_t& xv2, const d16_t& xv3,
+	uint32_t i0, uint32_t i1, double& x0, double& x1)
+{
+	uint32_t sv;
+#define HDDM_XP1(k) "s_set_gpr_idx_on %[i" #k "], gpr_idx(SRC0)\n" "v_mov_b64 %[v" #k "], v[64:65]\n" "s_set_gpr_idx_off\n"
+#define HDDM_XPIN2 "s_mov_b32 %[sv], m0\n" HDDM_XP1(0) HDDM_XP1(1) "s_mov_b32 m0, %[sv]\n"
+#define HDDM_XPOUT [v0] "=&v"(x0), [v1] "=&v"(x1), [sv] "=&s"(sv)
+#define HDDM_XPIDX [i0] "s"(i0), [i1] "s"(i1)
+	if constexpr (XB == 1)
+		asm volatile(HDDM_XPIN2 : HDDM_XPOUT : HDDM_XPIDX, "{v[64:95]}"(xv0));
+	else if constexpr (XB == 2)
+		asm volatile(HDDM_XPIN2 : HDDM_XPOUT : HDDM_XPIDX, "{v[64:95]}"(xv0), "{v[96:127]}"(xv1));
+	else
+		asm volatile(HDDM_XPIN2 : HDDM_XPOUT : HDDM_XPIDX, "{v[64:95]}"(xv0), "{v[96:127]}"(xv1),
+			"{v[128:159]}"(xv2), "{v[160:191]}"(xv3));
+#undef HDDM_XPIDX
+#undef HDDM_XPOUT
+#undef HDDM_XPIN2
+#undef HDDM_XP1
+	(void)xv1; (void)xv2; (void)xv3;
+}
+
+// Three coordinates.
 template <int XB>
 __device__ __forceinline__ void xpin3(const d16_t& xv0, const d16_t& xv1, const d16_t& xv2, const d16_t& xv3,
 	uint32_t i0, uint32_t i1, uint32_t i2, double& x0, double& x1, double& x2)
@@ -257,13 +281,26 @@ struct DxNodes
 					ih[k] = jh[k] = 0u;
 				}
 			}
+			// the group's coordinates: only as many as it has slots (the
+			// count is a constant once the loop is unrolled)
 			double xs[4];
-			if constexpr (NS == 3)   // one group of three slots (f0 == 0)
+			const int rem = NS - f0;
+			if (rem >= 4) xpin4<XB>(xv0, xv1, xv2, xv3, di[0], di[1], di[2], di[3], xs[0], xs[1], xs[2], xs[3]);
+			else if (rem == 3)
 			{
 				xpin3<XB>(xv0, xv1, xv2, xv3, di[0], di[1], di[2], xs[0], xs[1], xs[2]);
 				xs[3] = xs[0];
 			}
-			else xpin4<XB>(xv0, xv1, xv2, xv3, di[0], di[1], di[2], di[3], xs[0], xs[1], xs[2], xs[3]);
+			else if (rem == 2)
+			{
+				xpin2<XB>(xv0, xv1, xv2, xv3, di[0], di[1], xs[0], xs[1]);
+				xs[2] = xs[3] = xs[0];
+			}
+			else
+			{
+				xs[0] = xpin<XB>(xv0, xv1, xv2, xv3, di[0]);
+				xs[1] = xs[2] = xs[3] = xs[0];
+			}
 #pragma unroll
 			for (int k = 0; k < 4; k++)
 			{
@@ -839,8 +876,8 @@ __global__ void __launch_bounds__(64 * (kANC + 3)) adjoint_ws_kernel(AdjointArgs
 				{
 					const int chunk = uniform(m->a);
 					const uint32_t mask = (uint32_t)uniform(m->b);
-					// the state's table stride (4, 8, 16 slots) and, below 4,
-					// the slots its nodes use
+					// the state's table stride (4, 8, 16 slots) and the slots
+					// its nodes use (2, 3, 4 of 4; 5, 6, 8 of 8)
 					bool ok = true;
 					if (fs == 4)
 					{
@@ -848,7 +885,12 @@ __global__ void __launch_bounds__(64 * (kANC + 3)) adjoint_ws_kernel(AdjointArgs
 						else if (nfac == 3) ok = run_chunk(DxNodes<XB, 4, 3, CB>(), chunk, mask, par, n);
 						else ok = run_chunk(DxNodes<XB, 4, 4, CB>(), chunk, mask, par, n);
 					}
-					else if (fs == 8) ok = run_chunk(DxNodes<XB, 8, 8, CB>(), chunk, mask, par, n);
+					else if (fs == 8)
+					{
+						if (nfac <= 5) ok = run_chunk(DxNodes<XB, 8, 5, CB>(), chunk, mask, par, n);
+						else if (nfac == 6) ok = run_chunk(DxNodes<XB, 8, 6, CB>(), chunk, mask, par, n);
+						else ok = run_chunk(DxNodes<XB, 8, 8, CB>(), chunk, mask, par, n);
+					}
 					// 16 factor slots next to four coordinate banks would spill:
 					// the host runs such states on the per-tile kernels
 					else if constexpr (XB < 4) ok = run_chunk(DxNodes<XB, 16, 16, CB>(), chunk, mask, par, n);

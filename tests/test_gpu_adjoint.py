@@ -96,6 +96,30 @@ def test_adjoint_matches_oracle(torch_mod, dim, dofs, n, prec, form):
     ctx.close()
 
 
+@pytest.mark.parametrize("prec", [hddm_lb.FP64, hddm_lb.F32])
+@pytest.mark.parametrize("start,maxl,maxf", [(5, 6, 5), (6, 7, 6)])
+def test_adjoint_five_and_six_factor_nodes(torch_mod, start, maxl, maxf, prec):
+    """Regular D=6 grids whose nodes have up to 5 and 6 factors: the
+    wave-specialized kernel's dx waves read 5 and 6 of the 8 factor slots.
+    Against the oracle, 64 dofs (two tiles)."""
+    ii, jj, lv, _ = hddm_lb.generate_grid(6, start, maxl, 0.0, 1, 0)
+    S = hddm_lb.synthetic_surplus(lv, 64, 2)
+    X = hddm_lb.synthetic_points(200, 6, 3, 0.01, 0.99, kink_level=maxl, kink_gap=1e-3)
+    G = np.random.default_rng(4).standard_normal((200, 64))
+    ctx = adj_ctx(2)
+    ctx.upload_state(0, ii, jj, S)
+    assert ctx.state_info(0)["max_factors"] == maxf
+    say("adjoint D=6 nno", ii.shape[0], "max factors", maxf, "precision", prec)
+    dx, ds = run_adjoint(torch_mod, ctx, X, G, prec=prec)
+    assert ctx.last_adjoint_config()["kernels"] == 2
+    dxo, dso = oracle.adjoint(ii, jj, S, X, G, nthreads=8)
+    ds = ds.reshape(ii.shape[0], 64)
+    say("  dx err", normwise_rel_err(dx, dxo), "ds err", normwise_rel_err(ds, dso))
+    assert normwise_rel_err(dx, dxo) <= TOL_ADJ[prec][0]
+    assert normwise_rel_err(ds, dso) <= TOL_ADJ[prec][1]
+    ctx.close()
+
+
 def test_adjoint_kernels_agree(torch_mod):
     """The wave-specialized FP64 adjoint and the per-tile FP64 kernels give
     the same dx and dS to rounding (four dof tiles, node segments on)."""
