@@ -609,7 +609,7 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 	// points, F32: D=24 / 12k nodes / 128 dofs 8.7 ms against 38.7 per-tile,
 	// D=16 / 60k / 128 dofs 24.9 against 56.3, D=32 / 40k / 64 dofs 24.5
 	// against 49.1, D=20 / 100k / 64 dofs 43.3 against 65.9; D=20 / 100k /
-	// 32 dofs 56.5 against 31.3). The per-tile kernels compute in FP64
+	// 32 dofs 42.7 against 31.3). The per-tile kernels compute in FP64
 	// whatever the precision asked for, which meets the F32 bound too.
 	const bool wide = xbanks_for(ctx->dim) == 4;
 	// nodes of more than 8 factors next to four coordinate banks exceed the
