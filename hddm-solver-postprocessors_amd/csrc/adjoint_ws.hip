@@ -418,15 +418,25 @@ struct AdjCons<HDDM_LB_FP64>
 		}
 	}
 
-	__device__ __forceinline__ void w(const StateDesc* sd, double* Wb, int chunk, uint32_t mask, int tile, int DP, int lane)
+	// The surplus rows of the w product, loaded by wrows() before the dS
+	// product so that they arrive meanwhile.
+	struct WRows { double v[8]; };
+
+	__device__ __forceinline__ void wrows(const StateDesc* sd, int chunk, uint32_t mask, int tile, int DP, int lane,
+		WRows& sv)
 	{
 		const int col = lane & 15, kq = lane >> 4;
 		const double* s64 = ldc(&sd->s64);
 		const bool on = (mask >> col) & 1u;
-		double sv[8];
 		const double* srow = s64 + (size_t)(chunk * 16 + col) * DP + 32 * tile + kq;
 #pragma unroll
-		for (int s = 0; s < 8; s++) sv[s] = on ? srow[4 * s] : 0.0;
+		for (int s = 0; s < 8; s++) sv.v[s] = on ? srow[4 * s] : 0.0;
+	}
+
+	__device__ __forceinline__ void w(const WRows& sr, double* Wb, int lane)
+	{
+		const int col = lane & 15, kq = lane >> 4;
+		const double (&sv)[8] = sr.v;
 		d4_t acc[4];
 #pragma unroll
 		for (int pb = 0; pb < 4; pb++) acc[pb] = d4_t{ 0.0, 0.0, 0.0, 0.0 };
@@ -519,15 +529,25 @@ struct AdjCons<HDDM_LB_F32>
 		}
 	}
 
-	__device__ __forceinline__ void w(const StateDesc* sd, float* Wb, int chunk, uint32_t mask, int tile, int DP, int lane)
+	// The surplus rows of the w product, loaded by wrows() before the dS
+	// product so that they arrive meanwhile.
+	struct WRows { float v[16]; };
+
+	__device__ __forceinline__ void wrows(const StateDesc* sd, int chunk, uint32_t mask, int tile, int DP, int lane,
+		WRows& sv)
 	{
 		const int col = lane & 31, kh = lane >> 5;
 		const float* s32r = ldc(&sd->s32r);
 		const bool on = (mask >> col) & 1u;
-		float sv[16];
 		const float* srow = s32r + (size_t)(chunk * 32 + col) * DP + 32 * tile + kh;
 #pragma unroll
-		for (int s = 0; s < 16; s++) sv[s] = on ? srow[2 * s] : 0.0f;
+		for (int s = 0; s < 16; s++) sv.v[s] = on ? srow[2 * s] : 0.0f;
+	}
+
+	__device__ __forceinline__ void w(const WRows& sr, float* Wb, int lane)
+	{
+		const int col = lane & 31, kh = lane >> 5;
+		const float (&sv)[16] = sr.v;
 		f16_t acc[2];
 #pragma unroll
 		for (int h = 0; h < 2; h++)
@@ -973,6 +993,9 @@ __global__ void __launch_bounds__(64 * (kANC + 3)) adjoint_ws_kernel(AdjointArgs
 				const uint32_t mask = (uint32_t)uniform(m->b);
 				const T* Bc = slot(n);
 				const uint64_t t1 = clk();
+				// the w product's surplus rows load during the dS product
+				typename AdjCons<PREC>::WRows sr;
+				if (role & kRoleW) cons.wrows(sd, chunk, mask, tile, DP, lane, sr);
 				if ((role & kRoleDs) && a.ds) cons.ds(a, sd, dsb, Bc, chunk, mask, tile, dofs, lane, a.ds_own > 0);
 				const uint64_t t2 = clk();
 				if (role & kRoleW)
@@ -981,7 +1004,7 @@ __global__ void __launch_bounds__(64 * (kANC + 3)) adjoint_ws_kernel(AdjointArgs
 					// previous chunk
 					if (wmsg[par] >= 0 && !a_wait(&sy->done[kANC + dxw(par)], wmsg[par] + 1, sy, a.status)) break;
 					wmsg[par] = n;
-					cons.w(sd, wbuf(par, c), chunk, mask, tile, DP, lane);
+					cons.w(sr, wbuf(par, c), lane);
 				}
 				cyc_ds += t2 - t1;
 				cyc_wc += clk() - t2;
