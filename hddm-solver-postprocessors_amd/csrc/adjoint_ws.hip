@@ -23,10 +23,10 @@
 //        written to an LDS buffer of its own (two buffers, by the chunk's
 //        parity within the item).
 // A dx wave takes each chunk of its parity (counted from the item's start)
-// once every consumer is done with it. For each surviving node k of the chunk it sums the tiles' w in
-// consumer order, reads k's factors phi_f (dim d_f) from the per-node factor
-// table, recomputes them at the block's points with the walk's own FP64
-// operations, and adds
+// once every consumer is done with it. For each surviving node k of the
+// chunk it sums the tiles' w in consumer order, reads k's factors phi_f (dim
+// d_f) from the per-node factor table, recomputes them at the block's points
+// with the walk's own FP64 operations, and adds
 //   w_pk (prod_{g != f} phi_g) phi_f'       (prefix and suffix products)
 // to dx[d_f] in LDS rows of its own: no division, and a factor that is zero
 // contributes exactly zero. Each wave adds in a fixed order and the two
