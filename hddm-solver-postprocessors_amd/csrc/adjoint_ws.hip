@@ -789,21 +789,30 @@ __global__ void __launch_bounds__(64 * (kANC + 3)) adjoint_ws_kernel(AdjointArgs
 		// w of node nl: the tiles' buffers summed in consumer order. Every
 		// buffer is read (one wait for all four reads) and an idle
 		// consumer's value selected away, which adds an exact zero.
-		auto wsum = [&](int par, int nl) -> double {
+		// With every consumer holding w (all4, the common case of states of
+		// four or more tiles) the selects are left out; the order of the
+		// additions is the same.
+		static_assert(kANC == 4, "the all-consumer sum is written for four consumers");
+		auto wsum = [&](int par, int nl, auto all4) -> double {
 			const T* b = wbuf(par, 0) + wpos(nl, lane);
 			T v[kANC];
 #pragma unroll
 			for (int c = 0; c < kANC; c++) v[c] = b[c * (int)(kWBuf / sizeof(T))];
-			double w = 0.0;
+			if constexpr (decltype(all4)::value)
+				return (((double)v[0] + (double)v[1]) + (double)v[2]) + (double)v[3];
+			else
+			{
+				double w = 0.0;
 #pragma unroll
-			for (int c = 0; c < kANC; c++) w += ((wcons >> c) & 1u) ? (double)v[c] : 0.0;
-			return w;
+				for (int c = 0; c < kANC; c++) w += ((wcons >> c) & 1u) ? (double)v[c] : 0.0;
+				return w;
+			}
 		};
 		// One chunk (message n): its factor rows load while the consumers
 		// finish it; then its nodes in order, two at a time (a lone last node
 		// is paired with itself at w = 0, which adds zeros), the next pair's
 		// w read first. False when a wait gave up.
-		auto run_chunk = [&](auto&& dn, int chunk, uint32_t mask, int par, int n) -> bool {
+		auto run_chunk = [&](auto&& dn, int chunk, uint32_t mask, int par, int n, auto all4) -> bool {
 			dn.load(ftab, chunk, lane);
 			const uint64_t t1 = clk();
 #pragma unroll 1
@@ -817,8 +826,8 @@ __global__ void __launch_bounds__(64 * (kANC + 3)) adjoint_ws_kernel(AdjointArgs
 				const bool two = mask != 0u;
 				n1 = two ? __builtin_ctz(mask) : n0;
 				mask &= mask - 1u;
-				w0 = wsum(par, n0);
-				w1 = two ? wsum(par, n1) : 0.0;
+				w0 = wsum(par, n0, all4);
+				w1 = two ? wsum(par, n1, all4) : 0.0;
 			};
 			if (!mask) return true;
 			int n0, n1;
@@ -901,19 +910,28 @@ __global__ void __launch_bounds__(64 * (kANC + 3)) adjoint_ws_kernel(AdjointArgs
 					bool ok = true;
 					if (fs == 4)
 					{
-						if (nfac <= 2) ok = run_chunk(DxNodes<XB, 4, 2, CB>(), chunk, mask, par, n);
-						else if (nfac == 3) ok = run_chunk(DxNodes<XB, 4, 3, CB>(), chunk, mask, par, n);
-						else ok = run_chunk(DxNodes<XB, 4, 4, CB>(), chunk, mask, par, n);
+						const std::false_type some;
+						const std::true_type all;
+						if (wcons == 0xFu)
+						{
+							if (nfac <= 2) ok = run_chunk(DxNodes<XB, 4, 2, CB>(), chunk, mask, par, n, all);
+							else if (nfac == 3) ok = run_chunk(DxNodes<XB, 4, 3, CB>(), chunk, mask, par, n, all);
+							else ok = run_chunk(DxNodes<XB, 4, 4, CB>(), chunk, mask, par, n, all);
+						}
+						else if (nfac <= 2) ok = run_chunk(DxNodes<XB, 4, 2, CB>(), chunk, mask, par, n, some);
+						else if (nfac == 3) ok = run_chunk(DxNodes<XB, 4, 3, CB>(), chunk, mask, par, n, some);
+						else ok = run_chunk(DxNodes<XB, 4, 4, CB>(), chunk, mask, par, n, some);
 					}
 					else if (fs == 8)
 					{
-						if (nfac <= 5) ok = run_chunk(DxNodes<XB, 8, 5, CB>(), chunk, mask, par, n);
-						else if (nfac == 6) ok = run_chunk(DxNodes<XB, 8, 6, CB>(), chunk, mask, par, n);
-						else ok = run_chunk(DxNodes<XB, 8, 8, CB>(), chunk, mask, par, n);
+						const std::false_type some;
+						if (nfac <= 5) ok = run_chunk(DxNodes<XB, 8, 5, CB>(), chunk, mask, par, n, some);
+						else if (nfac == 6) ok = run_chunk(DxNodes<XB, 8, 6, CB>(), chunk, mask, par, n, some);
+						else ok = run_chunk(DxNodes<XB, 8, 8, CB>(), chunk, mask, par, n, some);
 					}
 					// 16 factor slots next to four coordinate banks would spill:
 					// the host runs such states on the per-tile kernels
-					else if constexpr (XB < 4) ok = run_chunk(DxNodes<XB, 16, 16, CB>(), chunk, mask, par, n);
+					else if constexpr (XB < 4) ok = run_chunk(DxNodes<XB, 16, 16, CB>(), chunk, mask, par, n, std::false_type());
 					if (!ok) break;
 				}
 			}
