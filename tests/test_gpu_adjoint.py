@@ -215,10 +215,11 @@ def test_adjoint_multistate_offsets(torch_mod):
     ctx.close()
 
 
-def owned_pair(torch, cases, X, G, sid, prec, seg_nodes):
+def owned_pair(torch, cases, X, G, sid, prec, seg_nodes, one_copy=False):
     """The wave-specialized adjoint twice on the same states: dS through
     owned units, then through atomics. Returns ((dx, ds) owned, (dx, ds) a
-    second owned call, (dx, ds) atomics, the owned call's configuration)."""
+    second owned call, (dx, ds) atomics, the owned call's configuration).
+    one_copy: the owned call must add to dsurplus itself (K = 1)."""
     out = []
     for owned in (1, 0):
         ctx = adj_ctx(2)
@@ -231,7 +232,10 @@ def owned_pair(torch, cases, X, G, sid, prec, seg_nodes):
         say("  owned option", owned, cfg)
         assert cfg["kernels"] == 2
         if owned:
-            assert cfg["owned_copies"] >= 2
+            if one_copy:
+                assert cfg["owned_copies"] == 1 and cfg["ds_copies"] == 1
+            else:
+                assert cfg["owned_copies"] >= 2
             owned_cfg = cfg
             out.append(run_adjoint(torch, ctx, X, G, sid, prec))
         else:
@@ -258,6 +262,38 @@ def test_adjoint_owned_units(torch_mod, prec):
     say("  dx err", normwise_rel_err(own[0], dxo), "ds err", normwise_rel_err(ds_o, dso))
     assert normwise_rel_err(own[0], dxo) <= TOL_ADJ[prec][0]
     assert normwise_rel_err(ds_o, dso) <= TOL_ADJ[prec][1]
+
+
+def test_adjoint_owned_units_single_copy(torch_mod):
+    """Owned units with one dS copy: ten D=48 states split at every
+    top-level run (about 180 segments each) give every workgroup several
+    (state, segment) units, so each unit adds to dsurplus itself and nothing
+    is summed after. dS against the atomic path and the oracle, bitwise
+    repeatable."""
+    cases, Xs = [], []
+    for s in range(10):
+        ii, jj, S, X, _ = grid_case(48, 2, 5, 0.5, 300 + s, 4000, 64, 240, 5)
+        cases.append((ii, jj, S))
+        Xs.append(X)
+    X = np.concatenate(Xs)
+    sid = np.repeat(np.arange(10), 240).astype(np.int32)
+    rng = np.random.default_rng(12)
+    perm = rng.permutation(X.shape[0])
+    X, sid = X[perm], sid[perm]
+    G = rng.standard_normal((X.shape[0], 64))
+    own, own2, atom, cfg = owned_pair(torch_mod, cases, X, G, sid, hddm_lb.FP64, 1, one_copy=True)
+    assert np.array_equal(own[1], own2[1])
+    assert np.array_equal(own[0], atom[0])
+    assert normwise_rel_err(own[1].reshape(-1, 64), atom[1].reshape(-1, 64)) <= 1e-14
+    for s in (0, 7):
+        ii, jj, S = cases[s]
+        sel = sid == s
+        dxo, dso = oracle.adjoint(ii, jj, S, X[sel], G[sel])
+        off = sum(c[0].shape[0] * 64 for c in cases[:s])
+        dss = own[1][off:off + ii.shape[0] * 64].reshape(ii.shape[0], 64)
+        say("  state", s, "dx err", normwise_rel_err(own[0][sel], dxo), "ds err", normwise_rel_err(dss, dso))
+        assert normwise_rel_err(own[0][sel], dxo) <= 1e-10
+        assert normwise_rel_err(dss, dso) <= 1e-12
 
 
 def test_adjoint_owned_units_multistate(torch_mod):
