@@ -74,10 +74,17 @@ template <int PREC> using adj_t = typename std::conditional<PREC == HDDM_LB_FP64
 // one chunk buffer: nodes x padded row
 template <int PREC> constexpr size_t adj_buf_bytes() { return ((size_t)1 << adj_chunk_bits<PREC>()) * kARow * sizeof(adj_t<PREC>); }
 // one consumer's w buffer: nodes x 64 points, point p of node n at n * 64 +
-// (p ^ n) (the swizzle spreads the MFMA layout's column writes over banks
-// without padding)
+// (p ^ s(n)); the swizzle spreads the MFMA layout's column writes over banks
+// without padding. FP64: s(n) = n. F32: s(n) = 4 n mod 64, which keeps each
+// aligned group of four points together, so a lane writes four of them with
+// one 16-byte store (the FP64 consumer writes single values and measures
+// slightly faster with s(n) = n).
 template <int PREC> constexpr size_t adj_wbuf_bytes() { return ((size_t)1 << adj_chunk_bits<PREC>()) * kWave * sizeof(adj_t<PREC>); }
-__device__ __forceinline__ int wpos(int node, int point) { return node * kWave + (point ^ node); }
+template <typename T> __device__ __forceinline__ int wpos(int node, int point)
+{
+	if constexpr (sizeof(T) == 4) return node * kWave + (point ^ ((node << 2) & (kWave - 1)));
+	else return node * kWave + (point ^ node);
+}
 
 // LDS layout of the workgroup (bytes): kARing chunk slots, 2 x kANC w buffers,
 // messages, counters, the walk's path stack [PM][64], ndx sets of dx rows
@@ -449,7 +456,7 @@ struct AdjCons<HDDM_LB_FP64>
 #pragma unroll
 		for (int pb = 0; pb < 4; pb++)
 #pragma unroll
-			for (int e = 0; e < 4; e++) Wb[wpos(col, 16 * pb + 4 * e + kq)] = acc[pb][e];
+			for (int e = 0; e < 4; e++) Wb[wpos<double>(col, 16 * pb + 4 * e + kq)] = acc[pb][e];
 	}
 };
 
@@ -557,11 +564,15 @@ struct AdjCons<HDDM_LB_F32>
 #pragma unroll
 			for (int s = 0; s < 16; s++) acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(gw[h][s], sv[s], acc[h], 0, 0, 0);
 		}
-		// W[node][point]: node = lane % 32, point = 32 h + 8 (r / 4) + 4 (lane / 32) + r % 4
+		// W[node][point]: node = lane % 32, point = 32 h + 8 (r / 4) + 4 (lane / 32) + r % 4;
+		// the four r of one r / 4 are four consecutive points, one store
+		typedef float f4v_t __attribute__((ext_vector_type(4)));
 #pragma unroll
 		for (int h = 0; h < 2; h++)
 #pragma unroll
-			for (int r = 0; r < 16; r++) Wb[wpos(col, 32 * h + 8 * (r >> 2) + 4 * kh + (r & 3))] = acc[h][r];
+			for (int q = 0; q < 4; q++)
+				*(f4v_t*)(Wb + wpos<float>(col, 32 * h + 8 * q + 4 * kh)) =
+					f4v_t{ acc[h][4 * q], acc[h][4 * q + 1], acc[h][4 * q + 2], acc[h][4 * q + 3] };
 	}
 };
 
@@ -794,7 +805,7 @@ __global__ void __launch_bounds__(64 * (kANC + 3)) adjoint_ws_kernel(AdjointArgs
 		// additions is the same.
 		static_assert(kANC == 4, "the all-consumer sum is written for four consumers");
 		auto wsum = [&](int par, int nl, auto all4) -> double {
-			const T* b = wbuf(par, 0) + wpos(nl, lane);
+			const T* b = wbuf(par, 0) + wpos<T>(nl, lane);
 			T v[kANC];
 #pragma unroll
 			for (int c = 0; c < kANC; c++) v[c] = b[c * (int)(kWBuf / sizeof(T))];
