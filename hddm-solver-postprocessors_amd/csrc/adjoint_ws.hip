@@ -32,9 +32,10 @@
 // contributes exactly zero. Each wave adds in a fixed order and the two
 // waves' rows are summed in a fixed order, so dx is bitwise repeatable; it is
 // written out when the next item (or the exit) arrives.
-// FP64: v_mfma_f64_16x16x4f64 throughout. F32: v_mfma_f32_32x32x2f32 with
-// FP32 operands; dS partials of one chunk (64 points) are added in FP64, w is
-// summed over tiles in FP64.
+// FP64: v_mfma_f64_16x16x4f64 throughout. F32: v_mfma_f32_32x32x16_bf16 on
+// operands split into three bf16 parts (six terms per product, as accurate as
+// F32 operands and products); dS partials of one chunk (64 points) are added
+// in FP64, w is summed over tiles in FP64.
 // Synchronization is the forward WS sweep's: an LDS message ring with
 // release/acquire counters and bounded waits (sweep_ws.hip). The walker
 // reuses a ring slot once all six reader waves are done with it; a consumer
@@ -80,6 +81,7 @@ template <int PREC> constexpr size_t adj_buf_bytes() { return ((size_t)1 << adj_
 // one 16-byte store (the FP64 consumer writes single values and measures
 // slightly faster with s(n) = n).
 template <int PREC> constexpr size_t adj_wbuf_bytes() { return ((size_t)1 << adj_chunk_bits<PREC>()) * kWave * sizeof(adj_t<PREC>); }
+typedef float f4v_t __attribute__((ext_vector_type(4)));
 template <typename T> __device__ __forceinline__ int wpos(int node, int point)
 {
 	if constexpr (sizeof(T) == 4) return node * kWave + (point ^ ((node << 2) & (kWave - 1)));
@@ -460,33 +462,79 @@ struct AdjCons<HDDM_LB_FP64>
 	}
 };
 
-// F32, v_mfma_f32_32x32x2f32: A[i][k] from lane i + 32 k, B[k][j] from lane
-// j + 32 k, D[i][j] in register 4 (i / 8) + i % 4 of lane j + 32 ((i / 4) % 2).
-//   dS: D[node i][dof j] = sum over 32 steps of B[node i][point 2 s + k] g[point][dof j]
-//   w:  D[point 32 h + i][node j] = sum over 16 steps of g[point][dof 2 s + k] S[node j][dof]
+// F32 through bf16 parts, v_mfma_f32_32x32x16_bf16. Every operand is split
+// as x = x0 + x1 + x2, bf16 parts each rounded to nearest, so that
+// |x - x0 - x1 - x2| <= 2^-24 |x|; a product accumulates (in F32) the six
+// terms xa_p xb_q with p + q <= 2, smallest first, and the three left out are
+// below 2^-23 of it. The bf16 MFMA issues 8x the F32 one's flops per cycle,
+// so the six terms take 192 cycles per 16 steps of k where F32 takes 512.
+// A[i][k] from element k % 8 of lane i + 32 (k / 8), B[k][j] from element
+// k % 8 of lane j + 32 (k / 8), D[i][j] in register 4 (i / 8) + i % 4 of lane
+// j + 32 ((i / 4) % 2).
+//   dS: D[node i][dof j] = sum over 4 steps of B[node i][point 16 s + k] g[point][dof j]
+//   w:  D[point 32 h + i][node j] = sum over 2 steps of g[point][dof 16 s + k] S[node j][dof]
+__device__ __forceinline__ void split3_bf16(const float (&v)[8], b8_t (&t)[3])
+{
+	uint32_t h[4], m[4], l[4];
+#pragma unroll
+	for (int p = 0; p < 4; p++)
+	{
+		h[p] = pack_b2(v[2 * p], v[2 * p + 1]);
+		const float r0 = v[2 * p] - lo_half_bf16(h[p]), r1 = v[2 * p + 1] - hi_half_bf16(h[p]);
+		m[p] = pack_b2(r0, r1);
+		l[p] = pack_b2(r0 - lo_half_bf16(m[p]), r1 - hi_half_bf16(m[p]));
+	}
+	t[0] = as_vec8<b8_t>(h[0], h[1], h[2], h[3]);
+	t[1] = as_vec8<b8_t>(m[0], m[1], m[2], m[3]);
+	t[2] = as_vec8<b8_t>(l[0], l[1], l[2], l[3]);
+}
+
+// acc += a * b over the six terms of order <= 2 of the parts, smallest first
+__device__ __forceinline__ f16_t mfma_bf16x3(const b8_t (&a)[3], const b8_t (&b)[3], f16_t acc)
+{
+	acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[2], b[0], acc, 0, 0, 0);
+	acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1], b[1], acc, 0, 0, 0);
+	acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b[2], acc, 0, 0, 0);
+	acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1], b[0], acc, 0, 0, 0);
+	acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b[1], acc, 0, 0, 0);
+	return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b[0], acc, 0, 0, 0);
+}
+
 template <>
 struct AdjCons<HDDM_LB_F32>
 {
-	float gd[32];       // g[point 2 s + lane / 32][dof lane % 32]
-	float gw[2][16];    // g[point 32 h + lane % 32][dof 2 s + lane / 32]
+	b8_t gd[4][3];      // g[point 16 s + 8 (lane / 32) + e][dof lane % 32], parts
+	b8_t gw[2][2][3];   // g[point 32 h + lane % 32][dof 16 s + 8 (lane / 32) + e], parts
 
 	template <typename ORIG>
 	__device__ __forceinline__ void load(const AdjointArgs& a, ORIG orig, int np, int tile, int dofs, int role, int lane)
 	{
 		const int col = lane & 31, kh = lane >> 5;
 #pragma unroll
-		for (int s = 0; s < 32; s++)
+		for (int s = 0; s < 4; s++)
 		{
-			const int pt = 2 * s + kh, dof = 32 * tile + col;
-			gd[s] = (role & kRoleDs) && pt < np && dof < dofs ? (float)a.g[(int64_t)orig(pt) * a.ldg + dof] : 0.0f;
+			float v[8];
+#pragma unroll
+			for (int e = 0; e < 8; e++)
+			{
+				const int pt = 16 * s + 8 * kh + e, dof = 32 * tile + col;
+				v[e] = (role & kRoleDs) && pt < np && dof < dofs ? (float)a.g[(int64_t)orig(pt) * a.ldg + dof] : 0.0f;
+			}
+			split3_bf16(v, gd[s]);
 		}
 #pragma unroll
 		for (int h = 0; h < 2; h++)
 #pragma unroll
-			for (int s = 0; s < 16; s++)
+			for (int s = 0; s < 2; s++)
 			{
-				const int pt = 32 * h + col, dof = 32 * tile + 2 * s + kh;
-				gw[h][s] = (role & kRoleW) && pt < np && dof < dofs ? (float)a.g[(int64_t)orig(pt) * a.ldg + dof] : 0.0f;
+				float v[8];
+#pragma unroll
+				for (int e = 0; e < 8; e++)
+				{
+					const int pt = 32 * h + col, dof = 32 * tile + 16 * s + 8 * kh + e;
+					v[e] = (role & kRoleW) && pt < np && dof < dofs ? (float)a.g[(int64_t)orig(pt) * a.ldg + dof] : 0.0f;
+				}
+				split3_bf16(v, gw[h][s]);
 			}
 	}
 
@@ -519,10 +567,14 @@ struct AdjCons<HDDM_LB_F32>
 #pragma unroll
 		for (int r = 0; r < 16; r++) acc[r] = 0.0f;
 #pragma unroll
-		for (int s = 0; s < 32; s++)
+		for (int s = 0; s < 4; s++)
 		{
-			const float bv = on ? Bc[col * kARow + 2 * s + kh] : 0.0f;
-			acc = __builtin_amdgcn_mfma_f32_32x32x2f32(bv, gd[s], acc, 0, 0, 0);
+			float bv[8];
+#pragma unroll
+			for (int e = 0; e < 8; e++) bv[e] = on ? Bc[col * kARow + 16 * s + 8 * kh + e] : 0.0f;
+			b8_t bp[3];
+			split3_bf16(bv, bp);
+			acc = mfma_bf16x3(bp, gd[s], acc);
 		}
 #pragma unroll
 		for (int r = 0; r < 16; r++)
@@ -538,7 +590,9 @@ struct AdjCons<HDDM_LB_F32>
 
 	// The surplus rows of the w product, loaded by wrows() before the dS
 	// product so that they arrive meanwhile.
-	struct WRows { float v[16]; };
+	// S[node][dof 32 tile + 16 s + 8 (lane / 32) + e], e = 0..7: two 16-byte
+	// loads per step (rows of dofs_pad, a multiple of 32, floats)
+	struct WRows { f4v_t v[2][2]; };
 
 	__device__ __forceinline__ void wrows(const StateDesc* sd, int chunk, uint32_t mask, int tile, int DP, int lane,
 		WRows& sv)
@@ -546,15 +600,25 @@ struct AdjCons<HDDM_LB_F32>
 		const int col = lane & 31, kh = lane >> 5;
 		const float* s32r = ldc(&sd->s32r);
 		const bool on = (mask >> col) & 1u;
-		const float* srow = s32r + (size_t)(chunk * 32 + col) * DP + 32 * tile + kh;
+		const f4v_t* srow = (const f4v_t*)(s32r + (size_t)(chunk * 32 + col) * DP + 32 * tile + 8 * kh);
 #pragma unroll
-		for (int s = 0; s < 16; s++) sv.v[s] = on ? srow[2 * s] : 0.0f;
+		for (int s = 0; s < 2; s++)
+#pragma unroll
+			for (int q = 0; q < 2; q++) sv.v[s][q] = on ? srow[4 * s + q] : f4v_t{ 0.0f, 0.0f, 0.0f, 0.0f };
 	}
 
 	__device__ __forceinline__ void w(const WRows& sr, float* Wb, int lane)
 	{
 		const int col = lane & 31, kh = lane >> 5;
-		const float (&sv)[16] = sr.v;
+		b8_t sp[2][3];
+#pragma unroll
+		for (int s = 0; s < 2; s++)
+		{
+			float v[8];
+#pragma unroll
+			for (int e = 0; e < 8; e++) v[e] = sr.v[s][e >> 2][e & 3];
+			split3_bf16(v, sp[s]);
+		}
 		f16_t acc[2];
 #pragma unroll
 		for (int h = 0; h < 2; h++)
@@ -562,11 +626,10 @@ struct AdjCons<HDDM_LB_F32>
 #pragma unroll
 			for (int r = 0; r < 16; r++) acc[h][r] = 0.0f;
 #pragma unroll
-			for (int s = 0; s < 16; s++) acc[h] = __builtin_amdgcn_mfma_f32_32x32x2f32(gw[h][s], sv[s], acc[h], 0, 0, 0);
+			for (int s = 0; s < 2; s++) acc[h] = mfma_bf16x3(gw[h][s], sp[s], acc[h]);
 		}
 		// W[node][point]: node = lane % 32, point = 32 h + 8 (r / 4) + 4 (lane / 32) + r % 4;
 		// the four r of one r / 4 are four consecutive points, one store
-		typedef float f4v_t __attribute__((ext_vector_type(4)));
 #pragma unroll
 		for (int h = 0; h < 2; h++)
 #pragma unroll

@@ -172,9 +172,10 @@ int hddm_lb_forward_host(hddm_lb_ctx* ctx, int64_t npoints,
  * (HDDM_LB_OPT_ADJOINT_OWNED); elsewhere it is summed with FP64 atomics and
  * its last bits may vary between runs.
  * precision: HDDM_LB_FP64, or HDDM_LB_F32 (held to 1e-6 normwise: on the
- * wave-specialized kernel the products g.S and B^T g run in F32 on the MFMA
- * units with FP64 accumulation; the per-tile kernels compute in FP64 for
- * either precision). Stream-ordered. */
+ * wave-specialized kernel the products g.S and B^T g run on the bf16 MFMA
+ * units with each F32 operand split into three bf16 parts (F32 accuracy),
+ * F32 accumulation per 64-point chunk and FP64 accumulation beyond; the
+ * per-tile kernels compute in FP64 for either precision). Stream-ordered. */
 int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 	const double* x, int64_t ldx, const int32_t* state_ids,
 	const double* gbar, int64_t ldg, double* dx, int64_t lddx,
