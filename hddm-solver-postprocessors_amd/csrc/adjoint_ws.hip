@@ -873,7 +873,12 @@ __global__ void __launch_bounds__(64 * (kANC + 3)) adjoint_ws_kernel(AdjointArgs
 #pragma unroll
 			for (int c = 0; c < kANC; c++) v[c] = b[c * (int)(kWBuf / sizeof(T))];
 			if constexpr (decltype(all4)::value)
-				return (((double)v[0] + (double)v[1]) + (double)v[2]) + (double)v[3];
+			{
+				// F32 buffers are summed in F32 (three adds and one
+				// conversion instead of four conversions and three FP64 adds)
+				if constexpr (sizeof(T) == 4) return (double)(((v[0] + v[1]) + v[2]) + v[3]);
+				else return ((v[0] + v[1]) + v[2]) + v[3];
+			}
 			else
 			{
 				double w = 0.0;
