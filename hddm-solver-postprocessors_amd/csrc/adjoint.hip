@@ -613,11 +613,13 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 	// whatever the precision asked for, which meets the F32 bound too.
 	const bool wide = xbanks_for(ctx->dim) == 4;
 	// nodes of more than 8 factors next to four coordinate banks exceed the
-	// wave-specialized kernel's registers
-	const bool ws_fits = !(wide && ctx->fmax_all > 8);
+	// wave-specialized kernel's registers; its dx waves take each factor's
+	// centre -j / i from the factor table, exact when i is a power of two
+	const bool ws_fits = !(wide && ctx->fmax_all > 8) && ctx->centre_exact_all;
 	if (ctx->opt_form == 2 && !ws_fits)
 		return set_error(HDDM_LB_ERROR_UNSUPPORTED,
-			"adjoint: the wave-specialized kernel takes nodes of at most 8 factors when dim > 32");
+			"adjoint: the wave-specialized kernel takes nodes of at most 8 factors when dim > 32, "
+			"and index_i values that are powers of two");
 	const bool ws = ctx->opt_form == 2
 		|| (ctx->opt_form == 0 && ws_fits
 			&& (ctx->dt_needed >= 2 || (wide && precision == HDDM_LB_F32)));

@@ -314,7 +314,10 @@ struct DxNodes
 			for (int k = 0; k < 4; k++)
 			{
 				if (f0 + k >= NS) continue;
-				const double u = fma(xs[k], f64_from_high(ih[k]), f64_from_high(jh[k]));
+				// u = (x - j / i) i: the table holds the centre, so the two
+				// operations take their scalar operands directly (the walk's
+				// fma(x, i, -j) gives the same value: i is a power of two)
+				const double u = (xs[k] + f64_from_high(jh[k])) * f64_from_high(ih[k]);
 				phi[f0 + k] = fmax(1.0 - fabs(u), 0.0);
 				dp[f0 + k] = dphi(u, ih[k], phi[f0 + k]);
 			}
@@ -331,7 +334,9 @@ struct DxNodes
 #pragma unroll
 		for (int f = 0; f < NS; f++)
 		{
-			lds_add(DX + (d2[f] >> 1) * kWave + lane, pre * suf[f] * dp[f]);
+			// d2 is twice the dim (even: the table's first field), so row
+			// d2 / 2 starts at d2 * 32
+			lds_add(DX + d2[f] * (kWave / 2) + lane, pre * suf[f] * dp[f]);
 			pre *= phi[f];
 		}
 	}

@@ -255,6 +255,13 @@ extern "C" int hddm_lb_upload_state(hddm_lb_ctx* ctx, int istate, int dim, int n
 	std::unique_ptr<StateSlot> slot(new StateSlot);
 	int rc = HDDM_LB_SUCCESS;
 	StateDesc& d = slot->desc;
+	if (index_i && index_j && ld_index >= dim)
+		for (int64_t k = 0; k < nno && slot->centre_exact; k++)
+			for (int t = 0; t < dim; t++)
+			{
+				const int32_t i = index_i[k * ld_index + t], j = index_j[k * ld_index + t];
+				if ((i || j) && !(i > 0 && (i & (i - 1)) == 0)) { slot->centre_exact = false; break; }
+			}
 	std::memset(&d, 0, sizeof(d));
 	d.dim = dim; d.nno = nno; d.dofs = dofs;
 	d.nseg = 1;
@@ -476,6 +483,7 @@ int prepare_tables(hddm_lb_ctx* ctx)
 	std::vector<std::pair<double, int> > cost;
 	ctx->dt_needed = 1;
 	ctx->fmax_all = 1;
+	ctx->centre_exact_all = true;
 	ctx->max_nseg_dp = 0;
 	ctx->max_nseg = 1;
 	ctx->max_nseg_adj = 1;
@@ -529,6 +537,7 @@ int prepare_tables(hddm_lb_ctx* ctx)
 		ok[s] = 1;
 		ctx->dt_needed = std::max(ctx->dt_needed, d.dofs_pad / 32);
 		ctx->fmax_all = std::max(ctx->fmax_all, d.max_depth);
+		ctx->centre_exact_all = ctx->centre_exact_all && slot.centre_exact;
 		if (d.nseg > 1) ctx->max_nseg_dp = std::max<int64_t>(ctx->max_nseg_dp, (int64_t)d.nseg * d.dofs_pad);
 		ctx->max_nseg = std::max(ctx->max_nseg, d.nseg);
 		cost.push_back(std::make_pair(-(double)d.nops / d.nseg * d.dofs_pad, s));

@@ -39,6 +39,10 @@ struct StateSlot
 	int32_t* d_seg_op_adj = nullptr;
 	int64_t seg_len_adj = -1;
 	int nseg_adj = 1;
+	// every active factor has i a positive power of two (every sparse-grid
+	// node): the factor table's centre -j / i is then exact, which the
+	// wave-specialized adjoint's dx waves need
+	bool centre_exact = true;
 	~StateSlot();
 	void release();
 };
@@ -115,6 +119,7 @@ struct hddm_lb_ctx
 	bool tables_dirty = true;
 	int tables_n = 0, tables_cap = 0;
 	int dt_needed = 1, fmax_all = 1;
+	bool centre_exact_all = true;   // every state's StateSlot::centre_exact
 	int64_t max_nseg_dp = 0;
 	int max_nseg = 1;            // most node segments of a split state (1: none is split)
 	hddm_lb::StateDesc* d_states = nullptr;

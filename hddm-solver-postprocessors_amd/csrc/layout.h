@@ -66,8 +66,10 @@ struct StateDesc
 	const float* s32;               // [nchunks][dofs_pad][2][8] f32
 	const float* s32r;              // [nno * dofs_pad] f32 rows, trie order
 	// Factors of every node (trie order), fstride slots of 16 bytes each:
-	// {twice the dim, high word of i, high word of -j, 1} as in an op, then
-	// {0, 0, 0, 0} padding (phi == 1 at u = 0, no derivative).
+	// {twice the dim, high word of i, high word of -j / i (the centre
+	// negated), 1}, then {0, 0, 0, 0} padding (phi == 1 at u = 0, no
+	// derivative). u = (x - j / i) i equals the op's fma(x, i, -j) bit for
+	// bit: i is a power of two.
 	const uint32_t* ftab;           // [nno rounded up to 32][fstride][4]
 	int64_t fstride;
 	const double* iscale;           // [dofs_pad] 2^-sigma_c undoing the f16 scaling

@@ -81,6 +81,16 @@ bool f64_high_word(int64_t v, uint32_t& hi)
 	return (uint32_t)u == 0u;
 }
 
+// High word of -j / i, the node's centre negated (i a power of two: the
+// quotient is exact and has j's significand, so its low word is zero too).
+uint32_t f64_high_word_centre(int64_t i, int64_t j)
+{
+	const double c = -(double)j / (double)i;
+	uint64_t u;
+	std::memcpy(&u, &c, 8);
+	return (uint32_t)(u >> 32);
+}
+
 } // namespace
 
 std::vector<int32_t> segment_nodes(const HostLayout& L, int64_t seg_len)
@@ -238,7 +248,7 @@ int build_layout(int dim, int nno, int dofs,
 		{
 			e[0] = (uint32_t)cur[t].d << 1;
 			f64_high_word(cur[t].i, e[1]);
-			f64_high_word(-(int64_t)cur[t].j, e[2]);
+			e[2] = f64_high_word_centre(cur[t].i, cur[t].j);
 			e[3] = 1u;
 		}
 	}

@@ -343,7 +343,8 @@ __global__ void ftab_kernel(int nno, int fstride, const Factor* fact, const int3
 	{
 		e[0] = (uint32_t)a[t].d << 1;
 		e[1] = high_word(a[t].i);
-		e[2] = high_word(-(int64_t)a[t].j);
+		// the node's centre negated, -j / i (exact: i is a power of two)
+		e[2] = (uint32_t)((uint64_t)__double_as_longlong(-(double)a[t].j / (double)a[t].i) >> 32);
 		e[3] = 1u;
 	}
 }

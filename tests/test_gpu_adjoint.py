@@ -120,6 +120,38 @@ def test_adjoint_five_and_six_factor_nodes(torch_mod, start, maxl, maxf, prec):
     ctx.close()
 
 
+@pytest.mark.parametrize("prec", [hddm_lb.FP64, hddm_lb.F32])
+def test_adjoint_index_not_power_of_two(torch_mod, prec):
+    """index_i = 3 in dim 0 of some nodes (the engine takes any index whose
+    FP64 low word is zero; sparse grids have powers of two only). The
+    wave-specialized kernel's dx waves use the factor table's centres -j / i,
+    which are exact only for powers of two, so the automatic choice runs the
+    per-tile kernels (against the oracle), and forcing the wave-specialized
+    kernel is refused."""
+    ii, jj, lv, _ = hddm_lb.generate_grid(6, 3, 4, 0.0, 5, 0)
+    ii = ii.copy()
+    sel = ii[:, 0] == 4          # level-3 nodes in dim 0: i = 4, j = 1 or 3
+    assert sel.any()
+    ii[sel, 0] = 3
+    S = hddm_lb.synthetic_surplus(lv, 64, 6)
+    X = hddm_lb.synthetic_points(150, 6, 7, 0.01, 0.99)
+    G = np.random.default_rng(8).standard_normal((150, 64))
+    ctx = adj_ctx(0)
+    ctx.upload_state(0, ii, jj, S)
+    say("adjoint D=6 nno", ii.shape[0], "nodes with i = 3:", int(sel.sum()), "precision", prec)
+    dx, ds = run_adjoint(torch_mod, ctx, X, G, prec=prec)
+    assert ctx.last_adjoint_config()["kernels"] == 1
+    dxo, dso = oracle.adjoint(ii, jj, S, X, G, nthreads=8)
+    ds = ds.reshape(ii.shape[0], 64)
+    say("  dx err", normwise_rel_err(dx, dxo), "ds err", normwise_rel_err(ds, dso))
+    assert normwise_rel_err(dx, dxo) <= TOL_ADJ[prec][0]
+    assert normwise_rel_err(ds, dso) <= TOL_ADJ[prec][1]
+    ctx.set_option(hddm_lb.OPT_FORMULATION, 2)
+    with pytest.raises(hddm_lb.HddmLbError, match="powers of two"):
+        run_adjoint(torch_mod, ctx, X, G, prec=prec)
+    ctx.close()
+
+
 def test_adjoint_kernels_agree(torch_mod):
     """The wave-specialized FP64 adjoint and the per-tile FP64 kernels give
     the same dx and dS to rounding (four dof tiles, node segments on)."""

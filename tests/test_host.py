@@ -330,9 +330,9 @@ def flat_adjoint_dx(L, X, G, S_sorted):
     ft = L.ftab[:nno]
     d = (ft[:, :, 0] >> 1).astype(np.int64)
     iv = (ft[:, :, 1].astype(np.uint64) << np.uint64(32)).view(np.float64)
-    nj = (ft[:, :, 2].astype(np.uint64) << np.uint64(32)).view(np.float64)
+    nc = (ft[:, :, 2].astype(np.uint64) << np.uint64(32)).view(np.float64)   # -j / i
     on = ft[:, :, 3] != 0
-    u = X[:, d] * iv[None] + nj[None]                     # [points, nodes, slots]
+    u = (X[:, d] + nc[None]) * iv[None]                   # [points, nodes, slots]
     phi = np.maximum(1.0 - np.abs(u), 0.0)
     dp = np.where((phi > 0) & (u != 0), np.where(u > 0, -iv[None], iv[None]), 0.0)
     phi = np.where(on[None], phi, 1.0)
