@@ -8,8 +8,9 @@ Headline workload: high-dimensional multistate forward + adjoint.
 A step is one forward (y) and one adjoint (dx and dsurplus) over a batch of
 262,144 of those points (all four states; the workload's 2^20 points are four
 steps); one step over all 2^20 points is timed after the steps as a check
-that the rate does not depend on the batch. The mixed step runs the forward
-and the adjoint on FP32 MFMA (HDDM_LB_F32; the split f16 forward, HDDM_LB_F16X2,
+that the rate does not depend on the batch. The mixed step runs HDDM_LB_F32:
+the forward on FP32 MFMA, the adjoint's products on the bf16 MFMA with every
+F32 operand split into three bf16 parts (the split f16 forward, HDDM_LB_F16X2,
 misses 1e-6 on this workload and is reported beside it, not in the value);
 the plain FP64 step runs both in FP64 with the same work queue, trie walk and
 zero-skip.
@@ -458,11 +459,13 @@ def roofline_of(head, work, npoints):
     adj_flops = 4.0 * nnz * dofs
     fwd_flops = 2.0 * nnz * dofs
     # executed MFMA work of the adjoint kernel: per 32-node chunk and 32-dof
-    # tile (4 tiles), 32 v_mfma_f32_32x32x2f32 for dS and 32 for g.S, 4,096 FLOP each
-    exec_flops = float(work["adjoint_chunks32"]) * 4 * 64 * 4096
+    # tile (4 tiles), 24 v_mfma_f32_32x32x16_bf16 for dS (4 steps of 16 points
+    # x 6 terms of the three-part splits) and 24 for g.S (2 point halves x 2
+    # steps of 16 dofs x 6 terms), 32,768 FLOP each, on the bf16 units
+    exec_flops = float(work["adjoint_chunks32"]) * 4 * 48 * 32768
     dominant = "adjoint_ws_kernel (f32)" if adj_ms >= fwd_ms else "forward sweep (f32)"
     if adj_ms >= fwd_ms:
-        t, flops, peak = adj_ms, adj_flops, PEAK_TFLOPS["f32"]
+        t, flops, peak = adj_ms, adj_flops, PEAK_TFLOPS["bf16x3"]
     else:
         t, flops, peak = fwd_ms, fwd_flops, PEAK_TFLOPS["f32"]
     achieved = flops / (t * 1e-3) / 1e12
@@ -479,11 +482,13 @@ def roofline_of(head, work, npoints):
     return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
             "traffic": traffic, "kernel": dominant, "kernel_ms": t, "flops_per_launch": flops,
             "flops_basis": "non-zero (point, node) pairs: %d; per pair and dof 2 FLOP (forward), 4 (adjoint)" % int(nnz),
-            "frac_executed": exec_flops / (adj_ms * 1e-3) / 1e12 / PEAK_TFLOPS["f32"],
+            "frac_executed": exec_flops / (adj_ms * 1e-3) / 1e12 / PEAK_TFLOPS["bf16x3"],
             "flops_dense_per_launch": 4.0 * work["nno"] * dofs * npoints,
             "note": "frac: algorithmic FLOPs of the non-zero pairs only; frac_executed: the MFMA work the "
                     "adjoint kernel issues on its 32-node chunks (a 64-point block contracts every node "
-                    "that is non-zero at any of its points)"}
+                    "that is non-zero at any of its points). The F32 adjoint's products run on the bf16 "
+                    "MFMA units (peak 2,500 TFLOP/s dense) with each F32 operand split into three bf16 "
+                    "parts, six bf16 products per F32-accurate one (an F32-accurate peak of 417 TFLOP/s)"}
 
 
 def main():
@@ -550,7 +555,8 @@ def main():
                    "states": 4, "dim": 59, "nno_per_state": work["nno"], "dofs": 128, "points": n,
                    "points_total": 1 << 20,
                    "forward_precision": "f32 (FP32 MFMA, FP64 fold per chunk)",
-                   "adjoint_precision": "f32 (FP32 MFMA; w summed over tiles in FP64)",
+                   "adjoint_precision": "f32 (operands split into three bf16 parts, six-term products on the "
+                                        "bf16 MFMA; per-chunk F32 sums, FP64 beyond)",
                    "forward_config": head["mixed"]["forward_config"]},
         "roofline": roofline_of(head, work, n),
         "fp64_plain": {"value": n * 128 / (fp64_ms * 1e-3), "ms_per_step": fp64_ms, "steps": head["fp64"]["steps"],
