@@ -605,12 +605,13 @@ extern "C" int hddm_lb_adjoint(hddm_lb_ctx* ctx, int64_t npoints,
 	// automatic: the wave-specialized kernel, which walks each trie once for
 	// four dof tiles instead of once per tile, for states of two or more dof
 	// tiles and for wide states (D > 32) in F32; the per-tile kernels for one
-	// tile at D <= 32, where their edge-sum walk beats the dx waves (65,536
-	// points, F32: D=24 / 12k nodes / 128 dofs 8.7 ms against 38.7 per-tile,
-	// D=16 / 60k / 128 dofs 24.9 against 56.3, D=32 / 40k / 64 dofs 24.5
-	// against 49.1, D=20 / 100k / 64 dofs 43.3 against 65.9; D=20 / 100k /
-	// 32 dofs 42.7 against 31.3). The per-tile kernels compute in FP64
-	// whatever the precision asked for, which meets the F32 bound too.
+	// tile at D <= 32, where their edge-sum walk keeps up with the dx waves
+	// (65,536 points, F32: D=24 / 12k nodes / 128 dofs 6.3 ms against 37.8
+	// per-tile, D=16 / 60k / 128 dofs 20.6 against 57.5, D=32 / 40k / 64 dofs
+	// 20.4 against 49.2, D=20 / 100k / 64 dofs 37.0 against 71.3; one tile:
+	// D=20 / 100k / 32 dofs 36.7 against 35.8, D=16 / 60k / 32 dofs 18.7
+	// against 14.4). The per-tile kernels compute in FP64 whatever the
+	// precision asked for, which meets the F32 bound too.
 	const bool wide = xbanks_for(ctx->dim) == 4;
 	// nodes of more than 8 factors next to four coordinate banks exceed the
 	// wave-specialized kernel's registers; its dx waves take each factor's

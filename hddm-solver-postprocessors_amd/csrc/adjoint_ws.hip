@@ -132,45 +132,43 @@ __device__ __forceinline__ bool a_wait(const int* p, int target, ASync* sy, uint
 
 // x[dim] of the lane's point, read from the coordinate banks pinned in
 // v64.. (where the assembly walk keeps them) with GPR indexing; idx2 is
-// twice the dim (the op's dim field). m0 is saved and restored.
+// twice the dim (the op's dim field). The indexing mode writes m0, declared
+// clobbered (the compiler keeps nothing live in it across these reads).
 template <int XB>
 __device__ __forceinline__ double xpin(const d16_t& xv0, const d16_t& xv1, const d16_t& xv2, const d16_t& xv3,
 	uint32_t idx2)
 {
 	double v;
-	uint32_t sv;
-#define HDDM_XPIN "s_mov_b32 %[sv], m0\n" "s_set_gpr_idx_on %[i], gpr_idx(SRC0)\n" "v_mov_b64 %[v], v[64:65]\n" \
-	"s_set_gpr_idx_off\n" "s_mov_b32 m0, %[sv]\n"
+#define HDDM_XPIN "s_set_gpr_idx_on %[i], gpr_idx(SRC0)\n" "v_mov_b64 %[v], v[64:65]\n" \
+	"s_set_gpr_idx_off\n"
 	if constexpr (XB == 1)
-		asm volatile(HDDM_XPIN : [v] "=&v"(v), [sv] "=&s"(sv) : [i] "s"(idx2), "{v[64:95]}"(xv0));
+		asm volatile(HDDM_XPIN : [v] "=&v"(v) : [i] "s"(idx2), "{v[64:95]}"(xv0) : "m0");
 	else if constexpr (XB == 2)
-		asm volatile(HDDM_XPIN : [v] "=&v"(v), [sv] "=&s"(sv) : [i] "s"(idx2), "{v[64:95]}"(xv0), "{v[96:127]}"(xv1));
+		asm volatile(HDDM_XPIN : [v] "=&v"(v) : [i] "s"(idx2), "{v[64:95]}"(xv0), "{v[96:127]}"(xv1) : "m0");
 	else
-		asm volatile(HDDM_XPIN : [v] "=&v"(v), [sv] "=&s"(sv) : [i] "s"(idx2), "{v[64:95]}"(xv0), "{v[96:127]}"(xv1),
-			"{v[128:159]}"(xv2), "{v[160:191]}"(xv3));
+		asm volatile(HDDM_XPIN : [v] "=&v"(v) : [i] "s"(idx2), "{v[64:95]}"(xv0), "{v[96:127]}"(xv1),
+			"{v[128:159]}"(xv2), "{v[160:191]}"(xv3) : "m0");
 #undef HDDM_XPIN
 	(void)xv1; (void)xv2; (void)xv3;
 	return v;
 }
 
-// Four coordinates at once (twice the dims in i0..i3), m0 saved and
-// restored once.
+// Four coordinates at once (twice the dims in i0..i3).
 template <int XB>
 __device__ __forceinline__ void xpin4(const d16_t& xv0, const d16_t& xv1, const d16_t& xv2, const d16_t& xv3,
 	uint32_t i0, uint32_t i1, uint32_t i2, uint32_t i3, double& x0, double& x1, double& x2, double& x3)
 {
-	uint32_t sv;
 #define HDDM_XP1(k) "s_set_gpr_idx_on %[i" #k "], gpr_idx(SRC0)\n" "v_mov_b64 %[v" #k "], v[64:65]\n" "s_set_gpr_idx_off\n"
-#define HDDM_XPIN4 "s_mov_b32 %[sv], m0\n" HDDM_XP1(0) HDDM_XP1(1) HDDM_XP1(2) HDDM_XP1(3) "s_mov_b32 m0, %[sv]\n"
-#define HDDM_XPOUT [v0] "=&v"(x0), [v1] "=&v"(x1), [v2] "=&v"(x2), [v3] "=&v"(x3), [sv] "=&s"(sv)
+#define HDDM_XPIN4 HDDM_XP1(0) HDDM_XP1(1) HDDM_XP1(2) HDDM_XP1(3)
+#define HDDM_XPOUT [v0] "=&v"(x0), [v1] "=&v"(x1), [v2] "=&v"(x2), [v3] "=&v"(x3)
 #define HDDM_XPIDX [i0] "s"(i0), [i1] "s"(i1), [i2] "s"(i2), [i3] "s"(i3)
 	if constexpr (XB == 1)
-		asm volatile(HDDM_XPIN4 : HDDM_XPOUT : HDDM_XPIDX, "{v[64:95]}"(xv0));
+		asm volatile(HDDM_XPIN4 : HDDM_XPOUT : HDDM_XPIDX, "{v[64:95]}"(xv0) : "m0");
 	else if constexpr (XB == 2)
-		asm volatile(HDDM_XPIN4 : HDDM_XPOUT : HDDM_XPIDX, "{v[64:95]}"(xv0), "{v[96:127]}"(xv1));
+		asm volatile(HDDM_XPIN4 : HDDM_XPOUT : HDDM_XPIDX, "{v[64:95]}"(xv0), "{v[96:127]}"(xv1) : "m0");
 	else
 		asm volatile(HDDM_XPIN4 : HDDM_XPOUT : HDDM_XPIDX, "{v[64:95]}"(xv0), "{v[96:127]}"(xv1),
-			"{v[128:159]}"(xv2), "{v[160:191]}"(xv3));
+			"{v[128:159]}"(xv2), "{v[160:191]}"(xv3) : "m0");
 #undef HDDM_XPIDX
 #undef HDDM_XPOUT
 #undef HDDM_XPIN4
@@ -183,18 +181,17 @@ template <int XB>
 __device__ __forceinline__ void xpin2(const d16_t& xv0, const d16_t& xv1, const d16_t& xv2, const d16_t& xv3,
 	uint32_t i0, uint32_t i1, double& x0, double& x1)
 {
-	uint32_t sv;
 #define HDDM_XP1(k) "s_set_gpr_idx_on %[i" #k "], gpr_idx(SRC0)\n" "v_mov_b64 %[v" #k "], v[64:65]\n" "s_set_gpr_idx_off\n"
-#define HDDM_XPIN2 "s_mov_b32 %[sv], m0\n" HDDM_XP1(0) HDDM_XP1(1) "s_mov_b32 m0, %[sv]\n"
-#define HDDM_XPOUT [v0] "=&v"(x0), [v1] "=&v"(x1), [sv] "=&s"(sv)
+#define HDDM_XPIN2 HDDM_XP1(0) HDDM_XP1(1)
+#define HDDM_XPOUT [v0] "=&v"(x0), [v1] "=&v"(x1)
 #define HDDM_XPIDX [i0] "s"(i0), [i1] "s"(i1)
 	if constexpr (XB == 1)
-		asm volatile(HDDM_XPIN2 : HDDM_XPOUT : HDDM_XPIDX, "{v[64:95]}"(xv0));
+		asm volatile(HDDM_XPIN2 : HDDM_XPOUT : HDDM_XPIDX, "{v[64:95]}"(xv0) : "m0");
 	else if constexpr (XB == 2)
-		asm volatile(HDDM_XPIN2 : HDDM_XPOUT : HDDM_XPIDX, "{v[64:95]}"(xv0), "{v[96:127]}"(xv1));
+		asm volatile(HDDM_XPIN2 : HDDM_XPOUT : HDDM_XPIDX, "{v[64:95]}"(xv0), "{v[96:127]}"(xv1) : "m0");
 	else
 		asm volatile(HDDM_XPIN2 : HDDM_XPOUT : HDDM_XPIDX, "{v[64:95]}"(xv0), "{v[96:127]}"(xv1),
-			"{v[128:159]}"(xv2), "{v[160:191]}"(xv3));
+			"{v[128:159]}"(xv2), "{v[160:191]}"(xv3) : "m0");
 #undef HDDM_XPIDX
 #undef HDDM_XPOUT
 #undef HDDM_XPIN2
@@ -207,18 +204,17 @@ template <int XB>
 __device__ __forceinline__ void xpin3(const d16_t& xv0, const d16_t& xv1, const d16_t& xv2, const d16_t& xv3,
 	uint32_t i0, uint32_t i1, uint32_t i2, double& x0, double& x1, double& x2)
 {
-	uint32_t sv;
 #define HDDM_XP1(k) "s_set_gpr_idx_on %[i" #k "], gpr_idx(SRC0)\n" "v_mov_b64 %[v" #k "], v[64:65]\n" "s_set_gpr_idx_off\n"
-#define HDDM_XPIN3 "s_mov_b32 %[sv], m0\n" HDDM_XP1(0) HDDM_XP1(1) HDDM_XP1(2) "s_mov_b32 m0, %[sv]\n"
-#define HDDM_XPOUT [v0] "=&v"(x0), [v1] "=&v"(x1), [v2] "=&v"(x2), [sv] "=&s"(sv)
+#define HDDM_XPIN3 HDDM_XP1(0) HDDM_XP1(1) HDDM_XP1(2)
+#define HDDM_XPOUT [v0] "=&v"(x0), [v1] "=&v"(x1), [v2] "=&v"(x2)
 #define HDDM_XPIDX [i0] "s"(i0), [i1] "s"(i1), [i2] "s"(i2)
 	if constexpr (XB == 1)
-		asm volatile(HDDM_XPIN3 : HDDM_XPOUT : HDDM_XPIDX, "{v[64:95]}"(xv0));
+		asm volatile(HDDM_XPIN3 : HDDM_XPOUT : HDDM_XPIDX, "{v[64:95]}"(xv0) : "m0");
 	else if constexpr (XB == 2)
-		asm volatile(HDDM_XPIN3 : HDDM_XPOUT : HDDM_XPIDX, "{v[64:95]}"(xv0), "{v[96:127]}"(xv1));
+		asm volatile(HDDM_XPIN3 : HDDM_XPOUT : HDDM_XPIDX, "{v[64:95]}"(xv0), "{v[96:127]}"(xv1) : "m0");
 	else
 		asm volatile(HDDM_XPIN3 : HDDM_XPOUT : HDDM_XPIDX, "{v[64:95]}"(xv0), "{v[96:127]}"(xv1),
-			"{v[128:159]}"(xv2), "{v[160:191]}"(xv3));
+			"{v[128:159]}"(xv2), "{v[160:191]}"(xv3) : "m0");
 #undef HDDM_XPIDX
 #undef HDDM_XPOUT
 #undef HDDM_XPIN3
