@@ -861,18 +861,18 @@ __global__ void __launch_bounds__(64 * (kANC + 3)) adjoint_ws_kernel(AdjointArgs
 		d16_t xv0, xv1, xv2, xv3;
 		// consumers whose buffers hold w for the current item (bit c)
 		uint32_t wcons = 0;
-		// w of node nl: the tiles' buffers summed in consumer order. Every
-		// buffer is read (one wait for all four reads) and an idle
-		// consumer's value selected away, which adds an exact zero.
-		// With every consumer holding w (all4, the common case of states of
-		// four or more tiles) the selects are left out; the order of the
-		// additions is the same.
+		// w of node nl: the tiles' buffers summed in consumer order. rd()
+		// issues the four reads; wsum() adds them (an idle consumer's value
+		// selected away, which adds an exact zero). With every consumer
+		// holding w (all4, the common case of states of four or more tiles)
+		// the selects are left out; the order of the additions is the same.
 		static_assert(kANC == 4, "the all-consumer sum is written for four consumers");
-		auto wsum = [&](int par, int nl, auto all4) -> double {
+		auto rd = [&](int par, int nl, T (&v)[kANC]) {
 			const T* b = wbuf(par, 0) + wpos<T>(nl, lane);
-			T v[kANC];
 #pragma unroll
 			for (int c = 0; c < kANC; c++) v[c] = b[c * (int)(kWBuf / sizeof(T))];
+		};
+		auto wsum = [&](const T (&v)[kANC], auto all4) -> double {
 			if constexpr (decltype(all4)::value)
 			{
 				// F32 buffers are summed in F32 (three adds and one
@@ -890,8 +890,10 @@ __global__ void __launch_bounds__(64 * (kANC + 3)) adjoint_ws_kernel(AdjointArgs
 		};
 		// One chunk (message n): its factor rows load while the consumers
 		// finish it; then its nodes in order, two at a time (a lone last node
-		// is paired with itself at w = 0, which adds zeros), the next pair's
-		// w read first. False when a wait gave up.
+		// is paired with itself at w = 0, which adds zeros). The next pair's
+		// w values are read before a pair's terms and summed only after them,
+		// so the reads complete behind the FP64 work. False when a wait gave
+		// up.
 		auto run_chunk = [&](auto&& dn, int chunk, uint32_t mask, int par, int n, auto all4) -> bool {
 			dn.load(ftab, chunk, lane);
 			const uint64_t t1 = clk();
@@ -900,29 +902,37 @@ __global__ void __launch_bounds__(64 * (kANC + 3)) adjoint_ws_kernel(AdjointArgs
 				if (!a_wait(&sy->done[c], n + 1, sy, a.status)) return false;
 			const uint64_t t2 = clk();
 			cyc_wait += t2 - t1;
-			auto take = [&](int& n0, int& n1, double& w0, double& w1) {
+			// the next pair of surviving nodes (a lone one paired with itself)
+			// and the reads of their w
+			auto take = [&](int& n0, int& n1, bool& two, T (&r0)[kANC], T (&r1)[kANC]) {
 				n0 = __builtin_ctz(mask);
 				mask &= mask - 1u;
-				const bool two = mask != 0u;
+				two = mask != 0u;
 				n1 = two ? __builtin_ctz(mask) : n0;
 				mask &= mask - 1u;
-				w0 = wsum(par, n0, all4);
-				w1 = two ? wsum(par, n1, all4) : 0.0;
+				rd(par, n0, r0);
+				rd(par, n1, r1);
 			};
 			if (!mask) return true;
 			int n0, n1;
-			double w0, w1;
-			take(n0, n1, w0, w1);
+			bool two;
+			T r0[kANC], r1[kANC];
+			take(n0, n1, two, r0, r1);
+			double w0 = wsum(r0, all4), w1 = two ? wsum(r1, all4) : 0.0;
 			for (;;)
 			{
 				const bool more = mask != 0u;
 				int m0 = n0, m1 = n1;
-				double v0 = 0.0, v1 = 0.0;
-				if (more) take(m0, m1, v0, v1);
+				bool two2 = false;
+				if (more) take(m0, m1, two2, r0, r1);
 				dn.node(n0, w0, DX, lane, xv0, xv1, xv2, xv3);
 				dn.node(n1, w1, DX, lane, xv0, xv1, xv2, xv3);
 				if (!more) break;
-				n0 = m0; n1 = m1; w0 = v0; w1 = v1;
+				// summed after the pair's terms (in the loop body, so the
+				// wait counts only the pair's LDS adds issued since the reads)
+				n0 = m0; n1 = m1; two = two2;
+				w0 = wsum(r0, all4);
+				w1 = two ? wsum(r1, all4) : 0.0;
 			}
 			cyc_dx += clk() - t2;
 			return true;
