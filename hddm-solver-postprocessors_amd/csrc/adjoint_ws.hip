@@ -153,13 +153,15 @@ __device__ __forceinline__ double xpin(const d16_t& xv0, const d16_t& xv1, const
 	return v;
 }
 
-// Four coordinates at once (twice the dims in i0..i3).
+// Four coordinates at once (twice the dims in i0..i3), in one indexing run:
+// s_set_gpr_idx_idx changes only the index between the moves.
 template <int XB>
 __device__ __forceinline__ void xpin4(const d16_t& xv0, const d16_t& xv1, const d16_t& xv2, const d16_t& xv3,
 	uint32_t i0, uint32_t i1, uint32_t i2, uint32_t i3, double& x0, double& x1, double& x2, double& x3)
 {
-#define HDDM_XP1(k) "s_set_gpr_idx_on %[i" #k "], gpr_idx(SRC0)\n" "v_mov_b64 %[v" #k "], v[64:65]\n" "s_set_gpr_idx_off\n"
-#define HDDM_XPIN4 HDDM_XP1(0) HDDM_XP1(1) HDDM_XP1(2) HDDM_XP1(3)
+#define HDDM_XPI0 "s_set_gpr_idx_on %[i0], gpr_idx(SRC0)\n" "v_mov_b64 %[v0], v[64:65]\n"
+#define HDDM_XPN(k) "s_set_gpr_idx_idx %[i" #k "]\n" "v_mov_b64 %[v" #k "], v[64:65]\n"
+#define HDDM_XPIN4 HDDM_XPI0 HDDM_XPN(1) HDDM_XPN(2) HDDM_XPN(3) "s_set_gpr_idx_off\n"
 #define HDDM_XPOUT [v0] "=&v"(x0), [v1] "=&v"(x1), [v2] "=&v"(x2), [v3] "=&v"(x3)
 #define HDDM_XPIDX [i0] "s"(i0), [i1] "s"(i1), [i2] "s"(i2), [i3] "s"(i3)
 	if constexpr (XB == 1)
@@ -172,7 +174,8 @@ __device__ __forceinline__ void xpin4(const d16_t& xv0, const d16_t& xv1, const 
 #undef HDDM_XPIDX
 #undef HDDM_XPOUT
 #undef HDDM_XPIN4
-#undef HDDM_XP1
+#undef HDDM_XPI0
+#undef HDDM_XPN
 	(void)xv1; (void)xv2; (void)xv3;
 }
 
@@ -181,8 +184,9 @@ template <int XB>
 __device__ __forceinline__ void xpin2(const d16_t& xv0, const d16_t& xv1, const d16_t& xv2, const d16_t& xv3,
 	uint32_t i0, uint32_t i1, double& x0, double& x1)
 {
-#define HDDM_XP1(k) "s_set_gpr_idx_on %[i" #k "], gpr_idx(SRC0)\n" "v_mov_b64 %[v" #k "], v[64:65]\n" "s_set_gpr_idx_off\n"
-#define HDDM_XPIN2 HDDM_XP1(0) HDDM_XP1(1)
+#define HDDM_XPI0 "s_set_gpr_idx_on %[i0], gpr_idx(SRC0)\n" "v_mov_b64 %[v0], v[64:65]\n"
+#define HDDM_XPN(k) "s_set_gpr_idx_idx %[i" #k "]\n" "v_mov_b64 %[v" #k "], v[64:65]\n"
+#define HDDM_XPIN2 HDDM_XPI0 HDDM_XPN(1) "s_set_gpr_idx_off\n"
 #define HDDM_XPOUT [v0] "=&v"(x0), [v1] "=&v"(x1)
 #define HDDM_XPIDX [i0] "s"(i0), [i1] "s"(i1)
 	if constexpr (XB == 1)
@@ -195,7 +199,8 @@ __device__ __forceinline__ void xpin2(const d16_t& xv0, const d16_t& xv1, const 
 #undef HDDM_XPIDX
 #undef HDDM_XPOUT
 #undef HDDM_XPIN2
-#undef HDDM_XP1
+#undef HDDM_XPI0
+#undef HDDM_XPN
 	(void)xv1; (void)xv2; (void)xv3;
 }
 
@@ -204,8 +209,9 @@ template <int XB>
 __device__ __forceinline__ void xpin3(const d16_t& xv0, const d16_t& xv1, const d16_t& xv2, const d16_t& xv3,
 	uint32_t i0, uint32_t i1, uint32_t i2, double& x0, double& x1, double& x2)
 {
-#define HDDM_XP1(k) "s_set_gpr_idx_on %[i" #k "], gpr_idx(SRC0)\n" "v_mov_b64 %[v" #k "], v[64:65]\n" "s_set_gpr_idx_off\n"
-#define HDDM_XPIN3 HDDM_XP1(0) HDDM_XP1(1) HDDM_XP1(2)
+#define HDDM_XPI0 "s_set_gpr_idx_on %[i0], gpr_idx(SRC0)\n" "v_mov_b64 %[v0], v[64:65]\n"
+#define HDDM_XPN(k) "s_set_gpr_idx_idx %[i" #k "]\n" "v_mov_b64 %[v" #k "], v[64:65]\n"
+#define HDDM_XPIN3 HDDM_XPI0 HDDM_XPN(1) HDDM_XPN(2) "s_set_gpr_idx_off\n"
 #define HDDM_XPOUT [v0] "=&v"(x0), [v1] "=&v"(x1), [v2] "=&v"(x2)
 #define HDDM_XPIDX [i0] "s"(i0), [i1] "s"(i1), [i2] "s"(i2)
 	if constexpr (XB == 1)
@@ -218,7 +224,8 @@ __device__ __forceinline__ void xpin3(const d16_t& xv0, const d16_t& xv1, const 
 #undef HDDM_XPIDX
 #undef HDDM_XPOUT
 #undef HDDM_XPIN3
-#undef HDDM_XP1
+#undef HDDM_XPI0
+#undef HDDM_XPN
 	(void)xv1; (void)xv2; (void)xv3;
 }
 
