@@ -25,8 +25,9 @@
 // A dx wave takes each chunk of its parity (counted from the item's start)
 // once every consumer is done with it. For each surviving node k of the
 // chunk it sums the tiles' w in consumer order, reads k's factors phi_f (dim
-// d_f) from the per-node factor table, recomputes them at the block's points
-// with the walk's own FP64 operations, and adds
+// d_f, i_f and the centre j_f / i_f) from the per-node factor table,
+// recomputes them at the block's points in FP64 from u = (x - j/i) i (the
+// walk's fma(x, i, -j) bit for bit: i is a power of two), and adds
 //   w_pk (prod_{g != f} phi_g) phi_f'       (prefix and suffix products)
 // to dx[d_f] in LDS rows of its own: no division, and a factor that is zero
 // contributes exactly zero. Each wave adds in a fixed order and the two
@@ -35,7 +36,8 @@
 // FP64: v_mfma_f64_16x16x4f64 throughout. F32: v_mfma_f32_32x32x16_bf16 on
 // operands split into three bf16 parts (six terms per product, as accurate as
 // F32 operands and products); dS partials of one chunk (64 points) are added
-// in FP64, w is summed over tiles in FP64.
+// in FP64; w is summed over tiles in FP64, or in F32 for F32 buffers when all
+// four consumers hold w.
 // Synchronization is the forward WS sweep's: an LDS message ring with
 // release/acquire counters and bounded waits (sweep_ws.hip). The walker
 // reuses a ring slot once all six reader waves are done with it; a consumer
