@@ -67,7 +67,10 @@ typedef struct hddm_lb_grid hddm_lb_grid;
 #define HDDM_LB_OPT_WALK          9  /* op stream source: 0 automatic, 1 scalar loads per wave, 2 LDS tiles per workgroup */
 #define HDDM_LB_OPT_FORMULATION  10  /* forward and adjoint: 0 automatic, 1 each wave walks and contracts
                                         its block (per-tile kernels), 2 a walker wave feeds MFMA consumer
-                                        waves (wave-specialized kernels) */
+                                        waves (wave-specialized kernels). The wave-specialized adjoint
+                                        takes states whose index_i values are powers of two (every sparse
+                                        grid) and, when dim > 32, nodes of at most 8 factors; 0 runs
+                                        other states on the per-tile kernels, 2 fails for them */
 #define HDDM_LB_OPT_COALESCE     11  /* 1 (default): hddm_lb_forward_host calls of at most 64 points made
                                         at the same time from several threads run as one batched forward */
 #define HDDM_LB_OPT_DEVICE_LAYOUT 12 /* 1 (default): hddm_lb_upload_state builds the node layout (trie
