@@ -476,8 +476,9 @@ struct AdjCons<HDDM_LB_FP64>
 // as x = x0 + x1 + x2, bf16 parts each rounded to nearest, so that
 // |x - x0 - x1 - x2| <= 2^-24 |x|; a product accumulates (in F32) the six
 // terms xa_p xb_q with p + q <= 2, smallest first, and the three left out are
-// below 2^-23 of it. The bf16 MFMA issues 8x the F32 one's flops per cycle,
-// so the six terms take 192 cycles per 16 steps of k where F32 takes 512.
+// below 2^-23 of it. The bf16 MFMA does 16x the F32 one's flops per cycle
+// (32,768 in 32 cycles against 4,096 in 64), so the six terms take 192
+// cycles per 16 steps of k where F32 takes 512.
 // A[i][k] from element k % 8 of lane i + 32 (k / 8), B[k][j] from element
 // k % 8 of lane j + 32 (k / 8), D[i][j] in register 4 (i / 8) + i % 4 of lane
 // j + 32 ((i / 4) % 2).
