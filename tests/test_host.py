@@ -369,6 +369,35 @@ def test_factor_table_dx_matches_oracle(dim, start, level, target, seed):
     np.testing.assert_allclose(dx, dxo, rtol=0, atol=1e-11 * np.abs(dxo).max())
 
 
+def test_factor_table_fields():
+    """Each node's slots (trie order) hold its active factors in ascending
+    dim: {2 dim, high word of i, high word of -j / i (the centre), 1}, then
+    zero slots; the low words of i and -j / i are zero, and (x - j / i) i
+    equals x i - j (one rounding: x i is exact for i a power of two) bit for
+    bit, the value the walk forms with fma(x, i, -j)."""
+    ii, jj, lv, _ = hddm_lb.generate_grid(5, 2, 6, 0.4, 21, 600)
+    S = hddm_lb.synthetic_surplus(lv, 2, 21)
+    L = hddm_lb.Layout(ii, jj, S)
+    ft = L.ftab
+    hi = lambda v: np.uint32(np.float64(v).view(np.uint64) >> np.uint64(32))
+    lo = lambda v: np.uint32(np.float64(v).view(np.uint64) & np.uint64(0xffffffff))
+    x = np.random.default_rng(22).random(4096)
+    for n in range(L.nno):
+        k = L.order[n]
+        dims = [d for d in range(5) if ii[k, d] != 0 or jj[k, d] != 0]
+        for t, d in enumerate(dims):
+            i, j = int(ii[k, d]), int(jj[k, d])
+            c = -float(j) / i                     # -0.0 for j = 0, as the builders compute it
+            assert lo(i) == 0 and lo(c) == 0
+            assert tuple(ft[n, t]) == (2 * d, hi(i), hi(c), 1)
+        assert not ft[n, len(dims):].any()
+    for i in (2, 4, 8, 16, 32):
+        for j in range(0, i + 1):
+            u_centre = (x + np.float64(-float(j) / i)) * i
+            u_fma = x * i - j                     # x * i exact: one rounding
+            assert np.array_equal(u_centre.view(np.uint64), u_fma.view(np.uint64))
+
+
 def test_layout_rejects_bad_input():
     with pytest.raises(hddm_lb.HddmLbError) as e:
         hddm_lb.Layout(np.zeros((0, 3), np.int32), np.zeros((0, 3), np.int32), np.zeros((0, 2)))
